@@ -1,0 +1,57 @@
+/* Stage-level C ABI of the gfx950 Waltz library (no reference counterpart:
+ * the reference links its stages inside one C++ library, src/waltz.cu; these
+ * entry points expose the same stages individually so that callers and parity
+ * tests can drive one stage on device buffers). Device pointers, hipStream_t
+ * passed as void*, synchronous; 0 on success, -1 on failure with the message
+ * in waltz_amd_last_error() (thread-local).
+ */
+#ifndef WALTZ_AMD_H
+#define WALTZ_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* waltz_amd_last_error(void);
+/* Library build identity: "gfx950 ..." */
+const char* waltz_amd_version(void);
+
+/* Whole codec on `stream` (cpu/include/waltz_cpu.h:24-29 semantics with device
+ * buffers). width = 4 or 8. */
+size_t waltz_amd_max_compressed_bytes(uint32_t nx, uint32_t ny, uint32_t nz, int width);
+int waltz_amd_compress(int width, const void* input, uint32_t nx, uint32_t ny, uint32_t nz, double abs_bound,
+                       void* archive, size_t capacity, size_t* written, void* stream);
+int waltz_amd_decompress(int width, const void* archive, size_t bytes, uint32_t nx, uint32_t ny, uint32_t nz,
+                         void* output, void* stream);
+
+/* CDF 9/7 lifting of an nx*ny*nz volume (cpu/src/transform.cpp:409-456):
+ * forward reads `input`, writes coefficients to `output`; inverse reads
+ * coefficients from `input`, writes samples to `output`. input may equal
+ * output. Needs two scratch volumes tmp0/tmp1 (may be NULL: allocated). */
+int waltz_amd_transform(int width, int inverse, const void* input, void* output, uint32_t nx, uint32_t ny,
+                        uint32_t nz, unsigned levels_xy, unsigned levels_z, int dyadic, void* stream);
+
+/* Quantization + Morton packing (cpu/src/codec.cpp:198-234): coeff is
+ * replaced by dequantized values; mag[n], sign[ceil(n/32)] packed order;
+ * q-outliers sorted by index into qidx/qval (capacity n), count in *nq. */
+int waltz_amd_quantize(int width, void* coeff, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bz, double quantum,
+                       void* mag, void* sign, void* qidx, void* qval, uint32_t* nq, void* stream);
+
+/* WZP stream (cpu/src/wzp_cpu.cpp:780-949): encode mag/sign of n elements
+ * into out (capacity bytes), *written = stream bytes; decode back. */
+int waltz_amd_wzp_encode(const void* mag, const void* sign, uint32_t n, void* out, size_t capacity, size_t* written,
+                         void* stream);
+int waltz_amd_wzp_decode(const void* in, size_t bytes, uint32_t n, void* mag, void* sign, void* stream);
+
+/* Per-stage timing with HIP events on the pipeline's stream (off by default).
+ * enable returns the previous state; report writes a JSON object
+ * {"stage": [total_ms, launches], ...} accumulated since the last report and
+ * resets it; it returns the full length (write is truncated to cap-1). */
+int waltz_amd_profile_enable(int on);
+size_t waltz_amd_profile_report(char* buf, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

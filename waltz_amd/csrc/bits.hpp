@@ -1,0 +1,102 @@
+// Integer bit helpers shared by the WZP kernels. Host+device so the CPU test
+// harness (tests/test_bits_host.cpp) checks them against brute force.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define WZB_HD __host__ __device__ __forceinline__
+#else
+#define WZB_HD inline
+#endif
+
+namespace wz {
+
+WZB_HD uint32_t popc32(uint32_t v) { return uint32_t(__builtin_popcount(v)); }
+WZB_HD uint32_t popc64(uint64_t v) { return uint32_t(__builtin_popcountll(v)); }
+#if defined(__clang__)
+WZB_HD uint32_t bitrev16(uint32_t v) { return __builtin_bitreverse32(v) >> 16; }
+#else
+WZB_HD uint32_t bitrev16(uint32_t v) {
+    uint32_t r = 0;
+    for (int i = 0; i < 16; ++i) r |= ((v >> i) & 1u) << (15 - i);
+    return r;
+}
+#endif
+
+// In-place transpose of a 16x16 bit matrix held as 16 rows of 16 bits:
+// afterwards row c bit r holds what row r bit c held (Hacker's Delight
+// block-swap, four rounds of masked exchanges).
+WZB_HD void transpose16(uint32_t (&a)[16]) {
+#define WZ_SWAP(J, M)                                                   \
+    for (int k = 0; k < 16; ++k)                                        \
+        if (!(k & (J))) {                                               \
+            const uint32_t t = ((a[k] >> (J)) ^ a[k + (J)]) & (M);      \
+            a[k + (J)] ^= t;                                            \
+            a[k] ^= t << (J);                                           \
+        }
+    WZ_SWAP(8, 0x00FFu)
+    WZ_SWAP(4, 0x0F0Fu)
+    WZ_SWAP(2, 0x3333u)
+    WZ_SWAP(1, 0x5555u)
+#undef WZ_SWAP
+}
+
+// BIT_2 of one 16-magnitude block: plane p bit (15-j) = magnitude j bit (15-p)
+// (the LC BIT_2 orientation of cpu/src/wzp_cpu.cpp:146-193).
+// In: m[j] magnitudes. Out: m[p] planes.
+WZB_HD void bit2_forward(uint32_t (&m)[16]) {
+    uint32_t a[16];
+    for (int q = 0; q < 16; ++q) a[q] = bitrev16(m[15 - q]);
+    transpose16(a);
+    for (int p = 0; p < 16; ++p) m[p] = a[p];
+}
+// Inverse of bit2_forward. In: planes. Out: magnitudes.
+WZB_HD void bit2_inverse(uint32_t (&m)[16]) {
+    uint32_t a[16];
+    for (int p = 0; p < 16; ++p) a[p] = m[p];
+    transpose16(a);
+    for (int q = 0; q < 16; ++q) m[15 - q] = bitrev16(a[q]);
+}
+
+// Gather the bits of v selected by mask into the low end, in order (x86 pext).
+WZB_HD uint32_t compress_bits(uint32_t v, uint32_t mask) {
+    uint32_t out = 0, k = 0;
+    while (mask) {
+        const uint32_t low = mask & (0u - mask);
+        if (v & low) out |= 1u << k;
+        ++k;
+        mask ^= low;
+    }
+    return out;
+}
+// Scatter the low bits of v to the positions selected by mask (x86 pdep).
+WZB_HD uint32_t expand_bits(uint32_t v, uint32_t mask) {
+    uint32_t out = 0, k = 0;
+    while (mask) {
+        const uint32_t low = mask & (0u - mask);
+        if ((v >> k) & 1u) out |= low;
+        ++k;
+        mask ^= low;
+    }
+    return out;
+}
+
+// Per-byte nonzero flags of a 32-bit word packed into 4 bits (byte i -> bit i).
+WZB_HD uint32_t nonzero_bytes4(uint32_t w) {
+    const uint32_t hi = ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu | w) & 0x80808080u;
+    return ((hi >> 7) | (hi >> 14) | (hi >> 21) | (hi >> 28)) & 0xFu;
+}
+// Per-byte nonzero flags of a 64-bit word packed into 8 bits.
+WZB_HD uint32_t nonzero_bytes8(uint64_t w) {
+    return nonzero_bytes4(uint32_t(w)) | (nonzero_bytes4(uint32_t(w >> 32)) << 4);
+}
+// RZE change flags for 8 consecutive bitmap bytes: bit k set iff byte k of w
+// differs from byte k-1 (byte -1 is `prev`) (cpu/src/wzp_cpu.cpp:566-577).
+WZB_HD uint32_t change_flags8(uint64_t w, uint32_t prev) {
+    const uint64_t shifted = (w << 8) | uint64_t(prev & 0xFFu);
+    return nonzero_bytes8(w ^ shifted);
+}
+
+}  // namespace wz

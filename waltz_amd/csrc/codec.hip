@@ -1,0 +1,357 @@
+// Whole-field compress / decompress pipelines on one device and stream.
+//
+// Compress (cpu/src/codec.cpp:177-258 order):
+//   forward lifting (input screened for non-finite samples on its first pass)
+//   -> brick quantization (magnitudes, signs, dequantized coefficients, q-outliers)
+//   -> WZP stream written straight into the archive at offset 32
+//   -> inverse lifting of the dequantized coefficients
+//   -> native repair detection against the input
+//   -> side channels ordered by index into their sections, header, padding.
+// One 48-byte status/result read back at the end decides success.
+//
+// Decompress (cpu/src/codec.cpp:276-336): 32-byte header read and validated on
+// the host, WZP decode, dequantization into row-major coefficients, q-outlier
+// patches, inverse lifting into the caller's output, repair patches, finite
+// screen.
+//
+// Scratch lives in one cached, grow-only device allocation per device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <vector>
+
+#include "common.hpp"
+#include "launch.hpp"
+#include "profile.hpp"
+
+namespace wz {
+
+__global__ void zero_kernel(uint32_t* __restrict__ p, uint64_t words) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0;
+}
+
+void zero_words(void* p, uint64_t words, hipStream_t s) {
+    if (!words) return;
+    const uint64_t blocks = (words + 255) / 256;
+    hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s,
+                       static_cast<uint32_t*>(p), words);
+}
+
+namespace {
+
+struct DeviceWorkspace {
+    std::mutex lock;
+    void* base = nullptr;
+    size_t capacity = 0;
+    uint16_t* rank = nullptr;  // 8*8192 forward + 8*8192 inverse
+    Dims rank_dims{0, 0, 0};
+    uint32_t rank_bz = 0;
+    EncodeResult* result = nullptr;      // device
+    EncodeResult* host_result = nullptr;  // pinned
+
+    ~DeviceWorkspace() = default;  // process-lifetime cache
+
+    void* reserve(size_t bytes) {
+        if (bytes <= capacity) return base;
+        if (base) hip_check(hipFree(base), "hipFree");
+        base = nullptr;
+        capacity = 0;
+        hip_check(hipMalloc(&base, bytes), "Waltz workspace allocation");
+        capacity = bytes;
+        return base;
+    }
+    void init() {
+        if (!rank) {
+            hip_check(hipMalloc(reinterpret_cast<void**>(&rank), 2 * 8 * 8192 * sizeof(uint16_t)), "rank tables");
+            hip_check(hipMalloc(reinterpret_cast<void**>(&result), sizeof(EncodeResult)), "result");
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_result), sizeof(EncodeResult)), "pinned result");
+        }
+    }
+    const uint16_t* ranks(Dims d, uint32_t bz, hipStream_t s) {
+        if (!(rank_dims.nx == d.nx && rank_dims.ny == d.ny && rank_dims.nz == d.nz && rank_bz == bz)) {
+            build_rank_tables(d, bz, rank, rank + 8 * 8192, s);
+            rank_dims = d;
+            rank_bz = bz;
+        }
+        return rank;
+    }
+};
+
+DeviceWorkspace& workspace() {
+    static std::mutex m;
+    static std::vector<std::unique_ptr<DeviceWorkspace>> all;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> g(m);
+    if (all.size() <= size_t(dev)) all.resize(size_t(dev) + 1);
+    if (!all[size_t(dev)]) all[size_t(dev)] = std::make_unique<DeviceWorkspace>();
+    return *all[size_t(dev)];
+}
+
+// Bump allocator over the workspace with 256-byte alignment.
+struct Carve {
+    uint8_t* base;
+    size_t used = 0;
+    template <class U>
+    U* take(uint64_t count) {
+        U* p = reinterpret_cast<U*>(base + used);
+        used += (count * sizeof(U) + 255) & ~size_t(255);
+        return p;
+    }
+};
+
+struct Geometry {
+    Dims d;
+    uint32_t n;
+    unsigned lx, lz;
+    bool dyadic;
+    uint32_t bz;
+};
+
+Geometry geometry(Dims d, unsigned width) {
+    if (!d.nx || !d.ny || !d.nz) throw std::invalid_argument("nonzero dimensions required");
+    const uint64_t n = uint64_t(d.nx) * d.ny * d.nz;
+    if (n > 0xFFFFFFFFull / width) throw std::invalid_argument("volume exceeds WLT2 section limits");
+    if (n > 0xFFFFFFFFull / 2) throw std::invalid_argument("WZP element count exceeds the GPU format limit");
+    Geometry g;
+    g.d = d;
+    g.n = uint32_t(n);
+    g.lx = dwt_levels(d.nx < d.ny ? d.nx : d.ny);
+    g.lz = dwt_levels(d.nz);
+    g.dyadic = g.lx == g.lz;
+    g.bz = zband(d, g.lz, g.dyadic);
+    return g;
+}
+
+[[noreturn]] void raise_status(uint32_t st) {
+    if (st & kNonFiniteInput) throw std::runtime_error("nonfinite input is unsupported");
+    if (st & kQuantDomain) throw std::runtime_error("coefficient exceeds portable integer conversion domain");
+    if (st & kRepairBound) throw std::runtime_error("native repair cannot meet strict bound or nonfinite reconstruction");
+    if (st & kSideCapacity) throw std::runtime_error("portable GPU side-channel capacity reached");
+    if (st & kWzpCapacity) throw std::runtime_error("archive capacity too small");
+    if (st & kLookbackTimeout) throw std::runtime_error("WZP lookback did not complete");
+    if (st & kCorrupt) throw std::runtime_error("corrupt Waltz archive");
+    if (st & kNonFiniteOutput) throw std::runtime_error("nonfinite reconstruction");
+    throw std::runtime_error("Waltz device status error");
+}
+
+uint32_t qcap(uint32_t n) { return n <= (1u << 20) ? n : (1u << 20); }
+uint32_t rcap(uint32_t n) { return n <= (1u << 22) ? n : (1u << 22); }
+
+}  // namespace
+
+uint64_t max_compressed_bytes(Dims d, unsigned width) {
+    const Geometry g = geometry(d, width);
+    const WzpShape ws = wzp_shape(g.n);
+    const uint64_t wzp = ws.header + uint64_t(g.n) * 2 + (uint64_t(g.n) + 7) / 8;
+    return archive_layout(wzp, g.n, uint64_t(g.n) * width, g.n).end;
+}
+
+template <class T>
+uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, uint64_t capacity, hipStream_t s) {
+    if (!(abs_bound > 0) || !std::isfinite(abs_bound))
+        throw std::invalid_argument("positive finite absolute error required");
+    if (reinterpret_cast<uintptr_t>(archive) % 8 || reinterpret_cast<uintptr_t>(input) % sizeof(T))
+        throw std::invalid_argument("archive must be 8-byte aligned and input naturally aligned");
+    const Geometry g = geometry(d, sizeof(T));
+    const double quantum = 1.5 * abs_bound;
+    const T q = T(quantum), inv = T(1.0 / quantum);
+    if (!(q > T(0)) || !std::isfinite(q) || !std::isfinite(inv))
+        throw std::invalid_argument("quantum not representable in native type");
+    if (capacity < 32) throw std::runtime_error("archive capacity too small");
+    const uint32_t n = g.n;
+    const uint64_t words = (uint64_t(n) + 31) / 32, segs = (uint64_t(n) + 1023) / 1024;
+    const WzpShape ws = wzp_shape(n);
+    const uint32_t nq = qcap(n), nr = rcap(n);
+
+    DeviceWorkspace& w = workspace();
+    std::lock_guard<std::mutex> guard(w.lock);
+    w.init();
+    // layout of the per-call scratch
+    auto plan = [&](Carve& c) {
+        struct P {
+            T *coeff, *t0, *t1;
+            uint16_t* mag;
+            uint32_t *sign, *qbm, *rbm, *qidx, *qval, *ridx, *seg, *zeroed;
+            T* rerr;
+            unsigned long long* desc;
+            uint32_t *excl, *size, *counters;
+        } p;
+        p.coeff = c.take<T>(n);
+        p.t0 = c.take<T>(n);
+        p.t1 = c.take<T>(n);
+        p.mag = c.take<uint16_t>(n);
+        p.ridx = c.take<uint32_t>(nr);
+        p.rerr = c.take<T>(nr);
+        p.qidx = c.take<uint32_t>(nq);
+        p.qval = c.take<uint32_t>(nq);
+        p.rbm = c.take<uint32_t>(words);
+        p.seg = c.take<uint32_t>(2 * segs);
+        p.excl = c.take<uint32_t>(ws.chunks);
+        p.size = c.take<uint32_t>(ws.chunks);
+        // zero-initialised region: sign, q bitmap, lookback descriptors, counters
+        const size_t z0 = c.used;
+        p.sign = c.take<uint32_t>(words);
+        p.qbm = c.take<uint32_t>(words);
+        p.desc = c.take<unsigned long long>(ws.chunks);
+        p.counters = c.take<uint32_t>(4);  // ticket, q count, r count
+        p.zeroed = reinterpret_cast<uint32_t*>(c.base + z0);
+        return std::make_pair(p, c.used - z0);
+    };
+    Carve probe{nullptr};
+    plan(probe);
+    Carve carve{static_cast<uint8_t*>(w.reserve(probe.used))};
+    auto [p, zero_bytes] = plan(carve);
+
+    EncodeResult* res = w.result;
+    uint32_t* status = &res->status;
+    zero_words(res, sizeof(EncodeResult) / 4, s);
+    zero_words(p.zeroed, zero_bytes / 4, s);
+
+    transform_forward<T>(input, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, status, s);
+    const BrickGeometry bg = brick_geometry(g.d, g.bz);
+    const uint16_t* rank = w.ranks(g.d, g.bz, s);
+    SideList qo{p.qbm, p.qidx, p.qval, p.counters + 1, nq};
+    {
+        StageClock c("quantize", s);
+        quantize_bricks<T>(p.coeff, bg, quantum, rank, p.mag, p.sign, qo, status, s);
+    }
+    WzpScratch wsc{p.desc, p.excl, p.size, p.counters};
+    {
+        StageClock c("wzp_encode", s);
+        wzp_encode(p.mag, p.sign, n, archive + 32, capacity - 32, wsc, &res->wzp_bytes, status, s);
+    }
+    transform_inverse<T>(p.coeff, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s);
+    SideList rp{p.rbm, p.ridx, p.rerr, p.counters + 2, nr};
+    {
+        StageClock c("repairs", s);
+        detect_repairs<T>(input, p.coeff, n, abs_bound, rp, status, s);
+    }
+    const uint8_t flags = uint8_t((g.dyadic ? 1u : 0u) | (sizeof(T) == 8 ? 4u : 0u) | (3u << 3));
+    {
+        StageClock c("sections", s);
+        finish_archive<T>(archive, capacity, n, rp, qo, p.seg, flags, uint8_t(g.lx), uint8_t(g.lz), uint8_t(g.bz),
+                          quantum, res, s);
+    }
+    hip_check(hipMemcpyAsync(w.host_result, res, sizeof(EncodeResult), hipMemcpyDeviceToHost, s), "result copy");
+    hip_check(hipStreamSynchronize(s), "compress");
+    hip_check(hipGetLastError(), "compress launch");
+    profile_collect();
+    const EncodeResult r = *w.host_result;
+    if (r.status) raise_status(r.status);
+    return r.end;
+}
+
+uint64_t archive_size_from_header(const uint8_t* h, Dims d, unsigned width) {
+    const Geometry g = geometry(d, width);
+    const uint8_t f = h[0];
+    if ((f & 0xE0u) || !(f & 0x18u) || ((f & 4u) ? 8u : 4u) != width)
+        throw std::runtime_error("invalid Waltz archive header");
+    if (((f >> 3) & 3u) != 3u) throw std::runtime_error("unsupported Waltz synthesis mode");
+    if (f & 2u) throw std::runtime_error("compact repairs are not part of the canonical profile");
+    const unsigned lx = h[1], lz = h[2], bz = h[3];
+    const bool dyadic = f & 1u;
+    if (lx > 5 || lz > 5 || bz < 1 || bz > 32 || (dyadic && lx != lz))
+        throw std::runtime_error("invalid levels or z band");
+    if (lx != g.lx || lz != g.lz) throw std::runtime_error("dimensions/levels mismatch");
+    auto rd32 = [&](int o) {
+        return uint32_t(h[o]) | (uint32_t(h[o + 1]) << 8) | (uint32_t(h[o + 2]) << 16) | (uint32_t(h[o + 3]) << 24);
+    };
+    const uint64_t wzp = uint64_t(rd32(16)) | (uint64_t(rd32(20)) << 32);
+    const uint32_t pb = rd32(4), np = rd32(24), nq = rd32(28);
+    double quantum;
+    const uint64_t qb = uint64_t(rd32(8)) | (uint64_t(rd32(12)) << 32);
+    std::memcpy(&quantum, &qb, 8);
+    const double qn = width == 4 ? double(float(quantum)) : quantum;
+    if (!(quantum > 0) || !std::isfinite(quantum) || !(qn > 0) || !std::isfinite(qn))
+        throw std::runtime_error("invalid quantum");
+    if (np > g.n || nq > g.n) throw std::runtime_error("invalid section counts");
+    if (uint64_t(pb) != uint64_t(np) * width) throw std::runtime_error("invalid native correction section");
+    if (wzp > (1ull << 40)) throw std::runtime_error("invalid section counts");
+    return archive_layout(wzp, np, pb, nq).end;
+}
+
+template <class T>
+void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s) {
+    if (reinterpret_cast<uintptr_t>(archive) % 8 || reinterpret_cast<uintptr_t>(output) % sizeof(T))
+        throw std::invalid_argument("archive must be 8-byte aligned and output naturally aligned");
+    const Geometry g = geometry(d, sizeof(T));
+    DeviceWorkspace& w = workspace();
+    std::lock_guard<std::mutex> guard(w.lock);
+    w.init();
+    uint8_t* h = reinterpret_cast<uint8_t*>(w.host_result);  // 48 pinned bytes, reused for the header
+    hip_check(hipMemcpyAsync(h, archive, 32, hipMemcpyDeviceToHost, s), "header copy");
+    hip_check(hipStreamSynchronize(s), "header copy");
+    uint8_t hdr[32];
+    std::memcpy(hdr, h, 32);
+    const uint64_t end = archive_size_from_header(hdr, d, sizeof(T));
+    if (archive_bytes != ~uint64_t(0) && end != archive_bytes)
+        throw std::runtime_error("truncated archive or trailing data");
+    const uint32_t n = g.n;
+    const uint32_t bz = hdr[3];
+    const uint64_t wzp = uint64_t(hdr[16]) | (uint64_t(hdr[17]) << 8) | (uint64_t(hdr[18]) << 16) |
+                         (uint64_t(hdr[19]) << 24) | (uint64_t(hdr[20]) << 32) | (uint64_t(hdr[21]) << 40) |
+                         (uint64_t(hdr[22]) << 48) | (uint64_t(hdr[23]) << 56);
+    auto rd32 = [&](int o) {
+        return uint32_t(hdr[o]) | (uint32_t(hdr[o + 1]) << 8) | (uint32_t(hdr[o + 2]) << 16) |
+               (uint32_t(hdr[o + 3]) << 24);
+    };
+    const uint32_t np = rd32(24), nq = rd32(28);
+    double quantum;
+    const uint64_t qb = uint64_t(rd32(8)) | (uint64_t(rd32(12)) << 32);
+    std::memcpy(&quantum, &qb, 8);
+    const Layout L = archive_layout(wzp, np, uint64_t(np) * sizeof(T), nq);
+    const WzpShape ws = wzp_shape(n);
+    if (wzp < ws.header || wzp > ws.header + uint64_t(n) * 2 + (uint64_t(n) + 7) / 8)
+        throw std::runtime_error("WZP archive size is outside its valid bounds");
+    const uint64_t words = (uint64_t(n) + 31) / 32;
+    Carve probe{nullptr};
+    probe.take<T>(n);
+    probe.take<T>(n);
+    probe.take<T>(n);
+    probe.take<uint16_t>(n);
+    probe.take<uint32_t>(words);
+    Carve c{static_cast<uint8_t*>(w.reserve(probe.used))};
+    T* coeff = c.take<T>(n);
+    T* t0 = c.take<T>(n);
+    T* t1 = c.take<T>(n);
+    uint16_t* mag = c.take<uint16_t>(n);
+    uint32_t* sign = c.take<uint32_t>(words);
+    uint32_t* status = &w.result->status;
+    zero_words(w.result, sizeof(EncodeResult) / 4, s);
+    {
+        StageClock c("wzp_decode", s);
+        wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
+    }
+    const BrickGeometry bg = brick_geometry(g.d, bz);
+    const uint16_t* rank = w.ranks(g.d, bz, s);
+    {
+        StageClock c("dequantize", s);
+        dequantize_bricks<T>(mag, sign, bg, quantum, rank, coeff, s);
+        apply_qoutliers<T>(coeff, archive + L.qi, archive + L.qv, nq, n, quantum, status, s);
+    }
+    transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, g.dyadic, s);
+    {
+        StageClock c("patches", s);
+        apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);
+        check_finite<T>(output, n, status, s);
+    }
+    hip_check(hipMemcpyAsync(w.host_result, w.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s),
+              "status copy");
+    hip_check(hipStreamSynchronize(s), "decompress");
+    hip_check(hipGetLastError(), "decompress launch");
+    profile_collect();
+    if (w.host_result->status) raise_status(w.host_result->status);
+}
+
+template uint64_t compress<float>(const float*, Dims, double, uint8_t*, uint64_t, hipStream_t);
+template uint64_t compress<double>(const double*, Dims, double, uint8_t*, uint64_t, hipStream_t);
+template void decompress<float>(const uint8_t*, uint64_t, Dims, float*, hipStream_t);
+template void decompress<double>(const uint8_t*, uint64_t, Dims, double*, hipStream_t);
+
+}  // namespace wz

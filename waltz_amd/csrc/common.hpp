@@ -1,0 +1,145 @@
+// Shared device/host definitions for the gfx950 Waltz codec.
+//
+// Arithmetic profile: every lifting step is an explicit correctly rounded fma
+// (v_fma_f32 / v_fma_f64) and every other + and * is a separately rounded
+// IEEE operation; the library is compiled with -ffp-contract=off, no fast-math
+// and IEEE denormals, so device results are bitwise those of the reference CPU
+// codec (cpu/src/transform.cpp:26-36,41-48).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#define WZ_HD __host__ __device__ __forceinline__
+#define WZ_D __device__ __forceinline__
+
+namespace wz {
+
+// ------------------------------------------------------------------ geometry
+struct Dims {
+    uint32_t nx, ny, nz;
+};
+
+WZ_HD uint32_t half_len(uint32_t n) { return n / 2 + n % 2; }
+
+// Whole-sample mirror with period 2n-2 (cpu/src/transform.cpp:60-66). Inputs
+// lie within a few tiles of the axis, so the modulo path is cold.
+WZ_HD uint32_t reflect_pos(int32_t p, uint32_t n) {
+    if (p >= 0 && uint32_t(p) < n) return uint32_t(p);
+    if (n <= 1) return 0;
+    const int32_t period = 2 * int32_t(n) - 2;
+    int32_t r = p % period;
+    if (r < 0) r += period;
+    return uint32_t(r < int32_t(n) ? r : period - r);
+}
+// Split-band position of a reflected sample (cpu/src/transform.cpp:68-71).
+WZ_HD uint32_t split_pos(int32_t p, uint32_t n) {
+    const uint32_t r = reflect_pos(p, n);
+    return (r >> 1) + ((r & 1u) ? half_len(n) : 0u);
+}
+
+// Level count per axis group (include/utils/Blob.hpp:22-29).
+WZ_HD unsigned dwt_levels(uint32_t n) {
+    unsigned l = 0;
+    while (n > 8 && l < 5) { n -= n / 2; ++l; }
+    return l;
+}
+
+// Morton z band (cpu/src/codec.cpp:89-95).
+WZ_HD uint32_t zband(Dims d, unsigned lz, bool dyadic) {
+    if (dyadic) {
+        const uint32_t v = (d.nz + ((1u << lz) - 1u)) >> lz;
+        return v < 16u ? v : 16u;
+    }
+    if (d.nz < 4) return d.nz;
+    const uint64_t blocks = uint64_t((d.nx + 15u) / 16) * ((d.ny + 15u) / 16);
+    const uint64_t full = uint64_t(d.nx / 16) * (d.ny / 16);
+    return (blocks - full) * 32 >= blocks ? 4u : 2u;
+}
+
+// Recursive ceil-half octant rank inside one brick: z half, then y, then x
+// (cpu/src/codec.cpp:96-107).
+WZ_HD uint32_t morton_rank(uint32_t x, uint32_t y, uint32_t z, uint32_t nx, uint32_t ny, uint32_t nz) {
+    uint32_t rank = 0;
+    while (nx > 1 || ny > 1 || nz > 1) {
+        const uint32_t lx = nx - nx / 2, ly = ny - ny / 2, lz = nz - nz / 2;
+        const bool cx = x >= lx, cy = y >= ly, cz = z >= lz;
+        const uint32_t zn = cz ? nz / 2 : lz, yn = cy ? ny / 2 : ly;
+        if (cz) rank += lz * nx * ny;
+        if (cy) rank += zn * ly * nx;
+        if (cx) rank += zn * yn * lx;
+        if (cx) x -= lx;
+        if (cy) y -= ly;
+        if (cz) z -= lz;
+        nx = cx ? nx / 2 : lx;
+        ny = yn;
+        nz = zn;
+    }
+    return rank;
+}
+
+// ------------------------------------------------------------------ lifting constants
+// CDF 9/7 lifting steps and scale, rounded to T (cpu/src/transform.cpp:41-48).
+template <class T> struct Lift {
+    static constexpr T a = T(-1.586134342059924);
+    static constexpr T b = T(-0.052980118572961);
+    static constexpr T g = T(0.882911075530934);
+    static constexpr T d = T(0.443506852043971);
+    static constexpr T k = T(1.1496043988602418);
+    static constexpr T ik = T(1) / T(1.1496043988602418);
+};
+
+WZ_HD float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+WZ_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ------------------------------------------------------------------ WZP format
+constexpr uint32_t kChunk = 8192;        // magnitudes per chunk
+constexpr uint32_t kChunkBytes = 16384;  // raw magnitude bytes per chunk
+constexpr uint32_t kGroup = 16;          // chunks per group
+constexpr uint32_t kSignSlot = 1024;     // sign code 0 = full 1024-byte slot
+constexpr uint32_t kRawCode = 0x3fff;
+
+struct WzpShape {
+    uint32_t chunks, groups;
+    uint64_t header;
+};
+WZ_HD WzpShape wzp_shape(uint32_t n) {
+    WzpShape s;
+    s.chunks = n == 0 ? 0u : 1u + (n - 1u) / kChunk;
+    s.groups = (s.chunks + kGroup - 1u) / kGroup;
+    s.header = uint64_t(s.groups) * 4u + uint64_t(s.chunks) * 3u;
+    return s;
+}
+
+// ------------------------------------------------------------------ archive layout
+// 32-byte little-endian header followed by 8-byte aligned sections
+// (include/utils/Blob.hpp:38-79).
+WZ_HD uint64_t pad8(uint64_t n) { return (n + 7) & ~uint64_t(7); }
+struct Layout {
+    uint64_t pi, pv, qi, qv, end;
+};
+WZ_HD Layout archive_layout(uint64_t wzp, uint32_t np, uint64_t repair_bytes, uint32_t nq) {
+    Layout L;
+    L.pi = 32 + pad8(wzp);
+    L.pv = L.pi + pad8(uint64_t(np) * 4);
+    L.qi = L.pv + pad8(repair_bytes);
+    L.qv = L.qi + pad8(uint64_t(nq) * 4);
+    L.end = L.qv + pad8(uint64_t(nq) * 4);
+    return L;
+}
+
+// Device status bits shared by all kernels of one call.
+enum Status : uint32_t {
+    kNonFiniteInput = 1u << 0,
+    kQuantDomain = 1u << 1,
+    kRepairBound = 1u << 2,
+    kWzpCapacity = 1u << 3,
+    kLookbackTimeout = 1u << 4,
+    kCorrupt = 1u << 5,
+    kNonFiniteOutput = 1u << 6,
+    kSideCapacity = 1u << 7,
+};
+
+}  // namespace wz

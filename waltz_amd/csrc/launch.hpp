@@ -1,0 +1,104 @@
+// Host-side declarations of the stage launchers (one per pipeline stage).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "common.hpp"
+
+namespace wz {
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+inline void check_launch_size(uint64_t total) {
+    if (total > 0xFFFFFF00ull) throw std::invalid_argument("Waltz launch exceeds the 32-bit grid range");
+}
+
+// transform.hip
+template <class T>
+void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz,
+                       bool dyadic, uint32_t* status, hipStream_t s);
+template <class T>
+void transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       hipStream_t s);
+
+// quantize.hip
+struct BrickGeometry {
+    Dims d;
+    uint32_t bz, nbx, nby, nbz;
+};
+BrickGeometry brick_geometry(Dims d, uint32_t bz);
+// rank: 8*8192 u16 (local row-major -> Morton rank); inv: 8*8192 u16 (rank -> local index).
+void build_rank_tables(Dims d, uint32_t bz, uint16_t* rank, uint16_t* inv, hipStream_t s);
+
+struct SideList {          // unsorted (index, payload) list + row-major flag bitmap
+    uint32_t* bitmap;      // ceil(n/32) words, zeroed
+    uint32_t* index;       // capacity entries
+    void* payload;         // capacity entries of payload type
+    uint32_t* count;       // device counter (zeroed)
+    uint32_t capacity;
+};
+
+template <class T>
+void quantize_bricks(T* coeff, const BrickGeometry& g, double quantum, const uint16_t* rank, uint16_t* mag,
+                     uint32_t* sign, SideList q, uint32_t* status, hipStream_t s);
+template <class T>
+void dequantize_bricks(const uint16_t* mag, const uint32_t* sign, const BrickGeometry& g, double quantum,
+                       const uint16_t* rank, T* out, hipStream_t s);
+
+// wzp.hip
+struct WzpScratch {
+    unsigned long long* desc;  // chunks lookback descriptors (zeroed)
+    uint32_t* excl;            // chunks exclusive payload offsets
+    uint32_t* size;            // chunks payload sizes
+    uint32_t* ticket;          // 1 counter (zeroed)
+};
+// Encodes mag/sign into out (WZP stream at out[0..]); payload capacity bound
+// `capacity` (bytes available from out). Total bytes -> *total (device u64).
+void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity,
+                WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s);
+// Decodes a WZP stream of `bytes` bytes into packed mag/sign arrays.
+void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
+                hipStream_t s);
+
+// sections.hip
+template <class T>
+void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, SideList r,
+                    uint32_t* status, hipStream_t s);
+struct EncodeResult {
+    unsigned long long wzp_bytes;  // written by wzp_encode
+    unsigned long long end;        // archive bytes
+    uint32_t np, nq, status, pad;
+};
+// Orders both side lists by index into the archive sections and writes the
+// 32-byte header and all padding. `seg` needs ceil(n/1024)*2 u32 of scratch.
+template <class T>
+void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SideList repairs, SideList qout,
+                    uint32_t* seg, uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum,
+                    EncodeResult* result, hipStream_t s);
+template <class T>
+void apply_qoutliers(T* coeff, const uint8_t* idx, const uint8_t* val, uint32_t count, uint32_t n, double quantum,
+                     uint32_t* status, hipStream_t s);
+template <class T>
+void apply_repairs(T* out, const uint8_t* idx, const uint8_t* err, uint32_t count, uint32_t n, uint32_t* status,
+                   hipStream_t s);
+// Fills `words` 32-bit words with zero (a kernel, so calls stay graph-capturable).
+void zero_words(void* p, uint64_t words, hipStream_t s);
+
+// codec.hip: whole-field pipelines. Device pointers; synchronous on `s`.
+// Throw std::invalid_argument / std::runtime_error with the reference's messages.
+template <class T>
+uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, uint64_t capacity, hipStream_t s);
+template <class T>
+void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s);
+// Archive bytes implied by a 32-byte header (validated), or throws.
+uint64_t archive_size_from_header(const uint8_t* header, Dims d, unsigned width);
+uint64_t max_compressed_bytes(Dims d, unsigned width);
+template <class T>
+void check_finite(const T* out, uint32_t n, uint32_t* status, hipStream_t s);
+
+}  // namespace wz

@@ -1,0 +1,279 @@
+// CDF 9/7 lifting passes for gfx950.
+//
+// One launch lifts one axis of the active low box [0,act.nx)x[0,act.ny)x[0,act.nz)
+// of a full row-major volume (x fastest), out of place: src -> dst, both with
+// the full volume's strides. Each thread owns one canonical tile of one line:
+// it loads P = Tile/2+4 (even, odd) sample pairs into registers (reflected at
+// the axis ends), runs the lifting steps there and writes Tile outputs.
+//
+// Tiles follow the reference CPU halo schedule (cpu/src/transform.cpp:113-179,
+// 377-456). Forward tiles and the inverse X tile are bit-neutral; the inverse
+// Y tile (8) and Z tile (16 for f32, 8 for f64) are part of the format because
+// the two endpoint sums of a Y/Z tile are contracted into one fma each
+// (cpu/src/transform.cpp:115-129).
+//
+// Y and Z lines map consecutive lanes to consecutive x, so every register load
+// is a fully coalesced 256 B (f32) / 512 B (f64) wave access; X lines map lanes
+// to consecutive tiles of one row (overlapping 64 B windows served from L1).
+#include "common.hpp"
+#include "launch.hpp"
+#include "profile.hpp"
+
+namespace wz {
+
+// Forward lifting of P pairs in registers (cpu/src/transform.cpp:136-145).
+template <class T, int P>
+WZ_D void lift_forward(T (&e)[P], T (&o)[P]) {
+    using C = Lift<T>;
+#pragma unroll
+    for (int j = 0; j < P - 1; ++j) o[j] = fmadd(C::a, e[j] + e[j + 1], o[j]);
+#pragma unroll
+    for (int j = 1; j < P - 1; ++j) e[j] = fmadd(C::b, o[j - 1] + o[j], e[j]);
+#pragma unroll
+    for (int j = 1; j < P - 2; ++j) o[j] = fmadd(C::g, e[j] + e[j + 1], o[j]);
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) e[j] = fmadd(C::d, o[j - 1] + o[j], e[j]);
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) {
+        e[j] = e[j] * C::k;
+        o[j] = o[j] * (-C::ik);
+    }
+}
+
+// Inverse lifting of P pairs; Folded contracts the two tile-end sums
+// (cpu/src/transform.cpp:117-135).
+template <class T, int P, bool Folded>
+WZ_D void lift_inverse(T (&e)[P], T (&o)[P]) {
+    using C = Lift<T>;
+    const T first_odd = o[0], last_odd = o[P - 1];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        e[j] = e[j] * C::ik;
+        o[j] = o[j] * (-C::k);
+    }
+#pragma unroll
+    for (int j = 1; j < P; ++j) {
+        T s;
+        if (Folded && j == 1) s = fmadd(first_odd, -C::k, o[1]);
+        else if (Folded && j == P - 1) s = fmadd(last_odd, -C::k, o[P - 2]);
+        else s = o[j - 1] + o[j];
+        e[j] = fmadd(-C::d, s, e[j]);
+    }
+#pragma unroll
+    for (int j = 1; j < P - 1; ++j) o[j] = fmadd(-C::g, e[j] + e[j + 1], o[j]);
+#pragma unroll
+    for (int j = 2; j < P - 1; ++j) e[j] = fmadd(-C::b, o[j - 1] + o[j], e[j]);
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) o[j] = fmadd(-C::a, e[j] + e[j + 1], o[j]);
+}
+
+template <class T>
+WZ_D bool finite_value(T v) {
+    return __builtin_isfinite(v);
+}
+
+// Axis: 0 = X, 1 = Y, 2 = Z.
+template <class T, bool Inverse, int Axis, int Tile>
+__global__ void __launch_bounds__(256) axis_pass(const T* __restrict__ src, T* __restrict__ dst, Dims full,
+                                                 Dims act, uint32_t ntiles, uint32_t total,
+                                                 uint32_t* __restrict__ status) {
+    constexpr int P = Tile / 2 + 4;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    uint32_t x = 0, y = 0, z = 0, t;
+    if constexpr (Axis == 0) {
+        t = i % ntiles;
+        const uint32_t r = i / ntiles;
+        y = r % act.ny;
+        z = r / act.ny;
+    } else if constexpr (Axis == 1) {
+        x = i % act.nx;
+        const uint32_t r = i / act.nx;
+        t = r % ntiles;
+        z = r / ntiles;
+    } else {
+        x = i % act.nx;
+        const uint32_t r = i / act.nx;
+        y = r % act.ny;
+        t = r / act.ny;
+    }
+    const uint64_t plane = uint64_t(full.nx) * full.ny;
+    const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
+    const uint64_t stride = Axis == 0 ? 1u : Axis == 1 ? uint64_t(full.nx) : plane;
+    const uint64_t base = uint64_t(z) * plane + uint64_t(y) * full.nx + x;
+    const int32_t start = int32_t(t) * Tile;
+    T e[P], o[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int32_t p = start - 4 + 2 * j;
+        const uint32_t ia = Inverse ? split_pos(p, n) : reflect_pos(p, n);
+        const uint32_t ib = Inverse ? split_pos(p + 1, n) : reflect_pos(p + 1, n);
+        e[j] = src[base + uint64_t(ia) * stride];
+        o[j] = src[base + uint64_t(ib) * stride];
+    }
+    if (status) {  // input screen on the first forward pass (cpu/src/codec.cpp:183)
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < P; ++j) ok = ok && finite_value(e[j]) && finite_value(o[j]);
+        if (!ok) atomicOr(status, uint32_t(kNonFiniteInput));
+    }
+    if constexpr (Inverse) lift_inverse<T, P, Axis != 0>(e, o);
+    else lift_forward<T, P>(e, o);
+    const uint32_t h = half_len(n);
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) {
+        const uint32_t p = uint32_t(start) + 2u * uint32_t(j - 2);
+        if (p < n) dst[base + uint64_t(Inverse ? p : p >> 1) * stride] = e[j];
+        if (p + 1 < n) dst[base + uint64_t(Inverse ? p + 1 : h + (p >> 1)) * stride] = o[j];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) box_copy(const T* __restrict__ src, T* __restrict__ dst, Dims full, Dims act,
+                                                uint32_t total) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t x = i % act.nx, r = i / act.nx, y = r % act.ny, z = r / act.ny;
+    const uint64_t at = (uint64_t(z) * full.ny + y) * full.nx + x;
+    dst[at] = src[at];
+}
+
+namespace {
+Dims shrink(Dims d, bool xy, bool z) {
+    if (xy) { d.nx = half_len(d.nx); d.ny = half_len(d.ny); }
+    if (z) d.nz = half_len(d.nz);
+    return d;
+}
+
+template <class T, bool Inverse, int Axis, int Tile>
+void launch_axis(const T* src, T* dst, Dims full, Dims act, uint32_t* status, hipStream_t stream) {
+    const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
+    const uint32_t ntiles = (n + Tile - 1) / Tile;
+    const uint64_t lines = uint64_t(act.nx) * act.ny * act.nz / n;
+    const uint64_t total = lines * ntiles;
+    if (total == 0) return;
+    check_launch_size(total);
+    hipLaunchKernelGGL((axis_pass<T, Inverse, Axis, Tile>), dim3(uint32_t((total + 255) / 256)), dim3(256), 0, stream,
+                       src, dst, full, act, ntiles, uint32_t(total), status);
+}
+
+template <class T, bool Inverse>
+void axis(const T* src, T* dst, Dims full, Dims act, int axis, uint32_t* status, hipStream_t s) {
+    constexpr int ZTileInv = sizeof(T) == 4 ? 16 : 8;
+    if (axis == 0) launch_axis<T, Inverse, 0, 8>(src, dst, full, act, status, s);
+    else if (axis == 1) launch_axis<T, Inverse, 1, 8>(src, dst, full, act, status, s);
+    else if (Inverse) launch_axis<T, Inverse, 2, ZTileInv>(src, dst, full, act, status, s);
+    else launch_axis<T, Inverse, 2, 8>(src, dst, full, act, status, s);
+}
+
+template <class T>
+void copy_box(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
+    const uint64_t total = uint64_t(act.nx) * act.ny * act.nz;
+    if (!total) return;
+    check_launch_size(total);
+    hipLaunchKernelGGL((box_copy<T>), dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, src, dst, full, act,
+                       uint32_t(total));
+}
+
+// Runs a sequence of axis passes on one box: data starts in `home` (or in
+// `first_src` for the very first pass), every pass is out of place, and the
+// box ends in `home` (or in `last_dst` for the very last pass).
+template <class T, bool Inverse>
+struct Passes {
+    T* home;
+    T* tmp[2];
+    const T* first_src;
+    T* last_dst;
+    uint32_t* status;
+    hipStream_t s;
+    bool first = true;
+
+    // Passes: src -> tmp0 -> tmp1 -> ... -> final. A single pass whose source
+    // is already `home` goes through tmp0 and is copied back.
+    void level(Dims full, Dims act, const int* axes, int count, bool last_level, unsigned index) {
+        static const char* fwd[] = {"forward_L0", "forward_L1", "forward_L2", "forward_L3", "forward_L4"};
+        static const char* inv[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
+        StageClock clock(Inverse ? inv[index] : fwd[index], s);
+        const T* src = first ? first_src : home;
+        T* final_dst = last_level && last_dst ? last_dst : home;
+        for (int k = 0; k < count; ++k) {
+            T* dst = k == count - 1 ? final_dst : tmp[k & 1];
+            if (dst == src) dst = tmp[0];
+            axis<T, Inverse>(src, dst, full, act, axes[k], first && k == 0 ? status : nullptr, s);
+            src = dst;
+        }
+        if (src != final_dst) copy_box<T>(src, final_dst, full, act, s);
+        first = false;
+    }
+};
+}  // namespace
+
+// Full multi-level transform (cpu/src/transform.cpp:409-456).
+// Forward: reads `input` (may equal coeff), leaves coefficients in `coeff`.
+// Inverse: reads coefficients in `coeff` (overwritten), writes the last pass
+// into `output` (may equal coeff). tmp0/tmp1: two full-volume scratch arrays.
+template <class T>
+void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz,
+                       bool dyadic, uint32_t* status, hipStream_t s) {
+    Passes<T, false> p{coeff, {tmp0, tmp1}, input, nullptr, status, s};
+    if (lxy == 0 && lz == 0) {
+        if (input != coeff) copy_box<T>(input, coeff, full, full, s);
+        return;
+    }
+    if (dyadic) {
+        Dims act = full;
+        static const int yxz[3] = {1, 0, 2};
+        for (unsigned l = 0; l < lxy; ++l) {
+            p.level(full, act, yxz, 3, false, l);
+            act = shrink(act, true, true);
+        }
+        return;
+    }
+    static const int zonly[1] = {2};
+    static const int yx[2] = {1, 0};
+    Dims act = full;
+    for (unsigned l = 0; l < lz; ++l) {
+        p.level(full, act, zonly, 1, false, l);
+        act = shrink(act, false, true);
+    }
+    act = full;
+    for (unsigned l = 0; l < lxy; ++l) {
+        p.level(full, act, yx, 2, false, l);
+        act = shrink(act, true, false);
+    }
+}
+
+template <class T>
+void transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       hipStream_t s) {
+    Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s};
+    if (lxy == 0 && lz == 0) {
+        if (output != coeff) copy_box<T>(coeff, output, full, full, s);
+        return;
+    }
+    if (dyadic) {
+        Dims sizes[5], act = full;
+        for (unsigned l = 0; l < lxy; ++l) { sizes[l] = act; act = shrink(act, true, true); }
+        static const int zxy[3] = {2, 0, 1};
+        for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, sizes[l], zxy, 3, l == 0, unsigned(l));
+        return;
+    }
+    Dims xy[5], zs[5], act = full;
+    for (unsigned l = 0; l < lxy; ++l) { xy[l] = act; act = shrink(act, true, false); }
+    act = full;
+    for (unsigned l = 0; l < lz; ++l) { zs[l] = act; act = shrink(act, false, true); }
+    static const int xyo[2] = {0, 1};
+    static const int zonly[1] = {2};
+    for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, xy[l], xyo, 2, lz == 0 && l == 0, unsigned(l));
+    for (int l = int(lz) - 1; l >= 0; --l) p.level(full, zs[l], zonly, 1, l == 0, unsigned(l));
+}
+
+template void transform_forward<float>(const float*, float*, float*, float*, Dims, unsigned, unsigned, bool,
+                                       uint32_t*, hipStream_t);
+template void transform_forward<double>(const double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
+                                        uint32_t*, hipStream_t);
+template void transform_inverse<float>(float*, float*, float*, float*, Dims, unsigned, unsigned, bool, hipStream_t);
+template void transform_inverse<double>(double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
+                                        hipStream_t);
+
+}  // namespace wz

@@ -1,0 +1,504 @@
+// WZP magnitude/sign stream (cpu/src/wzp_cpu.cpp:769-949) on gfx950.
+//
+// Stream: [groups x u32 group offset][chunks x 24-bit meta][payload], one chunk
+// = 8192 u16 magnitudes, one group = 16 chunks; payload in logical chunk order.
+// Chunk payload = magnitude bytes (RZE over BIT_2 planes, or raw LE u16 when
+// RZE+2 >= raw size) then the signs of the nonzero magnitudes, LSB first.
+//
+// Encoder: one 256-thread workgroup (4 wave64s) per chunk, taken in ticket
+// order. Each thread owns 32 magnitudes = one sign word = 64 transformed
+// bytes = 8 bytes of the zero bitmap B0 = one byte of B1. All counts are
+// packed two per 32-bit word into block-wide scans (wave shuffles + LDS).
+// The payload offset comes from a decoupled lookback over chunk descriptors
+// (aggregate / inclusive flags, agent-scope acquire/release), so every chunk
+// writes its bytes straight to its final place in logical order; a tiny
+// second kernel fills the per-group directory from the chunk prefixes.
+//
+// Decoder: one workgroup per chunk; the chunk offset is the group offset plus
+// the sizes of the earlier chunks of its group (from their metas). RZE is
+// undone top-down with popcount prefix ranks, then BIT_2, then signs are
+// expanded under the nonzero mask. Every read is bounds-checked against the
+// stream; inconsistencies raise kCorrupt.
+#include "bits.hpp"
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace wz {
+
+namespace {
+constexpr uint32_t kThreads = 256;
+
+WZ_D uint32_t wave_inclusive(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan for 256 threads; `sh` holds >= 4 words.
+WZ_D uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const uint32_t inc = wave_inclusive(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 63u) sh[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kThreads / 64; ++i) {
+        const uint32_t s = sh[i];
+        before += i < w ? s : 0u;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return before + inc - v;
+}
+
+WZ_D uint32_t load24(const uint8_t* p) { return uint32_t(p[0]) | (uint32_t(p[1]) << 8) | (uint32_t(p[2]) << 16); }
+WZ_D uint32_t load32(const uint8_t* p) { return load24(p) | (uint32_t(p[3]) << 24); }
+
+constexpr unsigned long long kFlagAggregate = 1ull << 62;
+constexpr unsigned long long kFlagInclusive = 2ull << 62;
+constexpr unsigned long long kValueMask = (1ull << 62) - 1;
+constexpr uint32_t kSpinLimit = 1u << 26;
+
+WZ_D uint32_t chunk_elements(uint32_t n, uint32_t c) {
+    const uint32_t rest = n - c * kChunk;
+    return rest < kChunk ? rest : kChunk;
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) wzp_encode_kernel(const uint16_t* __restrict__ mag,
+                                                         const uint32_t* __restrict__ sign, uint32_t n,
+                                                         uint32_t chunks, uint32_t groups, uint8_t* __restrict__ out,
+                                                         uint64_t capacity, WzpScratch w,
+                                                         unsigned long long* __restrict__ total,
+                                                         uint32_t* __restrict__ status) {
+    __shared__ uint32_t tr[kChunk / 2];                // BIT_2 planes, 16 KiB
+    __shared__ unsigned long long b0s[kThreads];       // B0, 2048 bytes
+    __shared__ __attribute__((aligned(8))) uint8_t b1s[kThreads];
+    __shared__ __attribute__((aligned(8))) uint8_t b2s[32];
+    __shared__ uint32_t signbuf[kThreads + 1];
+    __shared__ uint32_t sh[8];
+    __shared__ uint32_t n3s[4], b3s[4];
+    __shared__ uint32_t chunk_sh;
+    __shared__ unsigned long long excl_sh;
+
+    const uint32_t t = threadIdx.x;
+    if (t == 0) chunk_sh = atomicAdd(w.ticket, 1u);
+    signbuf[t] = 0;
+    if (t == 0) signbuf[kThreads] = 0;
+    __syncthreads();
+    const uint32_t c = chunk_sh;
+    const uint32_t E = chunk_elements(n, c);
+    const uint32_t raw_bytes = 2 * E;
+    const uint32_t first = t * 32;
+
+    // 1. this thread's 32 magnitudes
+    uint32_t m[32];
+    const uint16_t* src = mag + uint64_t(c) * kChunk + first;
+    if (first + 32 <= E) {
+        const uint4* v = reinterpret_cast<const uint4*>(src);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 x = v[q];
+            const uint32_t wds[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                m[8 * q + 2 * h] = wds[h] & 0xFFFFu;
+                m[8 * q + 2 * h + 1] = wds[h] >> 16;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) m[i] = first + i < E ? uint32_t(src[i]) : 0u;
+    }
+    uint32_t nzmask = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) nzmask |= uint32_t(m[i] != 0) << i;
+    const uint32_t cnt = popc32(nzmask);
+
+    // 2. BIT_2 planes into LDS (plane p of block b at u16 index p*blocks+b)
+    uint16_t* tr16 = reinterpret_cast<uint16_t*>(tr);
+    const uint32_t blocks = E / 16;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t b = 2 * t + h;
+        if (b < blocks) {
+            uint32_t p[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) p[j] = m[16 * h + j];
+            bit2_forward(p);
+#pragma unroll
+            for (int pl = 0; pl < 16; ++pl) tr16[pl * blocks + b] = uint16_t(p[pl]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t i = first + 16 * h + j;
+                if (i < E) tr16[i] = uint16_t(m[16 * h + j]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. RZE bitmaps: B0 (nonzero bytes), B1/B2/B3 (changed bytes of the level below)
+    unsigned long long b0t = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) b0t |= (unsigned long long)nonzero_bytes4(tr[16 * t + q]) << (4 * q);
+    {
+        const int32_t valid = int32_t(raw_bytes) - int32_t(64 * t);
+        if (valid <= 0) b0t = 0;
+        else if (valid < 64) b0t &= (1ull << valid) - 1ull;
+    }
+    b0s[t] = b0t;
+    __syncthreads();
+    const uint32_t b1t = change_flags8(b0t, t ? uint32_t(b0s[t - 1] >> 56) : 0u);
+    b1s[t] = uint8_t(b1t);
+    __syncthreads();
+    uint32_t b2t = 0;
+    if (t < 32) {
+        const unsigned long long w8 = *reinterpret_cast<const unsigned long long*>(&b1s[8 * t]);
+        b2t = change_flags8(w8, t ? uint32_t(b1s[8 * t - 1]) : 0u);
+        b2s[t] = uint8_t(b2t);
+    }
+    __syncthreads();
+    if (t < 4) {
+        const unsigned long long w8 = *reinterpret_cast<const unsigned long long*>(&b2s[8 * t]);
+        const uint32_t b3t = change_flags8(w8, t ? uint32_t(b2s[8 * t - 1]) : 0u);
+        b3s[t] = b3t;
+        n3s[t] = popc32(b3t);
+    }
+    const uint32_t nzb = popc64(b0t), n1 = popc32(b1t), n2 = popc32(b2t);
+    uint32_t totA, totB;
+    const uint32_t exA = block_exclusive(nzb | (n1 << 16), sh, totA);   // also orders b3s/n3s
+    const uint32_t exB = block_exclusive(cnt | (n2 << 16), sh, totB);
+    const uint32_t e0 = exA & 0xFFFFu, e1 = exA >> 16, es = exB & 0xFFFFu, e2 = exB >> 16;
+    const uint32_t K0 = totA & 0xFFFFu, K1 = totA >> 16, NNZ = totB & 0xFFFFu, K2 = totB >> 16;
+    const uint32_t K3 = n3s[0] + n3s[1] + n3s[2] + n3s[3];
+    const uint32_t rze = K0 + K1 + K2 + K3 + 4;
+    const bool raw = rze + 2 >= raw_bytes;
+    const uint32_t stored = raw ? raw_bytes : rze;
+    const uint32_t sign_bytes = (NNZ + 7) / 8;
+    const uint32_t chunk_bytes = NNZ ? stored + sign_bytes : 0u;
+
+    // 4. compacted signs of the nonzero magnitudes, LSB first, into LDS
+    {
+        const uint64_t word = uint64_t(c) * (kChunk / 32) + t;
+        const uint32_t sw = first < E ? sign[word] : 0u;
+        const uint32_t comp = compress_bits(sw, nzmask);
+        if (cnt) {
+            const uint32_t off = es & 31u, wd = es >> 5;
+            atomicOr(&signbuf[wd], comp << off);
+            if (off && off + cnt > 32) atomicOr(&signbuf[wd + 1], comp >> (32 - off));
+        }
+    }
+
+    // 5. decoupled lookback for this chunk's payload offset
+    if (t == 0) {
+        unsigned long long acc = 0;
+        if (c == 0) {
+            __hip_atomic_store(&w.desc[0], kFlagInclusive | chunk_bytes, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&w.desc[c], kFlagAggregate | chunk_bytes, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = int64_t(c) - 1;
+            uint32_t spins = 0;
+            while (j >= 0) {
+                const unsigned long long d = __hip_atomic_load(&w.desc[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long flag = d & ~kValueMask;
+                if (flag == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(status, uint32_t(kLookbackTimeout));
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                acc += d & kValueMask;
+                if (flag == kFlagInclusive) break;
+                --j;
+            }
+            __hip_atomic_store(&w.desc[c], kFlagInclusive | (acc + chunk_bytes), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        w.excl[c] = uint32_t(acc);
+        w.size[c] = chunk_bytes;
+        if (c == chunks - 1) *total = uint64_t(groups) * 4 + uint64_t(chunks) * 3 + acc + chunk_bytes;
+        excl_sh = acc;
+    }
+    __syncthreads();
+
+    // 6. payload at its final position
+    const uint64_t header = uint64_t(groups) * 4 + uint64_t(chunks) * 3;
+    const uint64_t start = header + excl_sh;
+    if (start + chunk_bytes > capacity) {
+        if (t == 0) atomicOr(status, uint32_t(kWzpCapacity));
+        return;
+    }
+    if (t == 0) {
+        const uint32_t sign_code = sign_bytes == kSignSlot ? 0u : sign_bytes;
+        const uint32_t meta = NNZ ? ((raw ? kRawCode : stored) | (sign_code << 14)) : 0u;
+        uint8_t* mp = out + uint64_t(groups) * 4 + uint64_t(c) * 3;
+        mp[0] = uint8_t(meta);
+        mp[1] = uint8_t(meta >> 8);
+        mp[2] = uint8_t(meta >> 16);
+    }
+    if (!NNZ) return;
+    uint8_t* dst = out + start;
+    if (raw) {
+        for (int i = 0; i < 32; ++i) {
+            if (first + i < E) {
+                dst[2 * (first + i)] = uint8_t(m[i]);
+                dst[2 * (first + i) + 1] = uint8_t(m[i] >> 8);
+            }
+        }
+    } else {
+        // nonzero bytes in order
+        {
+            uint32_t k = e0;
+            const uint8_t* bytes = reinterpret_cast<const uint8_t*>(tr) + 64 * t;
+            unsigned long long bits = b0t;
+            while (bits) {
+                const uint32_t j = uint32_t(__builtin_ctzll(bits));
+                dst[k++] = bytes[j];
+                bits &= bits - 1;
+            }
+        }
+        // changed B0 bytes
+        {
+            uint32_t k = K0 + e1, bits = b1t;
+            while (bits) {
+                const uint32_t j = uint32_t(__builtin_ctz(bits));
+                dst[k++] = uint8_t(b0t >> (8 * j));
+                bits &= bits - 1;
+            }
+        }
+        // changed B1 bytes
+        if (t < 32) {
+            uint32_t k = K0 + K1 + e2, bits = b2t;
+            while (bits) {
+                const uint32_t j = uint32_t(__builtin_ctz(bits));
+                dst[k++] = b1s[8 * t + j];
+                bits &= bits - 1;
+            }
+        }
+        // changed B2 bytes, then B3 verbatim
+        if (t < 4) {
+            uint32_t k = K0 + K1 + K2;
+            for (uint32_t i = 0; i < t; ++i) k += n3s[i];
+            uint32_t bits = b3s[t];
+            while (bits) {
+                const uint32_t j = uint32_t(__builtin_ctz(bits));
+                dst[k++] = b2s[8 * t + j];
+                bits &= bits - 1;
+            }
+            dst[rze - 4 + t] = uint8_t(b3s[t]);
+        }
+    }
+    __syncthreads();  // signbuf complete
+    for (uint32_t i = t; i < sign_bytes; i += kThreads) dst[stored + i] = uint8_t(signbuf[i >> 2] >> (8 * (i & 3u)));
+}
+
+// Group directory: offset of the group's first payload byte, or 0 when the
+// whole group is empty (cpu/src/wzp_cpu.cpp:789-830).
+__global__ void wzp_directory_kernel(uint8_t* __restrict__ out, uint32_t chunks, uint32_t groups, WzpScratch w,
+                                     uint64_t capacity) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups || uint64_t(g) * 4 + 4 > capacity) return;
+    const uint32_t first = g * kGroup, last = min(chunks, first + kGroup);
+    uint32_t len = 0;
+    for (uint32_t c = first; c < last; ++c) len += w.size[c];
+    const uint32_t off = len ? w.excl[first] : 0u;
+    uint8_t* p = out + uint64_t(g) * 4;
+    p[0] = uint8_t(off);
+    p[1] = uint8_t(off >> 8);
+    p[2] = uint8_t(off >> 16);
+    p[3] = uint8_t(off >> 24);
+}
+
+void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity, WzpScratch w,
+                unsigned long long* total, uint32_t* status, hipStream_t s) {
+    const WzpShape ws = wzp_shape(n);
+    if (ws.chunks == 0) return;
+    hipLaunchKernelGGL(wzp_encode_kernel, dim3(ws.chunks), dim3(kThreads), 0, s, mag, sign, n, ws.chunks, ws.groups,
+                       out, capacity, w, total, status);
+    hipLaunchKernelGGL(wzp_directory_kernel, dim3((ws.groups + 255) / 256), dim3(256), 0, s, out, ws.chunks,
+                       ws.groups, w, capacity);
+}
+
+__global__ void __launch_bounds__(256) wzp_decode_kernel(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t n,
+                                                         uint32_t chunks, uint32_t groups,
+                                                         uint16_t* __restrict__ mag, uint32_t* __restrict__ sign,
+                                                         uint32_t* __restrict__ status) {
+    __shared__ uint32_t tr[kChunk / 2];
+    __shared__ uint32_t b1w[kThreads / 4];  // B1 as 64 words (256 bytes)
+    __shared__ uint32_t b2w[8];             // B2 as 8 words (32 bytes)
+    __shared__ uint32_t sh[8];
+    __shared__ uint32_t info[4];            // offset, ok
+
+    const uint32_t t = threadIdx.x;
+    const uint32_t c = blockIdx.x;
+    const uint32_t E = chunk_elements(n, c);
+    const uint32_t first = t * 32;
+    const uint64_t header = uint64_t(groups) * 4 + uint64_t(chunks) * 3;
+    const uint8_t* meta = in + uint64_t(groups) * 4;
+    const uint32_t mc = load24(meta + uint64_t(c) * 3);
+    const uint32_t code = mc & kRawCode, sc = mc >> 14;
+    const uint32_t raw_bytes = 2 * E;
+    const bool raw = code == kRawCode;
+    const uint32_t mbytes = code == 0 ? 0u : raw ? raw_bytes : code;
+    const uint32_t sbytes = code == 0 ? 0u : sc ? sc : kSignSlot;
+    if (t == 0) {
+        const uint32_t g = c / kGroup;
+        uint64_t off = load32(in + uint64_t(g) * 4);
+        uint32_t ok = 1;
+        for (uint32_t j = g * kGroup; j < c; ++j) {
+            const uint32_t mj = load24(meta + uint64_t(j) * 3);
+            const uint32_t cj = mj & kRawCode, sj = mj >> 14;
+            if (!cj) continue;
+            off += (cj == kRawCode ? 2 * chunk_elements(n, j) : cj) + (sj ? sj : kSignSlot);
+        }
+        if (code == 0 && sc != 0) ok = 0;
+        if (code && !raw && (code < 4 || code + 2 >= raw_bytes)) ok = 0;
+        if (code && sbytes > (E + 7) / 8) ok = 0;
+        if (header + off + mbytes + sbytes > bytes) ok = 0;
+        info[0] = uint32_t(off);
+        info[1] = ok;
+        if (!ok) atomicOr(status, uint32_t(kCorrupt));
+    }
+    __syncthreads();
+    if (!info[1]) return;
+    uint16_t* mdst = mag + uint64_t(c) * kChunk;
+    const uint64_t sword = uint64_t(c) * (kChunk / 32) + t;
+    if (code == 0) {
+        for (int i = 0; i < 32; ++i)
+            if (first + i < E) mdst[first + i] = 0;
+        if (first < E) sign[sword] = 0;
+        return;
+    }
+    const uint8_t* src = in + header + info[0];
+    uint32_t m[32];
+    bool bad = false;
+    if (raw) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            m[i] = first + i < E ? uint32_t(src[2 * (first + i)]) | (uint32_t(src[2 * (first + i) + 1]) << 8) : 0u;
+    } else {
+        // top-down: B3 -> B2 -> B1 -> B0 -> bytes, each level's changed bytes
+        // located by popcount ranks (cpu/src/wzp_cpu.cpp:595-660)
+        const uint32_t b3 = load32(src + code - 4);
+        const uint32_t K3 = popc32(b3);
+        const int32_t pos3 = int32_t(code) - 4 - int32_t(K3);
+        if (pos3 < 0) bad = true;
+        if (!bad && t < 32) {
+            const uint32_t r = popc32(t == 31 ? b3 : b3 & ((2u << t) - 1u));
+            const uint32_t v = r ? uint32_t(src[pos3 + int32_t(r) - 1]) : 0u;
+            reinterpret_cast<uint8_t*>(b2w)[t] = uint8_t(v);
+        }
+        __syncthreads();
+        uint32_t K2 = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) K2 += popc32(b2w[i]);
+        const int32_t pos2 = pos3 - int32_t(K2);
+        if (pos2 < 0) bad = true;
+        if (!bad) {
+            const uint32_t wd = t >> 5, bit = t & 31u;
+            uint32_t r = popc32(bit == 31 ? b2w[wd] : b2w[wd] & ((2u << bit) - 1u));
+            for (uint32_t i = 0; i < wd; ++i) r += popc32(b2w[i]);
+            const uint32_t v = r ? uint32_t(src[pos2 + int32_t(r) - 1]) : 0u;
+            reinterpret_cast<uint8_t*>(b1w)[t] = uint8_t(v);
+        }
+        __syncthreads();
+        const uint32_t b1t = reinterpret_cast<const uint8_t*>(b1w)[t];
+        uint32_t K1;
+        const uint32_t e1 = block_exclusive(popc32(b1t), sh, K1);
+        const int32_t pos1 = pos2 - int32_t(K1);
+        if (pos1 < 0) bad = true;
+        unsigned long long b0t = 0;
+        if (!bad) {
+            uint32_t r = e1;
+            uint32_t cur = r ? uint32_t(src[pos1 + int32_t(r) - 1]) : 0u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if ((b1t >> k) & 1u) cur = src[pos1 + int32_t(r++)];
+                b0t |= (unsigned long long)(cur & 0xFFu) << (8 * k);
+            }
+        }
+        {
+            const int32_t valid = int32_t(raw_bytes) - int32_t(64 * t);
+            const unsigned long long keep = valid <= 0 ? 0ull : valid >= 64 ? ~0ull : (1ull << valid) - 1ull;
+            if (b0t & ~keep) bad = true;
+        }
+        uint32_t K0;
+        const uint32_t e0 = block_exclusive(popc64(b0t), sh, K0);
+        if (pos1 - int32_t(K0) != 0) bad = true;
+        if (!bad) {
+            uint32_t r = e0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((b0t >> (4 * q + k)) & 1ull) word |= uint32_t(src[r++]) << (8 * k);
+                tr[16 * t + q] = word;
+            }
+        }
+        __syncthreads();
+        const uint16_t* tr16 = reinterpret_cast<const uint16_t*>(tr);
+        const uint32_t blocks = E / 16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t b = 2 * t + h;
+            if (b < blocks) {
+                uint32_t p[16];
+#pragma unroll
+                for (int pl = 0; pl < 16; ++pl) p[pl] = tr16[pl * blocks + b];
+                bit2_inverse(p);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) m[16 * h + j] = p[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const uint32_t i = first + 16 * h + j;
+                    m[16 * h + j] = i < E ? uint32_t(tr16[i]) : 0u;
+                }
+            }
+        }
+    }
+    uint32_t nzmask = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) nzmask |= uint32_t(m[i] != 0) << i;
+    uint32_t NNZ;
+    const uint32_t es = block_exclusive(popc32(nzmask), sh, NNZ);
+    if (NNZ == 0 || (NNZ + 7) / 8 != sbytes) bad = true;
+    if (bad) {
+        if (t == 0 || (threadIdx.x & 63u) == 0) atomicOr(status, uint32_t(kCorrupt));
+        return;
+    }
+    const uint8_t* ps = src + mbytes;
+    const uint32_t cnt = popc32(nzmask);
+    uint32_t bits = 0;
+    if (cnt) {
+        const uint32_t b = es >> 3, sh8 = es & 7u;
+        const uint32_t need = (sh8 + cnt + 7) >> 3;  // <= 5 bytes
+        unsigned long long v = 0;
+        for (uint32_t i = 0; i < need; ++i) v |= (unsigned long long)ps[b + i] << (8 * i);
+        bits = uint32_t(v >> sh8) & (cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u));
+    }
+    if (t == 0 && (NNZ & 7u) && (ps[sbytes - 1] >> (NNZ & 7u))) atomicOr(status, uint32_t(kCorrupt));
+    if (first < E) sign[sword] = expand_bits(bits, nzmask);
+#pragma unroll
+    for (int i = 0; i < 32; ++i)
+        if (first + i < E) mdst[first + i] = uint16_t(m[i]);
+}
+
+void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
+                hipStream_t s) {
+    const WzpShape ws = wzp_shape(n);
+    if (ws.chunks == 0) return;
+    hipLaunchKernelGGL(wzp_decode_kernel, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n, ws.chunks, ws.groups,
+                       mag, sign, status);
+}
+
+}  // namespace wz

@@ -1,0 +1,66 @@
+"""Single pipeline stages on device tensors (for callers that drive one stage,
+and for stage-level parity tests). Shapes are (z, y, x)."""
+import ctypes as C
+
+import torch
+
+from ._lib import check, lib
+from .codec import _stream
+
+_W = {torch.float32: 4, torch.float64: 8}
+
+
+def _dims(t):
+    nz, ny, nx = t.shape
+    return nx, ny, nz
+
+
+def transform(volume, levels_xy, levels_z, dyadic, inverse, out=None):
+    """CDF 9/7 lifting (forward: samples -> coefficients; inverse: back)."""
+    volume = volume.contiguous()
+    out = torch.empty_like(volume) if out is None else out
+    nx, ny, nz = _dims(volume)
+    check(lib().waltz_amd_transform(_W[volume.dtype], int(inverse), volume.data_ptr(), out.data_ptr(), nx, ny, nz,
+                                    levels_xy, levels_z, int(dyadic), _stream(volume.device)), "transform")
+    return out
+
+
+def quantize(coeff, bz, quantum):
+    """Returns (dequantized, mag u16 as int16 view, sign words as int32, q indices, q values).
+
+    q-outliers come back in device order (unsorted); the archive path orders them.
+    """
+    c = coeff.contiguous().clone()
+    n = c.numel()
+    nx, ny, nz = _dims(c)
+    dev = c.device
+    mag = torch.empty(n, dtype=torch.int16, device=dev)
+    sign = torch.empty((n + 31) // 32, dtype=torch.int32, device=dev)
+    qidx = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    qval = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    nq = C.c_uint32()
+    check(lib().waltz_amd_quantize(_W[c.dtype], c.data_ptr(), nx, ny, nz, bz, float(quantum), mag.data_ptr(),
+                                   sign.data_ptr(), qidx.data_ptr(), qval.data_ptr(), C.byref(nq), _stream(dev)),
+          "quantize")
+    return c, mag, sign, qidx[:nq.value], qval[:nq.value]
+
+
+def wzp_encode(mag, sign):
+    """WZP stream of packed u16 magnitudes (int16 tensor) and sign words (int32)."""
+    n = mag.numel()
+    chunks = (n + 8191) // 8192
+    cap = ((chunks + 15) // 16) * 4 + chunks * 3 + 2 * n + (n + 7) // 8
+    out = torch.zeros(max(cap, 1), dtype=torch.uint8, device=mag.device)
+    written = C.c_size_t()
+    check(lib().waltz_amd_wzp_encode(mag.data_ptr(), sign.data_ptr(), n, out.data_ptr(), cap, C.byref(written),
+                                     _stream(mag.device)), "wzp_encode")
+    return out[:written.value]
+
+
+def wzp_decode(stream, n):
+    dev = stream.device
+    mag = torch.empty(max(n, 1), dtype=torch.int16, device=dev)
+    sign = torch.zeros(max((n + 31) // 32, 1), dtype=torch.int32, device=dev)
+    check(lib().waltz_amd_wzp_decode(stream.data_ptr(), stream.numel(), n, mag.data_ptr(), sign.data_ptr(),
+                                     _stream(dev)), "wzp_decode")
+    return mag[:n], sign[:(n + 31) // 32]
