@@ -74,9 +74,11 @@ def profile(enable):
 def profile_report():
     """{stage: (total_ms, launches)} since the last report (then reset)."""
     import json
-    need = lib().waltz_amd_profile_report(None, 0)
-    buf = C.create_string_buffer(need + 1)
-    lib().waltz_amd_profile_report(buf, need + 1)
+    cap = 1 << 16
+    buf = C.create_string_buffer(cap)
+    need = lib().waltz_amd_profile_report(buf, cap)  # the call also resets the totals
+    if need >= cap:
+        raise WaltzError("profile report truncated")
     return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
 
 

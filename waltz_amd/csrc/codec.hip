@@ -141,7 +141,6 @@ Geometry geometry(Dims d, unsigned width) {
 }
 
 uint32_t qcap(uint32_t n) { return n <= (1u << 20) ? n : (1u << 20); }
-uint32_t rcap(uint32_t n) { return n <= (1u << 22) ? n : (1u << 22); }
 
 }  // namespace
 
@@ -167,7 +166,8 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     const uint32_t n = g.n;
     const uint64_t words = (uint64_t(n) + 31) / 32, segs = (uint64_t(n) + 1023) / 1024;
     const WzpShape ws = wzp_shape(n);
-    const uint32_t nq = qcap(n), nr = rcap(n);
+    const uint32_t nq = qcap(n);
+    const uint32_t rtiles = (n + kRepairTile - 1) / kRepairTile;
 
     DeviceWorkspace& w = workspace();
     std::lock_guard<std::mutex> guard(w.lock);
@@ -177,7 +177,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         struct P {
             T *coeff, *t0, *t1;
             uint16_t* mag;
-            uint32_t *sign, *qbm, *rbm, *qidx, *qval, *ridx, *seg, *zeroed;
+            uint32_t *sign, *qbm, *qidx, *qval, *ridx, *rcount, *seg, *zeroed;
             T* rerr;
             unsigned long long* desc;
             uint32_t *excl, *size, *counters;
@@ -186,12 +186,12 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.t0 = c.take<T>(n);
         p.t1 = c.take<T>(n);
         p.mag = c.take<uint16_t>(n);
-        p.ridx = c.take<uint32_t>(nr);
-        p.rerr = c.take<T>(nr);
+        p.ridx = c.take<uint32_t>(n);
+        p.rerr = c.take<T>(n);
+        p.rcount = c.take<uint32_t>(rtiles);
         p.qidx = c.take<uint32_t>(nq);
         p.qval = c.take<uint32_t>(nq);
-        p.rbm = c.take<uint32_t>(words);
-        p.seg = c.take<uint32_t>(2 * segs);
+        p.seg = c.take<uint32_t>(segs);
         p.excl = c.take<uint32_t>(ws.chunks);
         p.size = c.take<uint32_t>(ws.chunks);
         // zero-initialised region: sign, q bitmap, lookback descriptors, counters
@@ -199,7 +199,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.sign = c.take<uint32_t>(words);
         p.qbm = c.take<uint32_t>(words);
         p.desc = c.take<unsigned long long>(ws.chunks);
-        p.counters = c.take<uint32_t>(4);  // ticket, q count, r count
+        p.counters = c.take<uint32_t>(4);  // ticket, q count, repair total
         p.zeroed = reinterpret_cast<uint32_t*>(c.base + z0);
         return std::make_pair(p, c.used - z0);
     };
@@ -227,7 +227,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         wzp_encode(p.mag, p.sign, n, archive + 32, capacity - 32, wsc, &res->wzp_bytes, status, s);
     }
     transform_inverse<T>(p.coeff, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s);
-    SideList rp{p.rbm, p.ridx, p.rerr, p.counters + 2, nr};
+    TileList rp{p.ridx, p.rerr, p.rcount, p.counters + 2};
     {
         StageClock c("repairs", s);
         detect_repairs<T>(input, p.coeff, n, abs_bound, rp, status, s);

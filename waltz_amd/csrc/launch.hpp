@@ -66,8 +66,15 @@ void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, ui
                 hipStream_t s);
 
 // sections.hip
+constexpr uint32_t kRepairTile = 4096;  // row-major indices per repair tile
+struct TileList {        // per-tile ordered segments (tile k at k*kRepairTile)
+    uint32_t* index;     // n entries
+    void* payload;       // n entries of payload type
+    uint32_t* count;     // ceil(n/kRepairTile) counts, scanned in place to prefixes
+    uint32_t* total;     // device total after the scan
+};
 template <class T>
-void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, SideList r,
+void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, TileList r,
                     uint32_t* status, hipStream_t s);
 struct EncodeResult {
     unsigned long long wzp_bytes;  // written by wzp_encode
@@ -75,9 +82,9 @@ struct EncodeResult {
     uint32_t np, nq, status, pad;
 };
 // Orders both side lists by index into the archive sections and writes the
-// 32-byte header and all padding. `seg` needs ceil(n/1024)*2 u32 of scratch.
+// 32-byte header and all padding. `seg` needs ceil(n/1024) u32 of scratch.
 template <class T>
-void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SideList repairs, SideList qout,
+void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, TileList repairs, SideList qout,
                     uint32_t* seg, uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum,
                     EncodeResult* result, hipStream_t s);
 template <class T>

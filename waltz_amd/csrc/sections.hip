@@ -1,13 +1,16 @@
 // Side channels (native repairs, quantization outliers) and archive assembly.
 //
-// Producers append (index, payload) to an unsorted list with wave-aggregated
-// atomics and set the index's bit in a row-major flag bitmap. The canonical
-// archive needs both lists in increasing index order (cpu/src/codec.cpp:225-257);
-// since indices are unique, an entry's sorted position is simply the number of
-// flagged indices below it. finish_archive computes that rank from per-1024-bit
-// segment popcounts (one exclusive scan) plus in-segment word popcounts and
-// scatters every entry straight into its archive section: O(n) work, no sort,
-// and the same bytes as a comparison sort.
+// The canonical archive stores both side channels in increasing index order
+// (cpu/src/codec.cpp:225-257). Two producers, two atomic-free orderings:
+//  * repairs come from a row-major pass: each workgroup owns one contiguous
+//    tile of 4096 indices, compacts its repairs in order into the tile's own
+//    segment (block scan, no global atomics), and an exclusive scan over the
+//    tile counts gives every entry its final position;
+//  * q-outliers come from Morton bricks (not row-major): each sets its bit in a
+//    row-major flag bitmap and is appended to an unsorted list; since indices
+//    are unique, an entry's sorted position is the number of flagged indices
+//    below it (per-1024-bit segment popcounts + one exclusive scan + in-segment
+//    popcounts). O(n), no sort, same bytes as a comparison sort.
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -16,15 +19,30 @@ namespace wz {
 namespace {
 WZ_D uint32_t lane_id() { return threadIdx.x & 63u; }
 
-// Reserve `want` slots per lane with one atomic per wave.
-WZ_D uint32_t wave_reserve(bool want, uint32_t* counter) {
-    const unsigned long long mask = __ballot(want);
-    if (!mask) return 0;
-    const uint32_t leader = uint32_t(__ffsll((long long)mask) - 1);
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    base = __shfl(base, int(leader), 64);
-    return base + uint32_t(__popcll(mask & ((1ull << lane_id()) - 1ull)));
+WZ_D uint32_t wave_inclusive(uint32_t v) {
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane_id() >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; `sh` >= 4 words.
+WZ_D uint32_t block_exclusive256(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const uint32_t inc = wave_inclusive(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 63) sh[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        before += i < w ? sh[i] : 0u;
+        sum += sh[i];
+    }
+    __syncthreads();
+    total = sum;
+    return before + inc - v;
 }
 
 // Exact |value - original| <= bound with a TwoDiff tie check
@@ -39,56 +57,76 @@ WZ_D bool within_bound(double value, double original, double bound) {
 }
 }  // namespace
 
-// Native repairs (cpu/src/repair_avx2.cpp:58-74). Each wave covers 64
-// consecutive indices starting at a multiple of 64, so it owns two bitmap
-// words and writes them without atomics.
+// Native repairs (cpu/src/repair_avx2.cpp:58-74) over row-major tiles of
+// kRepairTile indices. Round k of tile t covers indices t*4096 + 256k + [0,256)
+// (one coalesced load per wave per round); entries are ordered round, wave,
+// lane — i.e. by index — through ballot ranks and a 64-entry scan in LDS.
 template <class T>
 __global__ void __launch_bounds__(256) repair_kernel(const T* __restrict__ original, const T* __restrict__ rec,
-                                                     uint32_t n, double bound, SideList r,
+                                                     uint32_t n, double bound, TileList r,
                                                      uint32_t* __restrict__ status) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    const bool active = i < n;
-    bool patch = false, ok = true;
-    T err = T(0);
-    if (active) {
-        T value = rec[i];
-        const T o = original[i];
-        err = value - o;
-        if (!__builtin_isfinite(value) || !__builtin_isfinite(err)) ok = false;
-        if (__builtin_fabs(err) > T(bound)) {
-            patch = true;
-            value = value - err;
+    constexpr uint32_t rounds = kRepairTile / 256;
+    __shared__ uint32_t cnt[rounds * 4];
+    const uint64_t tile0 = uint64_t(blockIdx.x) * kRepairTile;
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    T err[rounds];
+    unsigned long long ballots[rounds];
+    bool ok = true;
+    const T nb = T(bound);
+#pragma unroll
+    for (uint32_t k = 0; k < rounds; ++k) {
+        const uint64_t i = tile0 + 256u * k + threadIdx.x;
+        bool patch = false;
+        err[k] = T(0);
+        if (i < n) {
+            T value = rec[i];
+            const T o = original[i];
+            const T e = value - o;
+            if (!__builtin_isfinite(value) || !__builtin_isfinite(e)) ok = false;
+            if (__builtin_fabs(e) > nb) {
+                patch = true;
+                err[k] = e;
+                value = value - e;
+            }
+            if (!__builtin_isfinite(value) || !within_bound(double(value), double(o), bound)) ok = false;
         }
-        if (!__builtin_isfinite(value) || !within_bound(double(value), double(o), bound)) ok = false;
+        ballots[k] = __ballot(patch);
+        if (lane == 0) cnt[4 * k + wv] = uint32_t(__popcll(ballots[k]));
     }
-    const unsigned long long mask = __ballot(patch);
-    if (lane_id() == 0 && i < n) r.bitmap[i >> 5] = uint32_t(mask);
-    if (lane_id() == 32 && i < n) r.bitmap[i >> 5] = uint32_t(mask >> 32);
-    const uint32_t slot = wave_reserve(patch, r.count);
-    if (patch && slot < r.capacity) {
-        r.index[slot] = i;
-        static_cast<T*>(r.payload)[slot] = err;
+    __syncthreads();
+    if (wv == 0) {  // exclusive scan of the 64 (round, wave) counts, in index order
+        const uint32_t v = cnt[lane];
+        const uint32_t inc = wave_inclusive(v);
+        cnt[lane] = inc - v;
+        if (lane == 63) r.count[blockIdx.x] = inc;
     }
-    if (__ballot(!ok) && lane_id() == 0) atomicOr(status, uint32_t(kRepairBound));
+    __syncthreads();
+    uint32_t* idx = r.index + tile0;
+    T* val = static_cast<T*>(r.payload) + tile0;
+#pragma unroll
+    for (uint32_t k = 0; k < rounds; ++k)
+        if ((ballots[k] >> lane) & 1ull) {
+            const uint32_t at = cnt[4 * k + wv] + uint32_t(__popcll(ballots[k] & below));
+            idx[at] = uint32_t(tile0 + 256u * k + threadIdx.x);
+            val[at] = err[k];
+        }
+    if (__ballot(!ok) && lane == 0) atomicOr(status, uint32_t(kRepairBound));
 }
 
 template <class T>
-void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, SideList r,
+void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, TileList r,
                     uint32_t* status, hipStream_t s) {
-    hipLaunchKernelGGL((repair_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, s, original, reconstructed, n, bound,
-                       r, status);
+    const uint32_t tiles = (n + kRepairTile - 1) / kRepairTile;
+    hipLaunchKernelGGL((repair_kernel<T>), dim3(tiles), dim3(256), 0, s, original, reconstructed, n, bound, r,
+                       status);
 }
 
 // ------------------------------------------------------------------ ordering
-// seg[k] = popcount of bitmap bits [1024k, 1024k+1024) for list `blockIdx.y`.
-__global__ void __launch_bounds__(256) segment_count(const uint32_t* __restrict__ bm0,
-                                                     const uint32_t* __restrict__ bm1, uint32_t words,
-                                                     uint32_t segs, uint32_t* __restrict__ seg0,
-                                                     uint32_t* __restrict__ seg1, const uint32_t* cnt0,
-                                                     const uint32_t* cnt1) {
-    const uint32_t* bm = blockIdx.y ? bm1 : bm0;
-    uint32_t* seg = blockIdx.y ? seg1 : seg0;
-    if (*(blockIdx.y ? cnt1 : cnt0) == 0) return;
+// seg[k] = popcount of q-outlier bitmap bits [1024k, 1024k+1024).
+__global__ void __launch_bounds__(256) segment_count(const uint32_t* __restrict__ bm, uint32_t words, uint32_t segs,
+                                                     uint32_t* __restrict__ seg, const uint32_t* __restrict__ cnt) {
+    if (*cnt == 0) return;
     const uint32_t k = blockIdx.x * 8 + (threadIdx.x >> 5);  // 32 lanes per segment
     if (k >= segs) return;
     const uint32_t w = k * 32 + (threadIdx.x & 31u);
@@ -98,25 +136,20 @@ __global__ void __launch_bounds__(256) segment_count(const uint32_t* __restrict_
     if ((threadIdx.x & 31u) == 0) seg[k] = c;
 }
 
-// In-place exclusive scan of seg[0..segs) for list blockIdx.x (one workgroup each).
-__global__ void __launch_bounds__(1024) segment_scan(uint32_t* seg0, uint32_t* seg1, uint32_t segs, const uint32_t* cnt0,
-                                                     const uint32_t* cnt1) {
-    uint32_t* seg = blockIdx.x ? seg1 : seg0;
-    if (*(blockIdx.x ? cnt1 : cnt0) == 0) return;
+// In-place exclusive scan of a[0..len) by one 1024-thread workgroup; the sum
+// goes to *total. Skipped (a untouched, *total untouched) when *gate == 0.
+__global__ void __launch_bounds__(1024) exclusive_scan(uint32_t* a, uint32_t len, uint32_t* total,
+                                                       const uint32_t* gate) {
+    if (gate && *gate == 0) return;
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry_sh;
     if (threadIdx.x == 0) carry_sh = 0;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < segs; base += 1024) {
+    for (uint32_t base = 0; base < len; base += 1024) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < segs ? seg[i] : 0u;
-        uint32_t inc = v;
-#pragma unroll
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t t = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += t;
-        }
+        const uint32_t v = i < len ? a[i] : 0u;
+        const uint32_t inc = wave_inclusive(v);
         if (lane == 63) wsum[wv] = inc;
         __syncthreads();
         uint32_t before = carry_sh, all = 0;
@@ -124,21 +157,21 @@ __global__ void __launch_bounds__(1024) segment_scan(uint32_t* seg0, uint32_t* s
             before += k < wv ? wsum[k] : 0u;
             all += wsum[k];
         }
-        if (i < segs) seg[i] = before + inc - v;
+        if (i < len) a[i] = before + inc - v;
         __syncthreads();
         if (threadIdx.x == 0) carry_sh += all;
         __syncthreads();
     }
+    if (threadIdx.x == 0 && total) *total = carry_sh;
 }
 
 template <class T>
-__global__ void layout_kernel(uint8_t* __restrict__ archive, uint64_t capacity, uint32_t n, SideList rp, SideList qo,
-                              uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum,
-                              EncodeResult* __restrict__ res, uint32_t* __restrict__ status) {
-    const uint32_t np = *rp.count, nq = *qo.count;
-    uint32_t st = *status;
-    if (np > rp.capacity || nq > qo.capacity || (n > (1u << 20) && nq >= (1u << 20)) ||
-        (n > (1u << 22) && np >= (1u << 22)))
+__global__ void layout_kernel(uint8_t* __restrict__ archive, uint64_t capacity, uint32_t n,
+                              const uint32_t* __restrict__ np_dev, SideList qo, uint8_t flags, uint8_t lx, uint8_t lz,
+                              uint8_t bz, double quantum, EncodeResult* __restrict__ res) {
+    const uint32_t np = *np_dev, nq = *qo.count;
+    uint32_t st = res->status;
+    if (nq > qo.capacity || (n > (1u << 20) && nq >= (1u << 20)) || (n > (1u << 22) && np >= (1u << 22)))
         st |= kSideCapacity;
     const uint64_t wb = res->wzp_bytes;
     const Layout L = archive_layout(wb, np, uint64_t(np) * sizeof(T), nq);
@@ -147,7 +180,6 @@ __global__ void layout_kernel(uint8_t* __restrict__ archive, uint64_t capacity, 
     res->nq = nq;
     res->end = L.end;
     res->status = st;
-    *status = st;
     if (st) return;
     uint8_t h[32];
     h[0] = flags;
@@ -177,42 +209,59 @@ WZ_D uint32_t sorted_position(const uint32_t* __restrict__ bm, const uint32_t* _
     return pos + uint32_t(__builtin_popcount(bm[w] & ((1u << (idx & 31u)) - 1u)));
 }
 
+// Repairs: tile k's entries go to [prefix[k], prefix[k]+count) (counts were
+// scanned in place, so count = next prefix - prefix).
 template <class T>
-__global__ void __launch_bounds__(256) scatter_kernel(uint8_t* __restrict__ archive, SideList rp, SideList qo,
-                                                      const uint32_t* __restrict__ seg0,
-                                                      const uint32_t* __restrict__ seg1,
-                                                      const EncodeResult* __restrict__ res) {
+__global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archive, TileList rp, uint32_t tiles,
+                                                     const EncodeResult* __restrict__ res) {
+    if (res->status) return;
+    const uint32_t k = blockIdx.x;
+    const uint32_t np = res->np;
+    const uint32_t first = rp.count[k], last = k + 1 < tiles ? rp.count[k + 1] : np;
+    if (first == last) return;
+    const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * sizeof(T), res->nq);
+    const uint32_t* idx = rp.index + uint64_t(k) * kRepairTile;
+    const T* val = static_cast<const T*>(rp.payload) + uint64_t(k) * kRepairTile;
+    uint32_t* oi = reinterpret_cast<uint32_t*>(archive + L.pi);
+    T* ov = reinterpret_cast<T*>(archive + L.pv);
+    for (uint32_t j = threadIdx.x; j < last - first; j += 256) {
+        oi[first + j] = idx[j];
+        ov[first + j] = val[j];
+    }
+}
+
+__global__ void __launch_bounds__(256) place_qoutliers(uint8_t* __restrict__ archive, SideList qo,
+                                                       const uint32_t* __restrict__ seg, uint64_t repair_bytes_per,
+                                                       const EncodeResult* __restrict__ res) {
     if (res->status) return;
     const uint32_t np = res->np, nq = res->nq;
-    const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * sizeof(T), nq);
+    if (!nq) return;
+    const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * repair_bytes_per, nq);
     const uint32_t stride = gridDim.x * 256u;
-    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < np; j += stride) {
-        const uint32_t idx = rp.index[j];
-        const uint32_t pos = sorted_position(rp.bitmap, seg0, idx);
-        reinterpret_cast<uint32_t*>(archive + L.pi)[pos] = idx;
-        reinterpret_cast<T*>(archive + L.pv)[pos] = static_cast<const T*>(rp.payload)[j];
-    }
     for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < nq; j += stride) {
         const uint32_t idx = qo.index[j];
-        const uint32_t pos = sorted_position(qo.bitmap, seg1, idx);
+        const uint32_t pos = sorted_position(qo.bitmap, seg, idx);
         reinterpret_cast<uint32_t*>(archive + L.qi)[pos] = idx;
         reinterpret_cast<uint32_t*>(archive + L.qv)[pos] = static_cast<const uint32_t*>(qo.payload)[j];
     }
 }
 
 template <class T>
-void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SideList repairs, SideList qout, uint32_t* seg,
+void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, TileList repairs, SideList qout, uint32_t* seg,
                     uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum, EncodeResult* result,
                     hipStream_t s) {
     const uint32_t words = (n + 31) / 32, segs = (n + 1023) / 1024;
-    uint32_t* seg0 = seg;
-    uint32_t* seg1 = seg + segs;
-    hipLaunchKernelGGL(segment_count, dim3((segs + 7) / 8, 2), dim3(256), 0, s, repairs.bitmap, qout.bitmap, words,
-                       segs, seg0, seg1, repairs.count, qout.count);
-    hipLaunchKernelGGL(segment_scan, dim3(2), dim3(1024), 0, s, seg0, seg1, segs, repairs.count, qout.count);
-    hipLaunchKernelGGL((layout_kernel<T>), dim3(1), dim3(1), 0, s, archive, capacity, n, repairs, qout, flags, lx, lz,
-                       bz, quantum, result, &result->status);
-    hipLaunchKernelGGL((scatter_kernel<T>), dim3(1024), dim3(256), 0, s, archive, repairs, qout, seg0, seg1, result);
+    const uint32_t tiles = (n + kRepairTile - 1) / kRepairTile;
+    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, repairs.count, tiles, repairs.total,
+                       static_cast<const uint32_t*>(nullptr));
+    hipLaunchKernelGGL(segment_count, dim3((segs + 7) / 8), dim3(256), 0, s, qout.bitmap, words, segs, seg,
+                       qout.count);
+    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, seg, segs, static_cast<uint32_t*>(nullptr),
+                       qout.count);
+    hipLaunchKernelGGL((layout_kernel<T>), dim3(1), dim3(1), 0, s, archive, capacity, n, repairs.total, qout, flags,
+                       lx, lz, bz, quantum, result);
+    hipLaunchKernelGGL((place_repairs<T>), dim3(tiles), dim3(256), 0, s, archive, repairs, tiles, result);
+    hipLaunchKernelGGL(place_qoutliers, dim3(256), dim3(256), 0, s, archive, qout, seg, uint64_t(sizeof(T)), result);
 }
 
 // ------------------------------------------------------------------ decode patches
@@ -274,8 +323,8 @@ void check_finite(const T* out, uint32_t n, uint32_t* status, hipStream_t s) {
 }
 
 #define WZ_INST(T)                                                                                                \
-    template void detect_repairs<T>(const T*, const T*, uint32_t, double, SideList, uint32_t*, hipStream_t);        \
-    template void finish_archive<T>(uint8_t*, uint64_t, uint32_t, SideList, SideList, uint32_t*, uint8_t, uint8_t, \
+    template void detect_repairs<T>(const T*, const T*, uint32_t, double, TileList, uint32_t*, hipStream_t);        \
+    template void finish_archive<T>(uint8_t*, uint64_t, uint32_t, TileList, SideList, uint32_t*, uint8_t, uint8_t, \
                                     uint8_t, uint8_t, double, EncodeResult*, hipStream_t);                         \
     template void apply_qoutliers<T>(T*, const uint8_t*, const uint8_t*, uint32_t, uint32_t, double, uint32_t*,    \
                                      hipStream_t);                                                                \

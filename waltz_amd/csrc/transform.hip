@@ -15,7 +15,10 @@
 // Y and Z lines map consecutive lanes to consecutive x, so every register load
 // is a fully coalesced 256 B (f32) / 512 B (f64) wave access; X lines map lanes
 // to consecutive tiles of one row (overlapping 64 B windows served from L1).
+#include <cstdlib>
+
 #include "common.hpp"
+#include "fused.hpp"
 #include "launch.hpp"
 #include "profile.hpp"
 
@@ -208,6 +211,108 @@ struct Passes {
 };
 }  // namespace
 
+// ------------------------------------------------------------------ fused dyadic schedule
+namespace {
+bool fused_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("WALTZ_FUSED");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// z segment per workgroup: enough segments for ~1024 workgroups, a multiple of
+// 16 (the canonical f32 inverse Z tile, and twice the f64 one). Bit-neutral.
+uint32_t z_segment(Dims n) {
+    const uint64_t tiles = uint64_t((n.nx + 63) / 64) * ((n.ny + 31) / 32);
+    uint64_t segs = (1024 + tiles - 1) / tiles;
+    if (segs < 1) segs = 1;
+    uint64_t sz = (n.nz + segs - 1) / segs;
+    sz = (sz + 15) & ~uint64_t(15);
+    return uint32_t(sz < 16 ? 16 : sz);
+}
+
+// Compact boxes of levels >= 1 live in `scratch`: level l's box at offset 0
+// for odd l and after the largest level-1 box for even l (two never-aliasing
+// halves, each <= n/8 + lines).
+template <class T>
+T* low_box(T* scratch, Dims full, unsigned l) {
+    const Dims h{half_len(full.nx), half_len(full.ny), half_len(full.nz)};
+    const uint64_t first = uint64_t(h.nx) * h.ny * h.nz;
+    return (l & 1u) ? scratch : scratch + ((first + 63) & ~uint64_t(63));
+}
+}  // namespace
+
+template <class T>
+void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels, uint32_t* status, hipStream_t s) {
+    const T* src = input;
+    uint64_t spy = full.nx, spz = uint64_t(full.nx) * full.ny;
+    Dims n = full;
+    static const char* names[] = {"forward_L0", "forward_L1", "forward_L2", "forward_L3", "forward_L4"};
+    for (unsigned l = 0; l < levels; ++l) {
+        StageClock clock(names[l], s);
+        const Dims h{half_len(n.nx), half_len(n.ny), half_len(n.nz)};
+        FwdLevel<T> a;
+        a.src = src;
+        a.spy = spy;
+        a.spz = spz;
+        a.n = n;
+        a.hi = coeff;
+        a.hpy = full.nx;
+        a.hpz = uint64_t(full.nx) * full.ny;
+        if (l + 1 == levels) {
+            a.lo = coeff;
+            a.lpy = a.hpy;
+            a.lpz = a.hpz;
+        } else {
+            a.lo = low_box(tmp0, full, l + 1);
+            a.lpy = h.nx;
+            a.lpz = uint64_t(h.nx) * h.ny;
+        }
+        a.sz = z_segment(n);
+        a.status = l == 0 ? status : nullptr;
+        launch_fwd_level<T>(a, s);
+        src = a.lo;
+        spy = a.lpy;
+        spz = a.lpz;
+        n = h;
+    }
+}
+
+template <class T>
+void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, hipStream_t s) {
+    Dims sizes[5], n = full;
+    for (unsigned l = 0; l < levels; ++l) { sizes[l] = n; n = shrink(n, true, true); }
+    const T* lo = coeff;
+    uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
+    static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
+    for (int l = int(levels) - 1; l >= 0; --l) {
+        StageClock clock(names[l], s);
+        InvLevel<T> a;
+        a.hi = coeff;
+        a.hpy = full.nx;
+        a.hpz = uint64_t(full.nx) * full.ny;
+        a.lo = lo;
+        a.lpy = lpy;
+        a.lpz = lpz;
+        a.n = sizes[l];
+        if (l == 0) {
+            a.out = output;
+            a.opy = full.nx;
+            a.opz = uint64_t(full.nx) * full.ny;
+        } else {
+            a.out = low_box(tmp0, full, unsigned(l));
+            a.opy = sizes[l].nx;
+            a.opz = uint64_t(sizes[l].nx) * sizes[l].ny;
+        }
+        a.sz = z_segment(sizes[l]);
+        launch_inv_level<T>(a, s);
+        lo = a.out;
+        lpy = a.opy;
+        lpz = a.opz;
+    }
+}
+
 // Full multi-level transform (cpu/src/transform.cpp:409-456).
 // Forward: reads `input` (may equal coeff), leaves coefficients in `coeff`.
 // Inverse: reads coefficients in `coeff` (overwritten), writes the last pass
@@ -218,6 +323,14 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
     Passes<T, false> p{coeff, {tmp0, tmp1}, input, nullptr, status, s};
     if (lxy == 0 && lz == 0) {
         if (input != coeff) copy_box<T>(input, coeff, full, full, s);
+        return;
+    }
+    if (dyadic && fused_enabled()) {
+        if (input == coeff) {  // the level-0 kernel must not read what it writes
+            copy_box<T>(input, tmp1, full, full, s);
+            input = tmp1;
+        }
+        fused_forward<T>(input, coeff, tmp0, full, lxy, status, s);
         return;
     }
     if (dyadic) {
@@ -249,6 +362,15 @@ void transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
     Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s};
     if (lxy == 0 && lz == 0) {
         if (output != coeff) copy_box<T>(coeff, output, full, full, s);
+        return;
+    }
+    if (dyadic && fused_enabled()) {
+        if (output == coeff) {  // the level-0 kernel must not overwrite what it reads
+            fused_inverse<T>(coeff, tmp1, tmp0, full, lxy, s);
+            copy_box<T>(tmp1, coeff, full, full, s);
+        } else {
+            fused_inverse<T>(coeff, output, tmp0, full, lxy, s);
+        }
         return;
     }
     if (dyadic) {

@@ -10,7 +10,7 @@
 // bytes = 8 bytes of the zero bitmap B0 = one byte of B1. All counts are
 // packed two per 32-bit word into block-wide scans (wave shuffles + LDS).
 // The payload offset comes from a decoupled lookback over chunk descriptors
-// (aggregate / inclusive flags, agent-scope acquire/release), so every chunk
+// (aggregate / inclusive flags, relaxed agent-scope atomics), so every chunk
 // writes its bytes straight to its final place in logical order; a tiny
 // second kernel fills the per-group directory from the chunk prefixes.
 //
@@ -77,6 +77,7 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(const uint16_t* __restr
                                                          unsigned long long* __restrict__ total,
                                                          uint32_t* __restrict__ status) {
     __shared__ uint32_t tr[kChunk / 2];                // BIT_2 planes, 16 KiB
+    __shared__ uint32_t ob32[(kChunkBytes + kSignSlot) / 4 + 2];  // assembled payload
     __shared__ unsigned long long b0s[kThreads];       // B0, 2048 bytes
     __shared__ __attribute__((aligned(8))) uint8_t b1s[kThreads];
     __shared__ __attribute__((aligned(8))) uint8_t b2s[32];
@@ -195,37 +196,56 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(const uint16_t* __restr
         }
     }
 
-    // 5. decoupled lookback for this chunk's payload offset
-    if (t == 0) {
+    // 5. decoupled lookback for this chunk's payload offset: wave 0 reads 64
+    // predecessor descriptors per round trip, stops at the nearest inclusive
+    // prefix (ballot) and sums the aggregates up to it (wave reduction).
+    // Descriptors are the only data passed between chunks, so relaxed
+    // agent-scope atomics suffice: a release here would write back this XCD's
+    // whole L2 (full of freshly written payload) on every publish.
+    if (t < 64) {
+        const uint32_t lane = t;
+        if (lane == 0)
+            __hip_atomic_store(&w.desc[c], (c == 0 ? kFlagInclusive : kFlagAggregate) | chunk_bytes, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         unsigned long long acc = 0;
-        if (c == 0) {
-            __hip_atomic_store(&w.desc[0], kFlagInclusive | chunk_bytes, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&w.desc[c], kFlagAggregate | chunk_bytes, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (c > 0) {
             int64_t j = int64_t(c) - 1;
             uint32_t spins = 0;
-            while (j >= 0) {
-                const unsigned long long d = __hip_atomic_load(&w.desc[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            for (;;) {
+                const int64_t idx = j - int64_t(lane);
+                const unsigned long long d =
+                    idx >= 0 ? __hip_atomic_load(&w.desc[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : kFlagInclusive;  // nothing before chunk 0
                 const unsigned long long flag = d & ~kValueMask;
-                if (flag == 0) {
+                const unsigned long long incl = __ballot(flag == kFlagInclusive);
+                const unsigned long long waiting = __ballot(flag == 0);
+                const uint32_t stop = incl ? uint32_t(__ffsll((long long)incl) - 1) : 63u;
+                const unsigned long long need = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
+                if (waiting & need) {
                     if (++spins > kSpinLimit) {
-                        atomicOr(status, uint32_t(kLookbackTimeout));
+                        if (lane == 0) atomicOr(status, uint32_t(kLookbackTimeout));
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                acc += d & kValueMask;
-                if (flag == kFlagInclusive) break;
-                --j;
+                unsigned long long v = lane <= stop ? (d & kValueMask) : 0ull;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+                acc += v;
+                if (incl) break;
+                j -= 64;
             }
-            __hip_atomic_store(&w.desc[c], kFlagInclusive | (acc + chunk_bytes), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&w.desc[c], kFlagInclusive | (acc + chunk_bytes), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        w.excl[c] = uint32_t(acc);
-        w.size[c] = chunk_bytes;
-        if (c == chunks - 1) *total = uint64_t(groups) * 4 + uint64_t(chunks) * 3 + acc + chunk_bytes;
-        excl_sh = acc;
+        if (lane == 0) {
+            w.excl[c] = uint32_t(acc);
+            w.size[c] = chunk_bytes;
+            if (c == chunks - 1) *total = uint64_t(groups) * 4 + uint64_t(chunks) * 3 + acc + chunk_bytes;
+            excl_sh = acc;
+        }
     }
     __syncthreads();
 
@@ -245,59 +265,66 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(const uint16_t* __restr
         mp[2] = uint8_t(meta >> 16);
     }
     if (!NNZ) return;
-    uint8_t* dst = out + start;
+    // 7. assemble the chunk payload in LDS
+    uint8_t* ob = reinterpret_cast<uint8_t*>(ob32);
     if (raw) {
-        for (int i = 0; i < 32; ++i) {
-            if (first + i < E) {
-                dst[2 * (first + i)] = uint8_t(m[i]);
-                dst[2 * (first + i) + 1] = uint8_t(m[i] >> 8);
-            }
-        }
+        uint16_t* ob16 = reinterpret_cast<uint16_t*>(ob32);
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            if (first + i < E) ob16[first + i] = uint16_t(m[i]);
     } else {
-        // nonzero bytes in order
-        {
+        {  // nonzero bytes in order
             uint32_t k = e0;
             const uint8_t* bytes = reinterpret_cast<const uint8_t*>(tr) + 64 * t;
             unsigned long long bits = b0t;
             while (bits) {
                 const uint32_t j = uint32_t(__builtin_ctzll(bits));
-                dst[k++] = bytes[j];
+                ob[k++] = bytes[j];
                 bits &= bits - 1;
             }
         }
-        // changed B0 bytes
-        {
+        {  // changed B0 bytes
             uint32_t k = K0 + e1, bits = b1t;
             while (bits) {
                 const uint32_t j = uint32_t(__builtin_ctz(bits));
-                dst[k++] = uint8_t(b0t >> (8 * j));
+                ob[k++] = uint8_t(b0t >> (8 * j));
                 bits &= bits - 1;
             }
         }
-        // changed B1 bytes
-        if (t < 32) {
+        if (t < 32) {  // changed B1 bytes
             uint32_t k = K0 + K1 + e2, bits = b2t;
             while (bits) {
                 const uint32_t j = uint32_t(__builtin_ctz(bits));
-                dst[k++] = b1s[8 * t + j];
+                ob[k++] = b1s[8 * t + j];
                 bits &= bits - 1;
             }
         }
-        // changed B2 bytes, then B3 verbatim
-        if (t < 4) {
+        if (t < 4) {  // changed B2 bytes, then B3 verbatim
             uint32_t k = K0 + K1 + K2;
             for (uint32_t i = 0; i < t; ++i) k += n3s[i];
             uint32_t bits = b3s[t];
             while (bits) {
                 const uint32_t j = uint32_t(__builtin_ctz(bits));
-                dst[k++] = b2s[8 * t + j];
+                ob[k++] = b2s[8 * t + j];
                 bits &= bits - 1;
             }
-            dst[rze - 4 + t] = uint8_t(b3s[t]);
+            ob[rze - 4 + t] = uint8_t(b3s[t]);
         }
     }
-    __syncthreads();  // signbuf complete
-    for (uint32_t i = t; i < sign_bytes; i += kThreads) dst[stored + i] = uint8_t(signbuf[i >> 2] >> (8 * (i & 3u)));
+    for (uint32_t i = t; i < sign_bytes; i += kThreads) ob[stored + i] = uint8_t(signbuf[i >> 2] >> (8 * (i & 3u)));
+    __syncthreads();
+
+    // 8. copy out: aligned 32-bit stores, byte stores only in the two edge words
+    const uintptr_t g0 = reinterpret_cast<uintptr_t>(out + start), g1 = g0 + chunk_bytes;
+    for (uintptr_t a = (g0 & ~uintptr_t(3)) + 4u * t; a < g1; a += 4u * kThreads) {
+        if (a >= g0 && a + 4 <= g1) {
+            const uint32_t o = uint32_t(a - g0);
+            const uint32_t lo = ob32[o >> 2], hi = ob32[(o >> 2) + 1];
+            *reinterpret_cast<uint32_t*>(a) = uint32_t(((uint64_t(hi) << 32) | lo) >> (8 * (o & 3u)));
+        } else {
+            for (uintptr_t b = a < g0 ? g0 : a; b < a + 4 && b < g1; ++b) *reinterpret_cast<uint8_t*>(b) = ob[b - g0];
+        }
+    }
 }
 
 // Group directory: offset of the group's first payload byte, or 0 when the
@@ -332,6 +359,7 @@ __global__ void __launch_bounds__(256) wzp_decode_kernel(const uint8_t* __restri
                                                          uint16_t* __restrict__ mag, uint32_t* __restrict__ sign,
                                                          uint32_t* __restrict__ status) {
     __shared__ uint32_t tr[kChunk / 2];
+    __shared__ uint32_t ib32[(kChunkBytes + kSignSlot) / 4 + 4];  // staged chunk payload
     __shared__ uint32_t b1w[kThreads / 4];  // B1 as 64 words (256 bytes)
     __shared__ uint32_t b2w[8];             // B2 as 8 words (32 bytes)
     __shared__ uint32_t sh[8];
@@ -377,7 +405,24 @@ __global__ void __launch_bounds__(256) wzp_decode_kernel(const uint8_t* __restri
         if (first < E) sign[sword] = 0;
         return;
     }
-    const uint8_t* src = in + header + info[0];
+    // stage the chunk payload [g0, g1) in LDS with aligned word loads; the
+    // words never leave the stream [in, in + bytes)
+    const uintptr_t g0 = reinterpret_cast<uintptr_t>(in + header + info[0]), g1 = g0 + mbytes + sbytes;
+    const uintptr_t a0 = g0 & ~uintptr_t(3), limit = reinterpret_cast<uintptr_t>(in + bytes);
+    for (uintptr_t a = a0 + 4u * t; a < g1; a += 4u * kThreads) {
+        uint32_t v;
+        if (a >= reinterpret_cast<uintptr_t>(in) && a + 4 <= limit) {
+            v = *reinterpret_cast<const uint32_t*>(a);
+        } else {
+            v = 0;
+            for (uint32_t k = 0; k < 4; ++k)
+                if (a + k >= reinterpret_cast<uintptr_t>(in) && a + k < limit)
+                    v |= uint32_t(*reinterpret_cast<const uint8_t*>(a + k)) << (8 * k);
+        }
+        ib32[(a - a0) >> 2] = v;
+    }
+    __syncthreads();
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(ib32) + (g0 - a0);
     uint32_t m[32];
     bool bad = false;
     if (raw) {
@@ -488,9 +533,17 @@ __global__ void __launch_bounds__(256) wzp_decode_kernel(const uint8_t* __restri
     }
     if (t == 0 && (NNZ & 7u) && (ps[sbytes - 1] >> (NNZ & 7u))) atomicOr(status, uint32_t(kCorrupt));
     if (first < E) sign[sword] = expand_bits(bits, nzmask);
+    if (first + 32 <= E) {
+        uint4* v = reinterpret_cast<uint4*>(mdst + first);
 #pragma unroll
-    for (int i = 0; i < 32; ++i)
-        if (first + i < E) mdst[first + i] = uint16_t(m[i]);
+        for (int q = 0; q < 4; ++q)
+            v[q] = make_uint4(m[8 * q] | (m[8 * q + 1] << 16), m[8 * q + 2] | (m[8 * q + 3] << 16),
+                              m[8 * q + 4] | (m[8 * q + 5] << 16), m[8 * q + 6] | (m[8 * q + 7] << 16));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            if (first + i < E) mdst[first + i] = uint16_t(m[i]);
+    }
 }
 
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
