@@ -46,10 +46,11 @@ def stage_alg_bytes(name, n, w, wzp_bytes, sizes):
     table = {
         "quantize": 2 * w * n + 2 * n + n / 8,      # read coeff, write dequant, mags, signs
         "wzp_encode": 2 * n + n / 8 + wzp_bytes,    # read mags+signs, write stream
+        "quantize_encode": 2 * w * n + wzp_bytes,   # read coeff, write dequant + stream
         "repairs": 2 * w * n,                        # read reconstruction + original
         "wzp_decode": wzp_bytes + 2 * n + n / 8,    # read stream, write mags+signs
         "dequantize": 2 * n + n / 8 + w * n,         # read mags+signs, write coefficients
-        "patches": w * n,                            # finite screen of the output
+        "decode_dequantize": wzp_bytes + w * n,      # read stream, write coefficients
     }
     return table.get(name)
 
@@ -114,8 +115,14 @@ def main():
     log(f"compress {statistics.median(tc) * 1e3:.3f} ms ({c_gbps:.1f} GB/s), decompress "
         f"{statistics.median(td) * 1e3:.3f} ms ({d_gbps:.1f} GB/s), archive {A} B (CR {field_bytes / A:.3f})")
     if args.profile_only:
+        # calibration for the HBM byte counters: one plain elementwise kernel
+        # that reads 256 MiB and writes 256 MiB
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
         print(json.dumps({"profile_only": True, "compress_ms": statistics.median(tc) * 1e3,
-                          "decompress_ms": statistics.median(td) * 1e3}))
+                          "decompress_ms": statistics.median(td) * 1e3, "calibration_bytes": 2 * src.numel() * 4}))
         return
 
     # correctness: strict bound in FP64
@@ -162,6 +169,16 @@ def main():
     max_err2 = float((out.double() - x.double()).abs().max())
     assert max_err2 <= abs2
 
+    # pre-quantization coefficients in the input's precision vs an FP64 transform
+    from waltz_amd import autograd, stages as stage_api
+    lxy, lz, dyadic = autograd.default_levels(shape)
+    c32 = stage_api.transform(x, lxy, lz, dyadic, inverse=False)
+    c64 = stage_api.transform(x.double(), lxy, lz, dyadic, inverse=False)
+    coeff_err = float((c32.double() - c64).abs().max())
+    coeff_max = float(c64.abs().max())
+    del c32, c64
+    torch.cuda.empty_cache()
+
     # restore the REL archive for the identity check
     archive = compress(bound)
     arch_host = archive.cpu().numpy().tobytes()
@@ -182,6 +199,7 @@ def main():
         "compress_pct_hbm_alg": round(100 * alg_c / statistics.median(tc) / HBM_PEAK, 2),
         "decompress_pct_hbm_alg": round(100 * alg_d / statistics.median(td) / HBM_PEAK, 2),
         "archive_bytes": A, "compression_ratio": round(field_bytes / A, 4), "max_abs_error_fp64": max_err,
+        "coefficient_max_abs_error_vs_fp64_transform": coeff_err, "coefficient_max_abs": coeff_max,
         "abs_1e-4": {"compress_GBps": round(field_bytes / statistics.median(t2c) / 1e9, 2),
                      "decompress_GBps": round(field_bytes / statistics.median(t2d) / 1e9, 2),
                      "archive_bytes": len(arch2_host), "max_abs_error_fp64": max_err2},

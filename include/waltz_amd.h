@@ -32,6 +32,14 @@ int waltz_amd_decompress(int width, const void* archive, size_t bytes, uint32_t 
 int waltz_amd_transform(int width, int inverse, const void* input, void* output, uint32_t nx, uint32_t ny,
                         uint32_t nz, unsigned levels_xy, unsigned levels_z, int dyadic, void* stream);
 
+/* Transpose of waltz_amd_transform with the same geometry (no reference
+ * counterpart; the gradient of the transform for autograd): of_inverse = 0
+ * maps a coefficient-space gradient to sample space (DWT^T), of_inverse = 1
+ * maps a sample-space gradient to coefficient space (IDWT^T). input may
+ * equal output. */
+int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, void* output, uint32_t nx, uint32_t ny,
+                                uint32_t nz, unsigned levels_xy, unsigned levels_z, int dyadic, void* stream);
+
 /* Quantization + Morton packing (cpu/src/codec.cpp:198-234): coeff is
  * replaced by dequantized values; mag[n], sign[ceil(n/32)] packed order;
  * q-outliers sorted by index into qidx/qval (capacity n), count in *nq. */
@@ -43,6 +51,28 @@ int waltz_amd_quantize(int width, void* coeff, uint32_t nx, uint32_t ny, uint32_
 int waltz_amd_wzp_encode(const void* mag, const void* sign, uint32_t n, void* out, size_t capacity, size_t* written,
                          void* stream);
 int waltz_amd_wzp_decode(const void* in, size_t bytes, uint32_t n, void* mag, void* sign, void* stream);
+
+/* Batched decode of many archives whose tile shapes may differ (no reference
+ * counterpart; the reference's waltz_gpu_decode_batch takes one tile shape).
+ * create: dims = jobs x {nx, ny, nz} (x fastest), width 4 or 8; NULL on error
+ * (message in waltz_amd_batch_error()). Tiles are written to the output
+ * concatenated in job order (output_elements() samples in total). decode:
+ * `packed` holds the archives (device, packed_bytes long; every span must lie
+ * inside it or the job is reported corrupt), `spans` is a device array of jobs
+ * little-endian uint64 (offset, size) pairs (offsets 8-byte aligned; size 0
+ * = tile of zeros); it only enqueues work on `stream` (no synchronisation,
+ * allocation or memset), so it can be captured in a hipGraph and replayed.
+ * check: synchronises `stream` and reports the device status of the last
+ * decode (0, or -1 for corrupt archives / nonfinite output). */
+typedef struct waltz_amd_batch waltz_amd_batch;
+waltz_amd_batch* waltz_amd_batch_create(int width, uint32_t jobs, const uint32_t* dims);
+const char* waltz_amd_batch_error(void);
+void waltz_amd_batch_destroy(waltz_amd_batch* batch);
+size_t waltz_amd_batch_workspace_bytes(const waltz_amd_batch* batch);
+size_t waltz_amd_batch_output_elements(const waltz_amd_batch* batch);
+int waltz_amd_batch_decode(waltz_amd_batch* batch, const void* packed, size_t packed_bytes, const void* spans,
+                           void* output, void* workspace, size_t workspace_bytes, void* stream);
+int waltz_amd_batch_check(waltz_amd_batch* batch, void* stream);
 
 /* Per-stage timing with HIP events on the pipeline's stream (off by default).
  * enable returns the previous state; report writes a JSON object

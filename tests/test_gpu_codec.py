@@ -158,6 +158,28 @@ def test_codec_matches_oracle(wz, oracle, shape, dtype):
         assert np.max(np.abs(d.astype(np.float64) - v.astype(np.float64))) <= bound
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_codec_paired_bricks(wz, oracle, dtype):
+    """Full 16^3 bricks with nx % 32 == 0 (quantizer fused into the WZP
+    encoder, dequantizer into the decoder): a zero slab gives empty chunks,
+    sparse spikes give q-outliers."""
+    waltz_amd, _ = wz
+    shape = (496, 144, 160)
+    rng = np.random.default_rng(11)
+    z, y, x = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    v = np.sin(0.05 * x) * np.cos(0.03 * y + 0.01 * z) + 0.01 * rng.standard_normal(shape)
+    v[:200] = 0.0
+    spikes = rng.integers(0, v.size, 300)
+    v.reshape(-1)[spikes] = rng.choice([-1.0, 1.0], 300) * 5e3
+    v = v.astype(dtype)
+    ref = oracle.compress(v, 1e-3)
+    a = waltz_amd.compress(dev(v), abs_error=1e-3)
+    assert host(a).tobytes() == ref
+    d = host(waltz_amd.decompress(a, shape=shape, dtype=TDT[dtype]))
+    assert same_bits(d, oracle.decompress(ref, shape, dtype))
+    assert np.max(np.abs(d.astype(np.float64) - v.astype(np.float64))) <= 1e-3
+
+
 def test_rejects_nonfinite(wz):
     waltz_amd, _ = wz
     v = np.ones((16, 16, 16), np.float32)

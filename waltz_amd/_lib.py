@@ -45,6 +45,16 @@ _SIGS = {
     "waltz_amd_profile_enable": (C.c_int, [C.c_int]),
     "waltz_amd_profile_report": (_SZ, [C.c_char_p, _SZ]),
 }
+_SIGS.update({
+    "waltz_amd_batch_create": (_P, [C.c_int, _U32, C.POINTER(_U32)]),
+    "waltz_amd_batch_error": (C.c_char_p, []),
+    "waltz_amd_batch_destroy": (None, [_P]),
+    "waltz_amd_batch_workspace_bytes": (_SZ, [_P]),
+    "waltz_amd_batch_output_elements": (_SZ, [_P]),
+    "waltz_amd_batch_decode": (C.c_int, [_P, _P, _SZ, _P, _P, _P, _SZ, _P]),
+    "waltz_amd_batch_check": (C.c_int, [_P, _P]),
+    "waltz_amd_transform_adjoint": (C.c_int, [C.c_int, C.c_int, _P, _P, _U32, _U32, _U32, _U, _U, C.c_int, _P]),
+})
 OPTIONAL = {
     "waltz_gpu_decode_batch": (C.c_char_p, [C.POINTER(BatchShape), _P, _P, _SZ, _P, _SZ, _P, _SZ, C.c_double, _P]),
 }

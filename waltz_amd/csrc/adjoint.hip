@@ -1,0 +1,237 @@
+// Adjoints (transposes) of the multi-level CDF 9/7 transforms, for gradients:
+// DWT^T maps a coefficient-space gradient to sample space, IDWT^T maps a
+// sample-space gradient to coefficient space. No reference counterpart.
+//
+// One axis pass of the forward transform lifts every line of the active box:
+// the line is extended by whole-sample symmetric reflection, the four lifting
+// steps and the scaling run on the extended sequence, and the result is
+// restricted to [0, n) and split (even samples first, odd after). The lifting
+// steps keep the reflection symmetry, so the pass equals
+// split . restrict . L . reflect with L the lifting of an unbounded line. Its
+// transpose is reflect^T . L^T . restrict^T . split^T: interleave the split
+// gradient, extend it by zeros, run the transposed steps in reverse order
+// (each "x[k] += c (x[k-1] + x[k+1])" on one parity becomes the same update on
+// the other parity), and fold every extended position back onto the sample it
+// reflects to. The inverse pass is the mirror image (split input, natural
+// output). The support of L^T is 4 samples wide on each side, so a line chunk
+// of 64 outputs is computed in LDS from a zero-extended window with 16
+// samples of margin.
+//
+// The adjoint of a multi-level transform runs the transposed passes in the
+// reverse order of the transform's schedule.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace wz {
+namespace {
+
+constexpr int kChunkOut = 64;                  // outputs per line chunk
+constexpr int kMargin = 16;                    // window margin on each side
+constexpr int kWin = kChunkOut + 2 * kMargin;  // 96 window samples
+constexpr int kPairs = kWin / 2;
+
+WZ_HD uint32_t split_index(uint32_t p, uint32_t n) { return (p >> 1) + ((p & 1u) ? half_len(n) : 0u); }
+
+// Lines per workgroup: X lines are rows (window along x); Y and Z lines are
+// 32 consecutive x, so every global access is a 128-byte (f32) row segment.
+template <int Axis> constexpr int lines_per_group() { return Axis == 0 ? 8 : 32; }
+
+// One update step on the LDS window: for every pair j of every line,
+// OddFromEven: o_j += c (e_j + e_{j+1}), else e_j += c (o_{j-1} + o_j).
+template <class T, int L, bool OddFromEven>
+WZ_D void step(T (*v)[kWin + 1], T c) {
+    for (uint32_t w = threadIdx.x; w < uint32_t(L * kPairs); w += 256) {
+        const uint32_t l = w % L, j = w / L;
+        if (OddFromEven) {
+            if (j + 1 < kPairs) v[l][2 * j + 1] = fmadd(c, v[l][2 * j] + v[l][2 * j + 2], v[l][2 * j + 1]);
+        } else {
+            if (j >= 1) v[l][2 * j] = fmadd(c, v[l][2 * j - 1] + v[l][2 * j + 1], v[l][2 * j]);
+        }
+    }
+    __syncthreads();
+}
+
+// OfInverse: transpose of the inverse pass (natural -> split), else of the
+// forward pass (split -> natural).
+template <class T, bool OfInverse, int Axis>
+__global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T* __restrict__ dst, Dims full,
+                                                    Dims act, uint32_t chunks, uint32_t groups) {
+    constexpr int L = lines_per_group<Axis>();
+    __shared__ T v[L][kWin + 1];
+    using C = Lift<T>;
+    const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
+    const uint32_t chunk = blockIdx.x % chunks, rest = blockIdx.x / chunks;
+    const uint32_t group = rest % groups, outer = rest / groups;
+    const uint32_t c0 = chunk * kChunkOut;
+    const uint64_t plane = uint64_t(full.nx) * full.ny;
+    const uint64_t stride = Axis == 0 ? 1u : Axis == 1 ? uint64_t(full.nx) : plane;
+
+    // line l of this workgroup: base offset, or ~0 when out of the box
+    auto line_base = [&](uint32_t l) -> uint64_t {
+        if constexpr (Axis == 0) {
+            const uint64_t li = uint64_t(group) * L + l;
+            if (li >= uint64_t(act.ny) * act.nz) return ~uint64_t(0);
+            const uint32_t y = uint32_t(li % act.ny), z = uint32_t(li / act.ny);
+            return uint64_t(z) * plane + uint64_t(y) * full.nx;
+        } else {
+            const uint32_t x = group * L + l;
+            if (x >= act.nx) return ~uint64_t(0);
+            return Axis == 1 ? uint64_t(outer) * plane + x : uint64_t(outer) * full.nx + x;
+        }
+    };
+
+    // zero-extended window [c0 - margin, c0 + 64 + margin) of the interleaved line
+    for (uint32_t w = threadIdx.x; w < uint32_t(L * kWin); w += 256) {
+        const uint32_t l = Axis == 0 ? w / kWin : w % L, i = Axis == 0 ? w % kWin : w / L;
+        const int32_t q = int32_t(c0) - kMargin + int32_t(i);
+        const uint64_t base = line_base(l);
+        T x = T(0);
+        if (base != ~uint64_t(0) && q >= 0 && uint32_t(q) < n) {
+            const uint32_t pos = OfInverse ? uint32_t(q) : split_index(uint32_t(q), n);
+            x = src[base + uint64_t(pos) * stride];
+        }
+        if (!OfInverse) x = (i & 1u) ? x * (-C::ik) : x * C::k;  // scaling first
+        v[l][i] = x;
+    }
+    __syncthreads();
+    if constexpr (OfInverse) {  // transposes of A', B', G', D' in that order, then 1/K
+        step<T, L, false>(v, -C::a);
+        step<T, L, true>(v, -C::b);
+        step<T, L, false>(v, -C::g);
+        step<T, L, true>(v, -C::d);
+    } else {  // transposes of D, G, B, A in that order (after the scaling)
+        step<T, L, true>(v, C::d);
+        step<T, L, false>(v, C::g);
+        step<T, L, true>(v, C::b);
+        step<T, L, false>(v, C::a);
+    }
+
+    // fold the extended positions onto the samples they reflect to
+    const int32_t lo = -4, hi = int32_t(n) + 4;  // support of the transposed lifting
+    for (uint32_t w = threadIdx.x; w < uint32_t(L * kChunkOut); w += 256) {
+        const uint32_t l = Axis == 0 ? w / kChunkOut : w % L, i = Axis == 0 ? w % kChunkOut : w / L;
+        const uint32_t p = c0 + i;
+        const uint64_t base = line_base(l);
+        if (base == ~uint64_t(0) || p >= n) continue;
+        auto u = [&](int32_t q) { return v[l][q - int32_t(c0) + kMargin]; };
+        T s = u(int32_t(p));
+        if (n >= 9) {  // at most two more images: -p and 2n-2-p
+            if (p >= 1 && p <= 4) s += u(-int32_t(p));
+            const int32_t m = 2 * int32_t(n) - 2 - int32_t(p);
+            if (m != int32_t(p) && m < hi) s += u(m);
+        } else {  // short line: the whole window is one chunk
+            for (int32_t q = lo; q < hi; ++q)
+                if (q != int32_t(p) && reflect_pos(q, n) == p) s += u(q);
+        }
+        if (OfInverse) s = (p & 1u) ? s * (-C::k) : s * C::ik;
+        dst[base + uint64_t(OfInverse ? split_index(p, n) : p) * stride] = s;
+    }
+}
+
+Dims shrink_box(Dims d, bool xy, bool z) {
+    if (xy) {
+        d.nx = half_len(d.nx);
+        d.ny = half_len(d.ny);
+    }
+    if (z) d.nz = half_len(d.nz);
+    return d;
+}
+
+template <class T, bool OfInverse, int Axis>
+void launch_adjoint(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
+    constexpr int L = lines_per_group<Axis>();
+    const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
+    if (!n || !act.nx || !act.ny || !act.nz) return;
+    const uint32_t chunks = (n + kChunkOut - 1) / kChunkOut;
+    const uint64_t groups =
+        Axis == 0 ? (uint64_t(act.ny) * act.nz + L - 1) / L : (uint64_t(act.nx) + L - 1) / L;
+    const uint64_t outer = Axis == 0 ? 1u : Axis == 1 ? act.nz : act.ny;
+    const uint64_t blocks = uint64_t(chunks) * groups * outer;
+    check_launch_size(blocks * 256);
+    hipLaunchKernelGGL((adjoint_pass<T, OfInverse, Axis>), dim3(uint32_t(blocks)), dim3(256), 0, s, src, dst, full,
+                       act, chunks, uint32_t(groups));
+}
+
+template <class T, bool OfInverse>
+void adjoint_axis(const T* src, T* dst, Dims full, Dims act, int axis, hipStream_t s) {
+    if (axis == 0) launch_adjoint<T, OfInverse, 0>(src, dst, full, act, s);
+    else if (axis == 1) launch_adjoint<T, OfInverse, 1>(src, dst, full, act, s);
+    else launch_adjoint<T, OfInverse, 2>(src, dst, full, act, s);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) adjoint_copy(const T* __restrict__ src, T* __restrict__ dst, Dims full,
+                                                    Dims act, uint32_t total) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t x = i % act.nx, r = i / act.nx, y = r % act.ny, z = r / act.ny;
+    const uint64_t at = (uint64_t(z) * full.ny + y) * full.nx + x;
+    dst[at] = src[at];
+}
+
+// One group of passes over one box, out of place through tmp0/tmp1, ending in `home`.
+template <class T, bool OfInverse>
+void box_passes(T* home, T* tmp0, T* tmp1, Dims full, Dims act, const int* axes, int count, hipStream_t s) {
+    const T* src = home;
+    for (int k = 0; k < count; ++k) {
+        T* dst = k == count - 1 ? home : (k & 1 ? tmp1 : tmp0);
+        if (dst == src) dst = tmp0;
+        adjoint_axis<T, OfInverse>(src, dst, full, act, axes[k], s);
+        src = dst;
+    }
+    if (src != home) {
+        const uint64_t total = uint64_t(act.nx) * act.ny * act.nz;
+        check_launch_size(total);
+        hipLaunchKernelGGL((adjoint_copy<T>), dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, src, home, full,
+                           act, uint32_t(total));
+    }
+}
+}  // namespace
+
+// In place on `data` (full volume); tmp0/tmp1: two full-volume scratch arrays.
+// Schedules are those of transform_forward / transform_inverse, reversed.
+template <class T>
+void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       bool of_inverse, hipStream_t s) {
+    if (dyadic) {
+        Dims sizes[5], act = full;
+        for (unsigned l = 0; l < lxy; ++l) {
+            sizes[l] = act;
+            act = shrink_box(act, true, true);
+        }
+        if (!of_inverse) {  // forward: levels 0..L-1, axes Y, X, Z -> reversed
+            static const int zxy[3] = {2, 0, 1};
+            for (int l = int(lxy) - 1; l >= 0; --l) box_passes<T, false>(data, tmp0, tmp1, full, sizes[l], zxy, 3, s);
+        } else {  // inverse: levels L-1..0, axes Z, X, Y -> reversed
+            static const int yxz[3] = {1, 0, 2};
+            for (unsigned l = 0; l < lxy; ++l) box_passes<T, true>(data, tmp0, tmp1, full, sizes[l], yxz, 3, s);
+        }
+        return;
+    }
+    Dims xy[5], zs[5], act = full;
+    for (unsigned l = 0; l < lxy; ++l) {
+        xy[l] = act;
+        act = shrink_box(act, true, false);
+    }
+    act = full;
+    for (unsigned l = 0; l < lz; ++l) {
+        zs[l] = act;
+        act = shrink_box(act, false, true);
+    }
+    static const int zonly[1] = {2};
+    if (!of_inverse) {  // forward: Z levels 0..lz-1, then XY levels (Y, X) 0..lxy-1 -> reversed
+        static const int xy_rev[2] = {0, 1};
+        for (int l = int(lxy) - 1; l >= 0; --l) box_passes<T, false>(data, tmp0, tmp1, full, xy[l], xy_rev, 2, s);
+        for (int l = int(lz) - 1; l >= 0; --l) box_passes<T, false>(data, tmp0, tmp1, full, zs[l], zonly, 1, s);
+    } else {  // inverse: XY levels (X, Y) lxy-1..0, then Z levels lz-1..0 -> reversed
+        static const int yx[2] = {1, 0};
+        for (unsigned l = 0; l < lz; ++l) box_passes<T, true>(data, tmp0, tmp1, full, zs[l], zonly, 1, s);
+        for (unsigned l = 0; l < lxy; ++l) box_passes<T, true>(data, tmp0, tmp1, full, xy[l], yx, 2, s);
+    }
+}
+
+template void transform_adjoint<float>(float*, float*, float*, Dims, unsigned, unsigned, bool, bool, hipStream_t);
+template void transform_adjoint<double>(double*, double*, double*, Dims, unsigned, unsigned, bool, bool,
+                                        hipStream_t);
+
+}  // namespace wz

@@ -1,36 +1,681 @@
-// Batched decode ABI (include/waltz_gpu_batch.h; reference semantics
-// src/batch_decode.cuh:29-186): spans and headers are validated from the host
-// copy, every archive is decoded on the device, and each tile is placed (with
-// clipping at the frame edge) into the (side, sample, z, y, x) output; holes
-// are filled.
+// Batched decode: many canonical archives, of one tile shape or of a mix of
+// shapes, decoded with one launch per pipeline stage for the whole batch.
+//
+// A plan is built on the host from the tile shapes alone: per job the level
+// schedule, the WZP chunk and Morton brick counts, offsets into three work
+// banks (coefficients, and two scratch banks that also hold the decoded
+// magnitudes and sign words), and the output placement (offset, pitches and
+// clipped extents). Each stage then runs as one launch whose blocks are dealt
+// to jobs through block prefix tables:
+//   plan     one workgroup parses every 32-byte header on the device, checks
+//            it against the job's shape and span, and scans the side-channel
+//            counts over jobs;
+//   wzp      one workgroup per WZP chunk of every job (the single-field
+//            chunk decoder, writing into the job's magnitude/sign banks);
+//   dequant  one workgroup per Morton brick (rank computed on the fly);
+//   qpatch   q-outliers of all jobs, grid-stride;
+//   passes   the inverse lifting schedule, one launch per schedule step, each
+//            job running its own axis, box and tile size at that step; the
+//            job's last pass writes straight into the output with clipping
+//            (holes are filled there);
+//   rpatch   native repairs of all jobs into the output.
+// No host synchronisation, allocation or memset happens between the stages,
+// so a batch can be captured in a hipGraph and replayed. Every value equals
+// the single-field decoder's (same kernels' arithmetic, same tile sizes).
+//
+// Entry points: the reference's waltz_gpu_decode_batch (one tile shape,
+// frames of tiles; src/batch_decode.cuh:29-186) and waltz_amd_batch_* (any
+// mix of shapes, tiles concatenated in job order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <vector>
 
+#include "../../include/waltz_amd.h"
 #include "../../include/waltz_gpu_batch.h"
 #include "common.hpp"
 #include "launch.hpp"
+#include "lifting.hpp"
+#include "wzp_device.hpp"
 
 namespace wz {
 namespace {
-thread_local std::string g_batch_error;
 
-struct BatchGeometry {
-    uint32_t gx, gy, gz, jobs;
-    uint64_t n, frame;
+enum PassKind : uint8_t { kPassX = 0, kPassY = 1, kPassZ = 2, kPassCopy = 3, kPassNone = 255 };
+enum BankId : uint8_t { kHome = 0, kTmp0 = 1, kTmp1 = 2, kOutput = 3 };
+enum JobState : uint32_t { kJobOk = 0, kJobHole = 1, kJobBad = 2 };
+
+struct JobShape {
+    Dims d;
+    uint32_t lx, lz, bz, dyadic;
+    uint32_t n, chunks, groups, nbx, nby, nbz;
+    uint64_t bank;      // element offset in each work bank
+    uint64_t mag;       // u16 offset of the magnitudes (in the tmp0 bank)
+    uint64_t sign;      // word offset of the sign words (in the tmp1 bank)
+    uint64_t out;       // element offset of the tile origin in the output
+    uint64_t opy, opz;  // output pitches (elements)
+    Dims clip;          // tile extents that lie inside the output
 };
+
+struct JobRun {
+    uint64_t arc;       // byte offset of the archive in the packed buffer
+    uint64_t wzp;       // WZP stream bytes
+    double quantum;
+    uint32_t np, nq;
+    uint64_t pi, pv, qi, qv;  // section offsets from the archive start
+    uint32_t state, pad;
+};
+
+struct Pass {
+    uint8_t kind, src, dst, pad;
+    Dims act;           // box of this pass
+    uint32_t ntiles;    // tiles per line
+    uint32_t work;      // work items (threads)
+};
+
+struct Banks {
+    void* home;
+    void* tmp0;
+    void* tmp1;
+    void* out;
+};
+
+// Largest j with pfx[j] <= b (pfx[jobs] > b): the job owning item b.
+WZ_D uint32_t job_of(const uint32_t* __restrict__ pfx, uint32_t jobs, uint32_t b) {
+    uint32_t lo = 0, hi = jobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pfx[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+WZ_D uint32_t rd32(const uint8_t* p) {
+    return uint32_t(p[0]) | (uint32_t(p[1]) << 8) | (uint32_t(p[2]) << 16) | (uint32_t(p[3]) << 24);
+}
+
+// Header checks of archive_size_from_header plus the job's own geometry
+// (include/utils/Blob.hpp:71-90, cpu/src/codec.cpp:276-296).
+WZ_D bool parse_header(const uint8_t* h, uint64_t size, const JobShape& sh, uint32_t width, JobRun& r) {
+    if (size < 32) return false;
+    const uint32_t f = h[0];
+    bool ok = !(f & 0xE0u) && (f & 0x18u) && (((f & 4u) ? 8u : 4u) == width) && ((f >> 3) & 3u) == 3u && !(f & 2u);
+    ok = ok && h[1] == sh.lx && h[2] == sh.lz && h[3] == sh.bz && (f & 1u) == sh.dyadic;
+    const uint64_t qb = uint64_t(rd32(h + 8)) | (uint64_t(rd32(h + 12)) << 32);
+    const double q = __longlong_as_double((long long)qb);
+    ok = ok && q > 0 && __builtin_isfinite(q);
+    if (width == 4) {
+        const float qf = float(q);
+        ok = ok && qf > 0.0f && __builtin_isfinite(qf);
+    }
+    const uint32_t pb = rd32(h + 4), np = rd32(h + 24), nq = rd32(h + 28);
+    const uint64_t wzp = uint64_t(rd32(h + 16)) | (uint64_t(rd32(h + 20)) << 32);
+    const WzpShape ws = wzp_shape(sh.n);
+    ok = ok && np <= sh.n && nq <= sh.n && uint64_t(pb) == uint64_t(np) * width;
+    ok = ok && wzp >= ws.header && wzp <= ws.header + uint64_t(sh.n) * 2 + (uint64_t(sh.n) + 7) / 8;
+    if (!ok) return false;
+    const Layout L = archive_layout(wzp, np, uint64_t(np) * width, nq);
+    if (L.end != size) return false;
+    r.wzp = wzp;
+    r.quantum = q;
+    r.np = np;
+    r.nq = nq;
+    r.pi = L.pi;
+    r.pv = L.pv;
+    r.qi = L.qi;
+    r.qv = L.qv;
+    return true;
+}
+
+// One workgroup of 1024 threads: per-job headers, then exclusive scans of
+// the repair and q-outlier counts over decodable jobs.
+__global__ void __launch_bounds__(1024) batch_plan_kernel(const uint8_t* __restrict__ packed, uint64_t packed_bytes,
+                                                          const uint64_t* __restrict__ spans,
+                                                          const JobShape* __restrict__ shape, JobRun* run,
+                                                          uint32_t jobs, uint32_t width, uint32_t* np_pfx,
+                                                          uint32_t* nq_pfx, uint32_t* status) {
+    __shared__ uint32_t wsum[2][16];
+    __shared__ uint32_t carry[2];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = t; j < jobs; j += 1024) {
+        JobRun r{};
+        r.arc = spans[2 * j];
+        const uint64_t size = spans[2 * j + 1];
+        if (size == 0) {
+            r.state = kJobHole;
+        } else if ((r.arc & 7u) || r.arc > packed_bytes || size > packed_bytes - r.arc ||
+                   !parse_header(packed + r.arc, size, shape[j], width, r)) {
+            r.state = kJobBad;
+            atomicOr(status, uint32_t(kCorrupt));
+        }
+        run[j] = r;
+    }
+    if (t < 2) carry[t] = 0;
+    __syncthreads();
+    const uint32_t lane = t & 63u, wv = t >> 6;
+    for (uint32_t base = 0; base <= jobs; base += 1024) {
+        const uint32_t j = base + t;
+        uint32_t vp = 0, vq = 0;
+        if (j < jobs && run[j].state == kJobOk) {
+            vp = run[j].np;
+            vq = run[j].nq;
+        }
+        uint32_t ip = vp, iq = vq;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t a = __shfl_up(ip, d, 64), b = __shfl_up(iq, d, 64);
+            if (lane >= d) {
+                ip += a;
+                iq += b;
+            }
+        }
+        if (lane == 63) {
+            wsum[0][wv] = ip;
+            wsum[1][wv] = iq;
+        }
+        __syncthreads();
+        uint32_t bp = carry[0], bq = carry[1], tp = 0, tq = 0;
+        for (uint32_t k = 0; k < 16; ++k) {
+            bp += k < wv ? wsum[0][k] : 0u;
+            bq += k < wv ? wsum[1][k] : 0u;
+            tp += wsum[0][k];
+            tq += wsum[1][k];
+        }
+        if (j <= jobs) {
+            np_pfx[j] = bp + ip - vp;
+            nq_pfx[j] = bq + iq - vq;
+        }
+        __syncthreads();
+        if (t == 0) {
+            carry[0] += tp;
+            carry[1] += tq;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_wzp_kernel(const uint8_t* __restrict__ packed,
+                                                        const JobShape* __restrict__ shape,
+                                                        const JobRun* __restrict__ run,
+                                                        const uint32_t* __restrict__ chunk_pfx, uint32_t jobs,
+                                                        uint16_t* mag, uint32_t* sign, uint32_t* status) {
+    const uint32_t j = job_of(chunk_pfx, jobs, blockIdx.x);
+    const JobRun& r = run[j];
+    if (r.state != kJobOk) return;
+    const JobShape& sh = shape[j];
+    decode_chunk(packed + r.arc + 32, r.wzp, sh.n, sh.chunks, sh.groups, blockIdx.x - chunk_pfx[j],
+                 MagSink{mag + sh.mag, sign + sh.sign}, status);
+}
+
+// Dequantization of one brick (quantize.hip's brick order and packed base;
+// rank from the recursive octant rule, cpu/src/codec.cpp:96-150).
+template <class T>
+__global__ void __launch_bounds__(256) batch_dequant_kernel(const JobShape* __restrict__ shape,
+                                                            const JobRun* __restrict__ run,
+                                                            const uint32_t* __restrict__ brick_pfx, uint32_t jobs,
+                                                            const uint16_t* __restrict__ mag,
+                                                            const uint32_t* __restrict__ sign, T* __restrict__ home) {
+    const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
+    const JobRun& r = run[j];
+    if (r.state != kJobOk) return;
+    const JobShape& sh = shape[j];
+    const uint32_t b = blockIdx.x - brick_pfx[j];
+    const uint32_t bxi = b % sh.nbx, rest = b / sh.nbx, byi = rest % sh.nby, bzi = rest / sh.nby;
+    const uint32_t x0 = bxi * 16, y0 = byi * 16, z0 = bzi * sh.bz;
+    const uint32_t bx = min(16u, sh.d.nx - x0), by = min(16u, sh.d.ny - y0), bzz = min(sh.bz, sh.d.nz - z0);
+    const uint64_t base = uint64_t(z0) * sh.d.nx * sh.d.ny + uint64_t(y0) * sh.d.nx * bzz + uint64_t(x0) * by * bzz;
+    const T q = T(r.quantum);
+    const uint16_t* m = mag + sh.mag;
+    const uint32_t* sg = sign + sh.sign;
+    T* c = home + sh.bank;
+    const uint32_t vol = bx * by * bzz;
+    for (uint32_t i = threadIdx.x; i < vol; i += 256) {
+        const uint32_t x = i % bx, y = (i / bx) % by, z = i / (bx * by);
+        const uint64_t p = base + morton_rank(x, y, z, bx, by, bzz);
+        const T v = T(m[p]) * q;  // cpu/src/codec.cpp:146-147
+        c[(uint64_t(z0 + z) * sh.d.ny + (y0 + y)) * sh.d.nx + x0 + x] = ((sg[p >> 5] >> (p & 31u)) & 1u) ? -v : v;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) batch_qpatch_kernel(const uint8_t* __restrict__ packed,
+                                                           const JobShape* __restrict__ shape,
+                                                           const JobRun* __restrict__ run,
+                                                           const uint32_t* __restrict__ nq_pfx, uint32_t jobs,
+                                                           T* __restrict__ home, uint32_t* status) {
+    const uint32_t total = nq_pfx[jobs];
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < total; k += gridDim.x * 256u) {
+        const uint32_t j = job_of(nq_pfx, jobs, k), e = k - nq_pfx[j];
+        const JobRun& r = run[j];
+        const JobShape& sh = shape[j];
+        const uint8_t* a = packed + r.arc;
+        const uint32_t i = reinterpret_cast<const uint32_t*>(a + r.qi)[e];
+        const uint32_t v = reinterpret_cast<const uint32_t*>(a + r.qv)[e];
+        if (i >= sh.n || v <= 65535u || v > 0x7FFFFFFFu) {
+            atomicOr(status, uint32_t(kCorrupt));
+            continue;
+        }
+        T* c = home + sh.bank;
+        const T x = T(v) * T(r.quantum);  // cpu/src/codec.cpp:299-302
+        c[i] = c[i] < T(0) ? -x : x;
+    }
+}
+
+// Stores one reconstructed sample: into a work bank, or (last pass) into the
+// output with clipping, the hole fill and the non-finite screen.
+template <class T>
+struct Put {
+    T* bank;  // null: output
+    T* out;
+    const JobShape* sh;
+    bool hole;
+    T fill;
+    WZ_D bool operator()(uint64_t at, uint32_t x, uint32_t y, uint32_t z, T v) const {
+        if (bank) {
+            bank[at] = v;
+            return true;
+        }
+        if (x >= sh->clip.nx || y >= sh->clip.ny || z >= sh->clip.nz) return true;
+        out[sh->out + uint64_t(z) * sh->opz + uint64_t(y) * sh->opy + x] = hole ? fill : v;
+        return hole || __builtin_isfinite(v);
+    }
+};
+
+// One tile of one line of an inverse axis pass (axis_pass of transform.hip
+// with the job's tile extents as the volume).
+template <class T, int Axis, int Tile>
+WZ_D bool inverse_line(const T* __restrict__ src, const Put<T>& put, Dims full, Dims act, uint32_t ntiles,
+                       uint32_t i) {
+    constexpr int P = Tile / 2 + 4;
+    uint32_t x = 0, y = 0, z = 0, t;
+    if constexpr (Axis == 0) {
+        t = i % ntiles;
+        const uint32_t r = i / ntiles;
+        y = r % act.ny;
+        z = r / act.ny;
+    } else if constexpr (Axis == 1) {
+        x = i % act.nx;
+        const uint32_t r = i / act.nx;
+        t = r % ntiles;
+        z = r / ntiles;
+    } else {
+        x = i % act.nx;
+        const uint32_t r = i / act.nx;
+        y = r % act.ny;
+        t = r / act.ny;
+    }
+    const uint64_t plane = uint64_t(full.nx) * full.ny;
+    const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
+    const uint64_t stride = Axis == 0 ? 1u : Axis == 1 ? uint64_t(full.nx) : plane;
+    const uint64_t base = uint64_t(z) * plane + uint64_t(y) * full.nx + x;
+    const int32_t start = int32_t(t) * Tile;
+    T e[P], o[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int32_t p = start - 4 + 2 * j;
+        e[j] = src[base + uint64_t(split_pos(p, n)) * stride];
+        o[j] = src[base + uint64_t(split_pos(p + 1, n)) * stride];
+    }
+    lift_inverse<T, P, Axis != 0>(e, o);
+    bool fin = true;
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) {
+        const uint32_t p = uint32_t(start) + 2u * uint32_t(j - 2);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t q = p + h;
+            if (q < n) {
+                const uint32_t X = Axis == 0 ? q : x, Y = Axis == 1 ? q : y, Z = Axis == 2 ? q : z;
+                fin = put(base + uint64_t(q) * stride, X, Y, Z, h ? o[j] : e[j]) && fin;
+            }
+        }
+    }
+    return fin;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) batch_pass_kernel(const JobShape* __restrict__ shape,
+                                                         const JobRun* __restrict__ run,
+                                                         const Pass* __restrict__ pass,
+                                                         const uint32_t* __restrict__ pfx, uint32_t jobs, Banks bk,
+                                                         T fill, uint32_t* status) {
+    constexpr int ZTile = sizeof(T) == 4 ? 16 : 8;  // canonical inverse Z tile
+    const uint32_t j = job_of(pfx, jobs, blockIdx.x);
+    const Pass ps = pass[j];
+    const uint32_t i = (blockIdx.x - pfx[j]) * 256u + threadIdx.x;
+    const uint32_t state = run[j].state;
+    if (ps.kind == kPassNone || state == kJobBad || (state == kJobHole && ps.dst != kOutput)) return;
+    const JobShape& sh = shape[j];
+    auto bank = [&](uint8_t id) -> T* {
+        T* b = static_cast<T*>(id == kHome ? bk.home : id == kTmp0 ? bk.tmp0 : bk.tmp1);
+        return b + sh.bank;
+    };
+    const T* src = bank(ps.src);
+    const Put<T> put{ps.dst == kOutput ? nullptr : bank(ps.dst), static_cast<T*>(bk.out), &sh, state == kJobHole,
+                     fill};
+    bool fin = true;
+    if (i < ps.work) {
+        switch (ps.kind) {
+            case kPassX: fin = inverse_line<T, 0, 8>(src, put, sh.d, ps.act, ps.ntiles, i); break;
+            case kPassY: fin = inverse_line<T, 1, 8>(src, put, sh.d, ps.act, ps.ntiles, i); break;
+            case kPassZ: fin = inverse_line<T, 2, ZTile>(src, put, sh.d, ps.act, ps.ntiles, i); break;
+            default: {
+                const uint32_t x = i % ps.act.nx, r = i / ps.act.nx, y = r % ps.act.ny, z = r / ps.act.ny;
+                const uint64_t at = (uint64_t(z) * sh.d.ny + y) * sh.d.nx + x;
+                fin = put(at, x, y, z, src[at]);
+            }
+        }
+    }
+    if (__ballot(!fin) && (threadIdx.x & 63u) == 0) atomicOr(status, uint32_t(kNonFiniteOutput));
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) batch_rpatch_kernel(const uint8_t* __restrict__ packed,
+                                                           const JobShape* __restrict__ shape,
+                                                           const JobRun* __restrict__ run,
+                                                           const uint32_t* __restrict__ np_pfx, uint32_t jobs,
+                                                           T* __restrict__ out, uint32_t* status) {
+    const uint32_t total = np_pfx[jobs];
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < total; k += gridDim.x * 256u) {
+        const uint32_t j = job_of(np_pfx, jobs, k), e = k - np_pfx[j];
+        const JobRun& r = run[j];
+        const JobShape& sh = shape[j];
+        const uint8_t* a = packed + r.arc;
+        const uint32_t i = reinterpret_cast<const uint32_t*>(a + r.pi)[e];
+        const T err = reinterpret_cast<const T*>(a + r.pv)[e];
+        if (i >= sh.n || !__builtin_isfinite(err)) {
+            atomicOr(status, uint32_t(kCorrupt));
+            continue;
+        }
+        const uint32_t x = i % sh.d.nx, y = (i / sh.d.nx) % sh.d.ny, z = i / (sh.d.nx * sh.d.ny);
+        if (x >= sh.clip.nx || y >= sh.clip.ny || z >= sh.clip.nz) continue;
+        T* o = out + sh.out + uint64_t(z) * sh.opz + uint64_t(y) * sh.opy + x;
+        const T v = *o - err;  // cpu/src/codec.cpp:330
+        *o = v;
+        if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
+    }
+}
+
+// ------------------------------------------------------------------ host plan
+Dims halve(Dims d, bool xy, bool z) {
+    if (xy) {
+        d.nx = half_len(d.nx);
+        d.ny = half_len(d.ny);
+    }
+    if (z) d.nz = half_len(d.nz);
+    return d;
+}
+
+Pass make_pass(uint8_t kind, uint8_t src, uint8_t dst, Dims act, unsigned width) {
+    Pass p{};
+    p.kind = kind;
+    p.src = src;
+    p.dst = dst;
+    p.act = act;
+    const uint64_t vol = uint64_t(act.nx) * act.ny * act.nz;
+    if (kind == kPassCopy) {
+        p.ntiles = 1;
+        p.work = uint32_t(vol);
+        return p;
+    }
+    const uint32_t n = kind == kPassX ? act.nx : kind == kPassY ? act.ny : act.nz;
+    const uint32_t tile = kind == kPassZ ? (width == 4 ? 16u : 8u) : 8u;
+    p.ntiles = (n + tile - 1) / tile;
+    p.work = uint32_t(vol / n * p.ntiles);
+    return p;
+}
+
+// The inverse schedule of transform_inverse, as out-of-place passes through
+// the three banks, the last one into the output.
+std::vector<Pass> inverse_schedule(const JobShape& sh, unsigned width) {
+    std::vector<Pass> out;
+    struct Level {
+        Dims act;
+        std::vector<uint8_t> axes;
+    };
+    std::vector<Level> levels;
+    if (sh.dyadic) {
+        Dims sizes[5], act = sh.d;
+        for (unsigned l = 0; l < sh.lx; ++l) {
+            sizes[l] = act;
+            act = halve(act, true, true);
+        }
+        for (int l = int(sh.lx) - 1; l >= 0; --l) levels.push_back({sizes[l], {kPassZ, kPassX, kPassY}});
+    } else {
+        Dims xy[5], zs[5], act = sh.d;
+        for (unsigned l = 0; l < sh.lx; ++l) {
+            xy[l] = act;
+            act = halve(act, true, false);
+        }
+        act = sh.d;
+        for (unsigned l = 0; l < sh.lz; ++l) {
+            zs[l] = act;
+            act = halve(act, false, true);
+        }
+        for (int l = int(sh.lx) - 1; l >= 0; --l) levels.push_back({xy[l], {kPassX, kPassY}});
+        for (int l = int(sh.lz) - 1; l >= 0; --l) levels.push_back({zs[l], {kPassZ}});
+    }
+    if (levels.empty()) {
+        out.push_back(make_pass(kPassCopy, kHome, kOutput, sh.d, width));
+        return out;
+    }
+    for (size_t k = 0; k < levels.size(); ++k) {
+        const bool last = k + 1 == levels.size();
+        const Level& L = levels[k];
+        const uint8_t fin = last ? uint8_t(kOutput) : uint8_t(kHome);
+        uint8_t src = kHome;
+        for (size_t a = 0; a < L.axes.size(); ++a) {
+            uint8_t dst = a + 1 == L.axes.size() ? fin : uint8_t(a & 1 ? kTmp1 : kTmp0);
+            if (dst == src) dst = kTmp0;
+            out.push_back(make_pass(L.axes[a], src, dst, L.act, width));
+            src = dst;
+        }
+        if (src != fin) out.push_back(make_pass(kPassCopy, src, fin, L.act, width));
+    }
+    return out;
+}
+
+struct HostPlan {
+    unsigned width = 4;
+    uint32_t jobs = 0;
+    std::vector<JobShape> shape;
+    std::vector<uint32_t> chunk_pfx, brick_pfx;
+    std::vector<std::vector<Pass>> step;      // step[s][job]
+    std::vector<std::vector<uint32_t>> pfx;   // block prefixes per step
+    uint64_t bank_elems = 0, mag_elems = 0, sign_words = 0, out_elems = 0;
+};
+
+// Geometry checks of the single-field codec (codec.hip geometry()).
+JobShape job_shape(Dims d, unsigned width) {
+    if (!d.nx || !d.ny || !d.nz) throw std::invalid_argument("nonzero dimensions required");
+    const uint64_t n = uint64_t(d.nx) * d.ny * d.nz;
+    if (n > 0xFFFFFFFFull / width || n > 0xFFFFFFFFull / 2) throw std::invalid_argument("tile exceeds format limits");
+    JobShape s{};
+    s.d = d;
+    s.n = uint32_t(n);
+    s.lx = dwt_levels(d.nx < d.ny ? d.nx : d.ny);
+    s.lz = dwt_levels(d.nz);
+    s.dyadic = s.lx == s.lz;
+    s.bz = zband(d, s.lz, s.dyadic);
+    const WzpShape ws = wzp_shape(s.n);
+    s.chunks = ws.chunks;
+    s.groups = ws.groups;
+    s.nbx = (d.nx + 15) / 16;
+    s.nby = (d.ny + 15) / 16;
+    s.nbz = (d.nz + s.bz - 1) / s.bz;
+    s.clip = d;
+    return s;
+}
+
+// Finishes a plan whose shapes (and output placement) are set: bank offsets,
+// block prefixes and the per-step pass tables.
+void finish_plan(HostPlan& p) {
+    const uint32_t J = p.jobs;
+    p.chunk_pfx.assign(J + 1, 0);
+    p.brick_pfx.assign(J + 1, 0);
+    std::vector<std::vector<Pass>> sched(J);
+    size_t steps = 0;
+    uint64_t bank = 0, mag = 0, sign = 0, chunks = 0, bricks = 0;
+    for (uint32_t j = 0; j < J; ++j) {
+        JobShape& s = p.shape[j];
+        s.bank = bank;
+        s.mag = mag;
+        s.sign = sign;
+        bank += (uint64_t(s.n) + 3) & ~uint64_t(3);
+        mag += (uint64_t(s.n) + 7) & ~uint64_t(7);
+        sign += (uint64_t(s.n) + 31) / 32;
+        p.chunk_pfx[j] = uint32_t(chunks);
+        p.brick_pfx[j] = uint32_t(bricks);
+        chunks += s.chunks;
+        bricks += uint64_t(s.nbx) * s.nby * s.nbz;
+        sched[j] = inverse_schedule(s, p.width);
+        steps = std::max(steps, sched[j].size());
+    }
+    if (chunks > 0x7FFFFFFFull || bricks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
+    p.chunk_pfx[J] = uint32_t(chunks);
+    p.brick_pfx[J] = uint32_t(bricks);
+    p.bank_elems = bank;
+    p.mag_elems = mag;
+    p.sign_words = sign;
+    p.step.assign(steps, std::vector<Pass>(J));
+    p.pfx.assign(steps, std::vector<uint32_t>(J + 1, 0));
+    for (size_t st = 0; st < steps; ++st) {
+        uint64_t blocks = 0;
+        for (uint32_t j = 0; j < J; ++j) {
+            Pass ps{};
+            ps.kind = kPassNone;
+            if (st < sched[j].size()) ps = sched[j][st];
+            p.step[st][j] = ps;
+            p.pfx[st][j] = uint32_t(blocks);
+            if (ps.kind != kPassNone) blocks += (uint64_t(ps.work) + 255) / 256;
+        }
+        if (blocks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
+        p.pfx[st][J] = uint32_t(blocks);
+    }
+}
+
+// Device copy of a plan: tables in one allocation owned by the plan.
+struct DevicePlan {
+    HostPlan host;
+    uint8_t* mem = nullptr;
+    JobShape* shape = nullptr;
+    JobRun* run = nullptr;
+    uint32_t *chunk_pfx = nullptr, *brick_pfx = nullptr, *np_pfx = nullptr, *nq_pfx = nullptr, *status = nullptr;
+    Pass* pass = nullptr;        // steps x jobs
+    uint32_t* pass_pfx = nullptr;  // steps x (jobs + 1)
+    uint32_t* host_status = nullptr;  // pinned
+    std::string error;
+
+    ~DevicePlan() {
+        if (mem) hipFree(mem);
+        if (host_status) hipHostFree(host_status);
+    }
+
+    void upload(hipStream_t s) {
+        const HostPlan& p = host;
+        const uint32_t J = p.jobs;
+        const size_t S = p.step.size();
+        size_t used = 0;
+        auto carve = [&](size_t bytes) {
+            const size_t at = used;
+            used += (bytes + 255) & ~size_t(255);
+            return at;
+        };
+        const size_t o_shape = carve(sizeof(JobShape) * J), o_run = carve(sizeof(JobRun) * J);
+        const size_t o_cp = carve(4 * (J + 1)), o_bp = carve(4 * (J + 1)), o_np = carve(4 * (J + 1)),
+                     o_nq = carve(4 * (J + 1)), o_st = carve(4), o_pass = carve(sizeof(Pass) * J * S),
+                     o_pp = carve(4 * (J + 1) * S);
+        hip_check(hipMalloc(reinterpret_cast<void**>(&mem), used), "batch plan allocation");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_status), 4), "batch status");
+        shape = reinterpret_cast<JobShape*>(mem + o_shape);
+        run = reinterpret_cast<JobRun*>(mem + o_run);
+        chunk_pfx = reinterpret_cast<uint32_t*>(mem + o_cp);
+        brick_pfx = reinterpret_cast<uint32_t*>(mem + o_bp);
+        np_pfx = reinterpret_cast<uint32_t*>(mem + o_np);
+        nq_pfx = reinterpret_cast<uint32_t*>(mem + o_nq);
+        status = reinterpret_cast<uint32_t*>(mem + o_st);
+        pass = reinterpret_cast<Pass*>(mem + o_pass);
+        pass_pfx = reinterpret_cast<uint32_t*>(mem + o_pp);
+        std::vector<Pass> flat;
+        std::vector<uint32_t> flat_pfx;
+        for (size_t st = 0; st < S; ++st) {
+            flat.insert(flat.end(), p.step[st].begin(), p.step[st].end());
+            flat_pfx.insert(flat_pfx.end(), p.pfx[st].begin(), p.pfx[st].end());
+        }
+        hip_check(hipMemcpyAsync(shape, p.shape.data(), sizeof(JobShape) * J, hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(chunk_pfx, p.chunk_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(brick_pfx, p.brick_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        if (S) {
+            hip_check(hipMemcpyAsync(pass, flat.data(), sizeof(Pass) * flat.size(), hipMemcpyHostToDevice, s), "plan");
+            hip_check(hipMemcpyAsync(pass_pfx, flat_pfx.data(), 4 * flat_pfx.size(), hipMemcpyHostToDevice, s),
+                      "plan");
+        }
+        hip_check(hipStreamSynchronize(s), "plan upload");
+    }
+
+    // Enqueues the whole decode on `s`: no synchronisation, allocation or memset.
+    template <class T>
+    void enqueue(const uint8_t* packed, uint64_t packed_bytes, const uint64_t* spans, Banks bk, double fill,
+                 hipStream_t s) const {
+        const HostPlan& p = host;
+        const uint32_t J = p.jobs;
+        zero_words(status, 1, s);
+        hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(1024), 0, s, packed, packed_bytes, spans, shape, run, J,
+                           p.width, np_pfx, nq_pfx, status);
+        if (p.chunk_pfx[J])
+            hipLaunchKernelGGL(batch_wzp_kernel, dim3(p.chunk_pfx[J]), dim3(256), 0, s, packed, shape, run, chunk_pfx,
+                               J, static_cast<uint16_t*>(bk.tmp0), static_cast<uint32_t*>(bk.tmp1), status);
+        if (p.brick_pfx[J])
+            hipLaunchKernelGGL((batch_dequant_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, run,
+                               brick_pfx, J, static_cast<const uint16_t*>(bk.tmp0),
+                               static_cast<const uint32_t*>(bk.tmp1), static_cast<T*>(bk.home));
+        hipLaunchKernelGGL((batch_qpatch_kernel<T>), dim3(256), dim3(256), 0, s, packed, shape, run, nq_pfx, J,
+                           static_cast<T*>(bk.home), status);
+        for (size_t st = 0; st < p.step.size(); ++st) {
+            const uint32_t blocks = p.pfx[st][J];
+            if (!blocks) continue;
+            hipLaunchKernelGGL((batch_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape, run,
+                               pass + st * J, pass_pfx + st * (J + 1), J, bk, T(fill), status);
+        }
+        hipLaunchKernelGGL((batch_rpatch_kernel<T>), dim3(256), dim3(256), 0, s, packed, shape, run, np_pfx, J,
+                           static_cast<T*>(bk.out), status);
+    }
+
+    void check(hipStream_t s) {
+        hip_check(hipGetLastError(), "batch launch");
+        hip_check(hipMemcpyAsync(host_status, status, 4, hipMemcpyDeviceToHost, s), "batch status");
+        hip_check(hipStreamSynchronize(s), "batch decode");
+        const uint32_t st = *host_status;
+        if (st & kCorrupt) throw std::runtime_error("corrupt Waltz archive in batch");
+        if (st & kNonFiniteOutput) throw std::runtime_error("nonfinite reconstruction");
+        if (st) throw std::runtime_error("Waltz device status error");
+    }
+
+    uint64_t bank_bytes() const { return host.bank_elems * host.width; }
+};
+
+thread_local std::string g_batch_error;
 
 void need(bool ok, const char* what) {
     if (!ok) throw std::runtime_error(what);
 }
 
-BatchGeometry batch_geometry(const waltz_batch_shape& d) {
+// ------------------------------------------------------------------ reference ABI
+struct FrameGeometry {
+    uint32_t gx, gy, gz, jobs;
+    uint64_t n, frame;
+};
+
+FrameGeometry frame_geometry(const waltz_batch_shape& d) {
     need(d.x && d.y && d.z && d.fx && d.fy && d.fz && d.samples && (d.sides == 1 || d.sides == 2),
          "invalid batch geometry");
     need(d.width == 4 || d.width == 8, "width must be 4 or 8");
-    BatchGeometry g;
+    FrameGeometry g;
     g.gx = (d.fx + d.x - 1) / d.x;
     g.gy = (d.fy + d.y - 1) / d.y;
     g.gz = (d.fz + d.z - 1) / d.z;
@@ -42,72 +687,75 @@ BatchGeometry batch_geometry(const waltz_batch_shape& d) {
     return g;
 }
 
-struct Placement {
-    uint32_t tx, ty, tz, fx, fy, fz, ox, oy, oz;
-    uint64_t base;  // element offset of this job's frame in the output
-};
-
-Placement place(const waltz_batch_shape& d, const BatchGeometry& g, uint32_t j) {
-    Placement p{d.x, d.y, d.z, d.fx, d.fy, d.fz, 0, 0, 0, 0};
-    p.ox = (j % g.gx) * d.x;
-    j /= g.gx;
-    p.oy = (j % g.gy) * d.y;
-    j /= g.gy;
-    p.oz = (j % g.gz) * d.z;
-    j /= g.gz;
-    p.base = (uint64_t(j % d.sides) * d.samples + j / d.sides) * g.frame;
+// Tiles of one shape in (sample, side, tile z, tile y, tile x) job order,
+// placed in (side, sample, z, y, x) frames with clipping.
+HostPlan frame_plan(const waltz_batch_shape& d, const FrameGeometry& g) {
+    HostPlan p;
+    p.width = d.width;
+    p.jobs = g.jobs;
+    const JobShape proto = job_shape(Dims{d.x, d.y, d.z}, d.width);
+    p.shape.assign(g.jobs, proto);
+    for (uint32_t j = 0; j < g.jobs; ++j) {
+        uint32_t r = j;
+        const uint32_t ox = (r % g.gx) * d.x;
+        r /= g.gx;
+        const uint32_t oy = (r % g.gy) * d.y;
+        r /= g.gy;
+        const uint32_t oz = (r % g.gz) * d.z;
+        r /= g.gz;
+        const uint64_t base = (uint64_t(r % d.sides) * d.samples + r / d.sides) * g.frame;
+        JobShape& s = p.shape[j];
+        s.out = base + (uint64_t(oz) * d.fy + oy) * d.fx + ox;
+        s.opy = d.fx;
+        s.opz = uint64_t(d.fx) * d.fy;
+        s.clip = Dims{std::min(d.x, d.fx - ox), std::min(d.y, d.fy - oy), std::min(d.z, d.fz - oz)};
+    }
+    p.out_elems = g.frame * d.samples * d.sides;
+    finish_plan(p);
     return p;
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) place_tile(const T* __restrict__ tile, T* __restrict__ out, Placement p,
-                                                  uint32_t n, bool fill_only, T fill) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t x = p.ox + i % p.tx, y = p.oy + (i / p.tx) % p.ty, z = p.oz + i / (p.tx * p.ty);
-    if (x >= p.fx || y >= p.fy || z >= p.fz) return;
-    out[p.base + (uint64_t(z) * p.fy + y) * p.fx + x] = fill_only ? fill : tile[i];
-}
-
-template <class T>
-void run(const waltz_batch_shape& d, const uint8_t* host, const uint8_t* packed, size_t bytes, void* scratch,
-         size_t scratch_bytes, T* out, size_t out_bytes, double fill, hipStream_t s) {
-    const BatchGeometry g = batch_geometry(d);
+void run_frames(const waltz_batch_shape& d, const uint8_t* host, const uint8_t* packed, size_t bytes, void* scratch,
+                size_t scratch_bytes, T* out, size_t out_bytes, double fill, hipStream_t s) {
+    const FrameGeometry g = frame_geometry(d);
     need(out_bytes == g.frame * d.samples * d.sides * sizeof(T), "incorrect output size");
-    need(scratch_bytes >= g.n * sizeof(T), "short batch buffers");
     need(bytes >= 16ull * g.jobs, "short batch buffers");
     const size_t table = bytes - 16ull * g.jobs;
     need(table % 16 == 0, "unaligned batch spans");
-    const Dims tile{d.x, d.y, d.z};
-    T* buf = static_cast<T*>(scratch);
+    // host-side validation of every span and header (src/batch_decode.cuh:90-120)
     for (uint32_t j = 0; j < g.jobs; ++j) {
         uint64_t pair[2];
         std::memcpy(pair, host + table + 16ull * j, 16);
-        const Placement p = place(d, g, j);
-        const uint32_t blocks = uint32_t((g.n + 255) / 256);
-        if (!pair[1]) {
-            hipLaunchKernelGGL((place_tile<T>), dim3(blocks), dim3(256), 0, s, buf, out, p, uint32_t(g.n), true,
-                               T(fill));
-            continue;
-        }
-        const uint64_t at = pair[0], sz = pair[1];
-        need(at % 16 == 0 && at <= table && sz >= 32 && sz <= table - at, "invalid batch span");
-        need(archive_size_from_header(host + at, tile, sizeof(T)) == sz, "invalid canonical batch archive");
-        decompress<T>(packed + at, sz, tile, buf, s);
-        hipLaunchKernelGGL((place_tile<T>), dim3(blocks), dim3(256), 0, s, buf, out, p, uint32_t(g.n), false,
-                           T(fill));
+        if (!pair[1]) continue;
+        need(pair[0] % 16 == 0 && pair[0] <= table && pair[1] >= 32 && pair[1] <= table - pair[0],
+             "invalid batch span");
+        need(archive_size_from_header(host + pair[0], Dims{d.x, d.y, d.z}, sizeof(T)) == pair[1],
+             "invalid canonical batch archive");
     }
-    hip_check(hipStreamSynchronize(s), "batch decode");
+    DevicePlan plan;
+    plan.host = frame_plan(d, g);
+    const uint64_t bank = plan.bank_bytes();
+    need(scratch_bytes >= 3 * bank, "short batch buffers");
+    plan.upload(s);
+    uint8_t* sc = static_cast<uint8_t*>(scratch);
+    const Banks bk{sc, sc + bank, sc + 2 * bank, out};
+    plan.enqueue<T>(packed, table, reinterpret_cast<const uint64_t*>(packed + table), bk, fill, s);
+    plan.check(s);
 }
 }  // namespace
 }  // namespace wz
+
+struct waltz_amd_batch {
+    wz::DevicePlan plan;
+};
 
 extern "C" {
 
 size_t waltz_gpu_batch_scratch_bytes(const waltz_batch_shape* shape) {
     try {
-        const wz::BatchGeometry g = wz::batch_geometry(*shape);
-        return size_t((g.n * shape->width + 255) & ~uint64_t(255));
+        const wz::FrameGeometry g = wz::frame_geometry(*shape);
+        return size_t(3 * g.jobs * ((g.n + 3) & ~uint64_t(3)) * shape->width);
     } catch (...) {
         return 0;
     }
@@ -122,11 +770,11 @@ const char* waltz_gpu_decode_batch(const waltz_batch_shape* shape, const void* h
         const auto* p = static_cast<const uint8_t*>(packed);
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (shape->width == 4)
-            wz::run<float>(*shape, h, p, bytes, scratch, scratch_bytes, static_cast<float*>(output), output_bytes,
-                           fill, s);
+            wz::run_frames<float>(*shape, h, p, bytes, scratch, scratch_bytes, static_cast<float*>(output),
+                                  output_bytes, fill, s);
         else
-            wz::run<double>(*shape, h, p, bytes, scratch, scratch_bytes, static_cast<double*>(output), output_bytes,
-                            fill, s);
+            wz::run_frames<double>(*shape, h, p, bytes, scratch, scratch_bytes, static_cast<double*>(output),
+                                   output_bytes, fill, s);
         return nullptr;
     } catch (const std::exception& e) {
         wz::g_batch_error = e.what();
@@ -134,6 +782,82 @@ const char* waltz_gpu_decode_batch(const waltz_batch_shape* shape, const void* h
         wz::g_batch_error = "unknown batch error";
     }
     return wz::g_batch_error.c_str();
+}
+
+waltz_amd_batch* waltz_amd_batch_create(int width, uint32_t jobs, const uint32_t* dims) {
+    try {
+        if (width != 4 && width != 8) throw std::invalid_argument("width must be 4 or 8");
+        if (!jobs || !dims) throw std::invalid_argument("empty batch");
+        if (jobs > 1000000) throw std::invalid_argument("batch launch limit");
+        auto b = std::make_unique<waltz_amd_batch>();
+        wz::HostPlan& p = b->plan.host;
+        p.width = unsigned(width);
+        p.jobs = jobs;
+        p.shape.reserve(jobs);
+        uint64_t out = 0;
+        for (uint32_t j = 0; j < jobs; ++j) {
+            wz::JobShape s = wz::job_shape(wz::Dims{dims[3 * j], dims[3 * j + 1], dims[3 * j + 2]}, unsigned(width));
+            s.out = out;
+            s.opy = s.d.nx;
+            s.opz = uint64_t(s.d.nx) * s.d.ny;
+            out += s.n;
+            p.shape.push_back(s);
+        }
+        p.out_elems = out;
+        wz::finish_plan(p);
+        b->plan.upload(nullptr);
+        return b.release();
+    } catch (const std::exception& e) {
+        wz::g_batch_error = e.what();
+    } catch (...) {
+        wz::g_batch_error = "unknown batch error";
+    }
+    return nullptr;
+}
+
+const char* waltz_amd_batch_error(void) { return wz::g_batch_error.c_str(); }
+
+void waltz_amd_batch_destroy(waltz_amd_batch* b) { delete b; }
+
+size_t waltz_amd_batch_workspace_bytes(const waltz_amd_batch* b) { return b ? size_t(3 * b->plan.bank_bytes()) : 0; }
+
+size_t waltz_amd_batch_output_elements(const waltz_amd_batch* b) { return b ? size_t(b->plan.host.out_elems) : 0; }
+
+int waltz_amd_batch_decode(waltz_amd_batch* b, const void* packed, size_t packed_bytes, const void* spans,
+                           void* output, void* workspace, size_t workspace_bytes, void* stream) {
+    try {
+        if (!b || !packed || !spans || !output || !workspace) throw std::invalid_argument("null argument");
+        const uint64_t bank = b->plan.bank_bytes();
+        if (workspace_bytes < 3 * bank) throw std::invalid_argument("short batch workspace");
+        uint8_t* w = static_cast<uint8_t*>(workspace);
+        const wz::Banks bk{w, w + bank, w + 2 * bank, output};
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (b->plan.host.width == 4)
+            b->plan.enqueue<float>(static_cast<const uint8_t*>(packed), packed_bytes,
+                                   static_cast<const uint64_t*>(spans), bk, 0.0, s);
+        else
+            b->plan.enqueue<double>(static_cast<const uint8_t*>(packed), packed_bytes,
+                                    static_cast<const uint64_t*>(spans), bk, 0.0, s);
+        return 0;
+    } catch (const std::exception& e) {
+        wz::g_batch_error = e.what();
+    } catch (...) {
+        wz::g_batch_error = "unknown batch error";
+    }
+    return -1;
+}
+
+int waltz_amd_batch_check(waltz_amd_batch* b, void* stream) {
+    try {
+        if (!b) throw std::invalid_argument("null batch");
+        b->plan.check(static_cast<hipStream_t>(stream));
+        return 0;
+    } catch (const std::exception& e) {
+        wz::g_batch_error = e.what();
+    } catch (...) {
+        wz::g_batch_error = "unknown batch error";
+    }
+    return -1;
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "../../include/waltz.hpp"
@@ -232,6 +233,32 @@ int waltz_amd_transform(int width, int inverse, const void* input, void* output,
         wz::hip_check(hipStreamSynchronize(s), "transform");
         hipFree(tmp);
         hipFree(status);
+    });
+}
+
+int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, void* output, uint32_t nx, uint32_t ny,
+                                uint32_t nz, unsigned lxy, unsigned lz, int dyadic, void* stream) {
+    return guarded([&] {
+        if (lxy > 5 || lz > 5 || (dyadic && lxy != lz)) throw std::invalid_argument("invalid transform levels");
+        if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
+        const wz::Dims d = dims_of(nx, ny, nz);
+        const uint64_t n = uint64_t(nx) * ny * nz;
+        hipStream_t s = as_stream(stream);
+        if (input != output)
+            wz::hip_check(hipMemcpyAsync(output, input, n * size_t(width), hipMemcpyDeviceToDevice, s), "copy");
+        void* tmp = nullptr;
+        wz::hip_check(hipMalloc(&tmp, 2 * n * size_t(width) + 512), "adjoint scratch");
+        std::unique_ptr<void, hipError_t (*)(void*)> hold(tmp, hipFree);
+        char* t0 = static_cast<char*>(tmp);
+        char* t1 = t0 + ((n * size_t(width) + 255) & ~size_t(255));
+        if (width == 4)
+            wz::transform_adjoint<float>(static_cast<float*>(output), reinterpret_cast<float*>(t0),
+                                         reinterpret_cast<float*>(t1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
+        else
+            wz::transform_adjoint<double>(static_cast<double*>(output), reinterpret_cast<double*>(t0),
+                                          reinterpret_cast<double*>(t1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
+        wz::hip_check(hipGetLastError(), "adjoint launch");
+        wz::hip_check(hipStreamSynchronize(s), "transform adjoint");
     });
 }
 

@@ -17,6 +17,7 @@
 // Scratch lives in one cached, grow-only device allocation per device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -25,7 +26,9 @@
 #include <vector>
 
 #include "common.hpp"
+#include "fused.hpp"
 #include "launch.hpp"
+#include "pairbrick.hpp"
 #include "profile.hpp"
 
 namespace wz {
@@ -164,10 +167,14 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         throw std::invalid_argument("quantum not representable in native type");
     if (capacity < 32) throw std::runtime_error("archive capacity too small");
     const uint32_t n = g.n;
-    const uint64_t words = (uint64_t(n) + 31) / 32, segs = (uint64_t(n) + 1023) / 1024;
+    const uint64_t words = (uint64_t(n) + 31) / 32;
     const WzpShape ws = wzp_shape(n);
     const uint32_t nq = qcap(n);
     const uint32_t rtiles = (n + kRepairTile - 1) / kRepairTile;
+    uint32_t fsegs = 0, fcap = 0;  // repair segments of a fused finest inverse level
+    level0_repair_segments(g.d, g.lx, g.dyadic, &fsegs, &fcap);
+    const uint64_t rslots = std::max<uint64_t>(uint64_t(rtiles) * kRepairTile, uint64_t(fsegs) * fcap);
+    const uint32_t rsegs = std::max(rtiles, fsegs);
 
     DeviceWorkspace& w = workspace();
     std::lock_guard<std::mutex> guard(w.lock);
@@ -175,9 +182,9 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     // layout of the per-call scratch
     auto plan = [&](Carve& c) {
         struct P {
-            T *coeff, *t0, *t1;
+            T *coeff, *t0, *t1, *rec;
             uint16_t* mag;
-            uint32_t *sign, *qbm, *qidx, *qval, *ridx, *rcount, *seg, *zeroed;
+            uint32_t *sign, *qbm, *rbm, *qidx, *qval, *ridx, *rcount, *seg, *zeroed;
             T* rerr;
             unsigned long long* desc;
             uint32_t *excl, *size, *counters;
@@ -185,13 +192,15 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.coeff = c.take<T>(n);
         p.t0 = c.take<T>(n);
         p.t1 = c.take<T>(n);
+        p.rec = c.take<T>(n);
         p.mag = c.take<uint16_t>(n);
-        p.ridx = c.take<uint32_t>(n);
-        p.rerr = c.take<T>(n);
-        p.rcount = c.take<uint32_t>(rtiles);
+        p.ridx = c.take<uint32_t>(rslots);
+        p.rerr = c.take<T>(rslots);
+        p.rcount = c.take<uint32_t>(rsegs);
+        p.rbm = c.take<uint32_t>(words);
         p.qidx = c.take<uint32_t>(nq);
         p.qval = c.take<uint32_t>(nq);
-        p.seg = c.take<uint32_t>(segs);
+        p.seg = c.take<uint32_t>(finish_scratch_words(n));
         p.excl = c.take<uint32_t>(ws.chunks);
         p.size = c.take<uint32_t>(ws.chunks);
         // zero-initialised region: sign, q bitmap, lookback descriptors, counters
@@ -214,23 +223,30 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     zero_words(p.zeroed, zero_bytes / 4, s);
 
     transform_forward<T>(input, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, status, s);
-    const BrickGeometry bg = brick_geometry(g.d, g.bz);
-    const uint16_t* rank = w.ranks(g.d, g.bz, s);
     SideList qo{p.qbm, p.qidx, p.qval, p.counters + 1, nq};
-    {
-        StageClock c("quantize", s);
-        quantize_bricks<T>(p.coeff, bg, quantum, rank, p.mag, p.sign, qo, status, s);
-    }
     WzpScratch wsc{p.desc, p.excl, p.size, p.counters};
-    {
+    if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz)) {
+        StageClock c("quantize_encode", s);
+        quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,
+                                 status, s);
+    } else {
+        const BrickGeometry bg = brick_geometry(g.d, g.bz);
+        const uint16_t* rank = w.ranks(g.d, g.bz, s);
+        {
+            StageClock c("quantize", s);
+            quantize_bricks<T>(p.coeff, bg, quantum, rank, p.mag, p.sign, qo, status, s);
+        }
         StageClock c("wzp_encode", s);
         wzp_encode(p.mag, p.sign, n, archive + 32, capacity - 32, wsc, &res->wzp_bytes, status, s);
     }
-    transform_inverse<T>(p.coeff, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s);
-    TileList rp{p.ridx, p.rerr, p.rcount, p.counters + 2};
-    {
+    // finest inverse level with native repair detection fused in when that
+    // level runs as a fused kernel; otherwise reconstruct, then a repair pass
+    Level0Emit<T> emit{kEmitRepair, status, input, abs_bound, SegList{p.rbm, p.ridx, p.rerr, p.rcount, fsegs, fcap}};
+    SegList rp = emit.rep;
+    if (!transform_inverse<T>(p.coeff, p.rec, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s, fsegs ? &emit : nullptr)) {
+        rp = SegList{p.rbm, p.ridx, p.rerr, p.rcount, rtiles, kRepairTile};
         StageClock c("repairs", s);
-        detect_repairs<T>(input, p.coeff, n, abs_bound, rp, status, s);
+        detect_repairs<T>(input, p.rec, n, abs_bound, rp, status, s);
     }
     const uint8_t flags = uint8_t((g.dyadic ? 1u : 0u) | (sizeof(T) == 8 ? 4u : 0u) | (3u << 3));
     {
@@ -324,22 +340,29 @@ void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* outpu
     uint32_t* sign = c.take<uint32_t>(words);
     uint32_t* status = &w.result->status;
     zero_words(w.result, sizeof(EncodeResult) / 4, s);
-    {
-        StageClock c("wzp_decode", s);
-        wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
-    }
-    const BrickGeometry bg = brick_geometry(g.d, bz);
-    const uint16_t* rank = w.ranks(g.d, bz, s);
-    {
+    if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz)) {
+        StageClock c("decode_dequantize", s);
+        decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s);
+    } else {
+        {
+            StageClock c("wzp_decode", s);
+            wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
+        }
+        const BrickGeometry bg = brick_geometry(g.d, bz);
+        const uint16_t* rank = w.ranks(g.d, bz, s);
         StageClock c("dequantize", s);
         dequantize_bricks<T>(mag, sign, bg, quantum, rank, coeff, s);
+    }
+    {
+        StageClock c("qoutliers", s);
         apply_qoutliers<T>(coeff, archive + L.qi, archive + L.qv, nq, n, quantum, status, s);
     }
-    transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, g.dyadic, s);
+    Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}};
+    const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, g.dyadic, s, &screen);
     {
         StageClock c("patches", s);
-        apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);
-        check_finite<T>(output, n, status, s);
+        apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);  // screens patched samples
+        if (!screened) check_finite<T>(output, n, status, s);
     }
     hip_check(hipMemcpyAsync(w.host_result, w.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s),
               "status copy");

@@ -94,6 +94,45 @@ template <class T> struct Lift {
 WZ_HD float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 WZ_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Exact |value - original| <= bound with a TwoDiff tie check
+// (cpu/src/repair_avx2.cpp:48-56).
+WZ_D bool within_bound_exact(double value, double original, double bound) {
+    const double high = value - original, magnitude = __builtin_fabs(high);
+    if (!__builtin_isfinite(high) || magnitude > bound) return false;
+    if (magnitude < bound) return true;
+    const double bv = value - high, av = high + bv;
+    const double br = bv - original, ar = value - av, low = ar + br;
+    return !((high > 0 && low > 0) || (high < 0 && low < 0));
+}
+
+// One native repair decision (cpu/src/repair_avx2.cpp:58-74): returns whether
+// `rec` needs a record (error in *err); clears *ok on a strict-bound failure.
+template <class T>
+WZ_D bool repair_decision(T rec, T orig, T native_bound, double bound, T* err, bool* ok) {
+    T value = rec;
+    const T e = value - orig;
+    bool patch = false;
+    if (!__builtin_isfinite(value) || !__builtin_isfinite(e)) *ok = false;
+    if (__builtin_fabs(e) > native_bound) {
+        patch = true;
+        value = value - e;
+    }
+    if (!__builtin_isfinite(value) || !within_bound_exact(double(value), double(orig), bound)) *ok = false;
+    *err = e;
+    return patch;
+}
+
+// Segmented side list: each producer workgroup appends to its own segment of
+// `cap` entries and records the count; a complete row-major flag bitmap lets
+// the archive assembly find every entry's index-order position.
+struct SegList {
+    uint32_t* bitmap;   // ceil(n/32) words
+    uint32_t* index;    // segments * cap
+    void* payload;      // segments * cap of the payload type
+    uint32_t* count;    // per segment
+    uint32_t segments, cap;
+};
+
 // ------------------------------------------------------------------ WZP format
 constexpr uint32_t kChunk = 8192;        // magnitudes per chunk
 constexpr uint32_t kChunkBytes = 16384;  // raw magnitude bytes per chunk

@@ -18,6 +18,12 @@ struct FwdLevel {
     uint32_t* status;   // non-null: screen the input for non-finite samples
 };
 
+enum InvEmit : int {
+    kEmitStore = 0,        // write the reconstruction
+    kEmitStoreScreen = 1,  // write it and flag non-finite samples
+    kEmitRepair = 2,       // no write: native repair records against `orig`
+};
+
 template <class T>
 struct InvLevel {
     const T* hi;        // coefficient volume (high bands of this level)
@@ -26,9 +32,19 @@ struct InvLevel {
     uint64_t lpy, lpz;
     Dims n;             // level box extents
     T* out;             // reconstructed level box
-    uint64_t opy, opz;
+    uint64_t opy, opz;  // (also the pitches of `orig`)
     uint32_t sz;        // z segment length (multiple of 16)
+    int emit;           // InvEmit
+    uint32_t* status;   // screen / strict-bound failures
+    const T* orig;      // kEmitRepair: the original field
+    double bound;
+    SegList rep;        // kEmitRepair: one segment per workgroup, cap = 64*32*sz
 };
+
+// Repair segments needed by a level-0 inverse with z segment sz.
+inline uint64_t inv_segments(Dims n, uint32_t sz) {
+    return uint64_t((n.nx + 63) / 64) * ((n.ny + 31) / 32) * ((n.nz + sz - 1) / sz);
+}
 
 template <class T>
 void launch_fwd_level(const FwdLevel<T>& a, hipStream_t s);

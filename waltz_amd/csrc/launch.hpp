@@ -22,9 +22,30 @@ inline void check_launch_size(uint64_t total) {
 template <class T>
 void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz,
                        bool dyadic, uint32_t* status, hipStream_t s);
+// What the last (finest) inverse level does with its reconstruction when it
+// runs as a fused kernel: store it (optionally screening for non-finite
+// samples) or, without storing, derive native repairs against `orig`.
 template <class T>
-void transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       hipStream_t s);
+struct Level0Emit {
+    int mode;          // InvEmit (fused.hpp)
+    uint32_t* status;
+    const T* orig;
+    double bound;
+    SegList rep;       // segments/cap per level0_repair_segments()
+};
+// Returns true when `emit` was carried out inside the fused finest level;
+// false means the caller must run the equivalent separate pass on `output`.
+template <class T>
+bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       hipStream_t s, const Level0Emit<T>* emit = nullptr);
+// Transpose of transform_forward (of_inverse = false) or transform_inverse
+// (true) with the same geometry, in place on `data`; tmp0/tmp1 as above.
+template <class T>
+void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       bool of_inverse, hipStream_t s);
+// Segments and per-segment capacity the fused finest inverse level uses for
+// repairs (0 segments when that level is not fused for this geometry).
+void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, uint32_t* segments, uint32_t* cap);
 
 // quantize.hip
 struct BrickGeometry {
@@ -64,17 +85,21 @@ void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* 
 // Decodes a WZP stream of `bytes` bytes into packed mag/sign arrays.
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
                 hipStream_t s);
+// Paired-brick geometry only (pair_bricks_fit): quantization fused into the
+// encoder (coefficients dequantized in place, q-outliers appended to `qo`),
+// and dequantization fused into the decoder (writes the coefficient volume).
+template <class T>
+void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
+                           WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s);
+template <class T>
+void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double quantum, T* coeff, uint32_t* status,
+                             hipStream_t s);
 
 // sections.hip
-constexpr uint32_t kRepairTile = 4096;  // row-major indices per repair tile
-struct TileList {        // per-tile ordered segments (tile k at k*kRepairTile)
-    uint32_t* index;     // n entries
-    void* payload;       // n entries of payload type
-    uint32_t* count;     // ceil(n/kRepairTile) counts, scanned in place to prefixes
-    uint32_t* total;     // device total after the scan
-};
+constexpr uint32_t kRepairTile = 4096;  // row-major indices per standalone repair segment
+// Standalone repair pass: segment k = indices [4096k, 4096k+4096), cap 4096.
 template <class T>
-void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, TileList r,
+void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, SegList r,
                     uint32_t* status, hipStream_t s);
 struct EncodeResult {
     unsigned long long wzp_bytes;  // written by wzp_encode
@@ -82,10 +107,11 @@ struct EncodeResult {
     uint32_t np, nq, status, pad;
 };
 // Orders both side lists by index into the archive sections and writes the
-// 32-byte header and all padding. `seg` needs ceil(n/1024) u32 of scratch.
+// 32-byte header and all padding. `scratch` needs finish_scratch_words(n) u32.
+uint64_t finish_scratch_words(uint32_t n);
 template <class T>
-void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, TileList repairs, SideList qout,
-                    uint32_t* seg, uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum,
+void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SegList repairs, SideList qout,
+                    uint32_t* scratch, uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum,
                     EncodeResult* result, hipStream_t s);
 template <class T>
 void apply_qoutliers(T* coeff, const uint8_t* idx, const uint8_t* val, uint32_t count, uint32_t n, double quantum,

@@ -45,25 +45,17 @@ WZ_D uint32_t block_exclusive256(uint32_t v, uint32_t* sh, uint32_t& total) {
     return before + inc - v;
 }
 
-// Exact |value - original| <= bound with a TwoDiff tie check
-// (cpu/src/repair_avx2.cpp:48-56).
-WZ_D bool within_bound(double value, double original, double bound) {
-    const double high = value - original, magnitude = __builtin_fabs(high);
-    if (!__builtin_isfinite(high) || magnitude > bound) return false;
-    if (magnitude < bound) return true;
-    const double bv = value - high, av = high + bv;
-    const double br = bv - original, ar = value - av, low = ar + br;
-    return !((high > 0 && low > 0) || (high < 0 && low < 0));
-}
 }  // namespace
 
 // Native repairs (cpu/src/repair_avx2.cpp:58-74) over row-major tiles of
-// kRepairTile indices. Round k of tile t covers indices t*4096 + 256k + [0,256)
-// (one coalesced load per wave per round); entries are ordered round, wave,
-// lane — i.e. by index — through ballot ranks and a 64-entry scan in LDS.
+// kRepairTile indices (one segment of the SegList each). Round k of tile t
+// covers indices t*4096 + 256k + [0,256) (one coalesced load per wave per
+// round); each wave owns the two flag-bitmap words of its 64 indices, and the
+// tile's entries are compacted in index order through ballot ranks and a
+// 64-entry scan in LDS.
 template <class T>
 __global__ void __launch_bounds__(256) repair_kernel(const T* __restrict__ original, const T* __restrict__ rec,
-                                                     uint32_t n, double bound, TileList r,
+                                                     uint32_t n, double bound, SegList r,
                                                      uint32_t* __restrict__ status) {
     constexpr uint32_t rounds = kRepairTile / 256;
     __shared__ uint32_t cnt[rounds * 4];
@@ -79,19 +71,13 @@ __global__ void __launch_bounds__(256) repair_kernel(const T* __restrict__ origi
         const uint64_t i = tile0 + 256u * k + threadIdx.x;
         bool patch = false;
         err[k] = T(0);
-        if (i < n) {
-            T value = rec[i];
-            const T o = original[i];
-            const T e = value - o;
-            if (!__builtin_isfinite(value) || !__builtin_isfinite(e)) ok = false;
-            if (__builtin_fabs(e) > nb) {
-                patch = true;
-                err[k] = e;
-                value = value - e;
-            }
-            if (!__builtin_isfinite(value) || !within_bound(double(value), double(o), bound)) ok = false;
-        }
+        if (i < n) patch = repair_decision<T>(rec[i], original[i], nb, bound, &err[k], &ok);
         ballots[k] = __ballot(patch);
+        const uint64_t i0 = i - lane;  // multiple of 64
+        if (lane == 0 && i0 < n) {
+            r.bitmap[i0 >> 5] = uint32_t(ballots[k]);
+            if (i0 + 32 < n) r.bitmap[(i0 >> 5) + 1] = uint32_t(ballots[k] >> 32);
+        }
         if (lane == 0) cnt[4 * k + wv] = uint32_t(__popcll(ballots[k]));
     }
     __syncthreads();
@@ -102,8 +88,8 @@ __global__ void __launch_bounds__(256) repair_kernel(const T* __restrict__ origi
         if (lane == 63) r.count[blockIdx.x] = inc;
     }
     __syncthreads();
-    uint32_t* idx = r.index + tile0;
-    T* val = static_cast<T*>(r.payload) + tile0;
+    uint32_t* idx = r.index + uint64_t(blockIdx.x) * r.cap;
+    T* val = static_cast<T*>(r.payload) + uint64_t(blockIdx.x) * r.cap;
 #pragma unroll
     for (uint32_t k = 0; k < rounds; ++k)
         if ((ballots[k] >> lane) & 1ull) {
@@ -115,7 +101,7 @@ __global__ void __launch_bounds__(256) repair_kernel(const T* __restrict__ origi
 }
 
 template <class T>
-void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, TileList r,
+void detect_repairs(const T* original, const T* reconstructed, uint32_t n, double bound, SegList r,
                     uint32_t* status, hipStream_t s) {
     const uint32_t tiles = (n + kRepairTile - 1) / kRepairTile;
     hipLaunchKernelGGL((repair_kernel<T>), dim3(tiles), dim3(256), 0, s, original, reconstructed, n, bound, r,
@@ -123,17 +109,61 @@ void detect_repairs(const T* original, const T* reconstructed, uint32_t n, doubl
 }
 
 // ------------------------------------------------------------------ ordering
-// seg[k] = popcount of q-outlier bitmap bits [1024k, 1024k+1024).
-__global__ void __launch_bounds__(256) segment_count(const uint32_t* __restrict__ bm, uint32_t words, uint32_t segs,
-                                                     uint32_t* __restrict__ seg, const uint32_t* __restrict__ cnt) {
-    if (*cnt == 0) return;
-    const uint32_t k = blockIdx.x * 8 + (threadIdx.x >> 5);  // 32 lanes per segment
-    if (k >= segs) return;
-    const uint32_t w = k * 32 + (threadIdx.x & 31u);
-    uint32_t c = w < words ? uint32_t(__builtin_popcount(bm[w])) : 0u;
+// Index-order rank of a flagged sample: every flag bitmap word gets the count
+// of flags before it inside its block of kPrefixWords words (wpre), and every
+// block the count before it (bsum, scanned); then rank = bsum[block] +
+// wpre[word] + popcount of the lower bits of the word.
+constexpr uint32_t kPrefixWords = 8192;  // 256 threads x 32 words
+
+__global__ void __launch_bounds__(256) word_prefix(const uint32_t* __restrict__ bm, uint32_t words,
+                                                   uint32_t* __restrict__ wpre, uint32_t* __restrict__ bsum,
+                                                   const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    __shared__ uint32_t sh[4];
+    const uint64_t w0 = uint64_t(blockIdx.x) * kPrefixWords + 32u * threadIdx.x;
+    uint32_t c[32];
+    if (w0 + 32 <= words) {
+        const uint4* v = reinterpret_cast<const uint4*>(bm + w0);
 #pragma unroll
-    for (int d = 16; d >= 1; d >>= 1) c += __shfl_xor(c, d, 32);
-    if ((threadIdx.x & 31u) == 0) seg[k] = c;
+        for (int q = 0; q < 8; ++q) {
+            const uint4 x = v[q];
+            c[4 * q] = __builtin_popcount(x.x);
+            c[4 * q + 1] = __builtin_popcount(x.y);
+            c[4 * q + 2] = __builtin_popcount(x.z);
+            c[4 * q + 3] = __builtin_popcount(x.w);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) c[i] = w0 + i < words ? uint32_t(__builtin_popcount(bm[w0 + i])) : 0u;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) sum += c[i];
+    uint32_t total;
+    uint32_t run = block_exclusive256(sum, sh, total);
+    if (w0 + 32 <= words) {
+        uint4* o = reinterpret_cast<uint4*>(wpre + w0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            uint4 x;
+            x.x = run;
+            run += c[4 * q];
+            x.y = run;
+            run += c[4 * q + 1];
+            x.z = run;
+            run += c[4 * q + 2];
+            x.w = run;
+            run += c[4 * q + 3];
+            o[q] = x;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            if (w0 + i < words) wpre[w0 + i] = run;
+            run += c[i];
+        }
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
 // In-place exclusive scan of a[0..len) by one 1024-thread workgroup; the sum
@@ -202,37 +232,39 @@ __global__ void layout_kernel(uint8_t* __restrict__ archive, uint64_t capacity, 
     for (uint64_t p = L.qv + 4ull * nq; p < L.end; ++p) archive[p] = 0;
 }
 
-WZ_D uint32_t sorted_position(const uint32_t* __restrict__ bm, const uint32_t* __restrict__ seg, uint32_t idx) {
-    const uint32_t k = idx >> 10, w = idx >> 5;
-    uint32_t pos = seg[k];
-    for (uint32_t v = k * 32; v < w; ++v) pos += uint32_t(__builtin_popcount(bm[v]));
-    return pos + uint32_t(__builtin_popcount(bm[w] & ((1u << (idx & 31u)) - 1u)));
-}
+struct Ranks {
+    const uint32_t* bm;
+    const uint32_t* wpre;
+    const uint32_t* bsum;  // exclusive, per kPrefixWords block
+    WZ_D uint32_t of(uint32_t idx) const {
+        const uint32_t w = idx >> 5;
+        return bsum[w / kPrefixWords] + wpre[w] + uint32_t(__builtin_popcount(bm[w] & ((1u << (idx & 31u)) - 1u)));
+    }
+};
 
-// Repairs: tile k's entries go to [prefix[k], prefix[k]+count) (counts were
-// scanned in place, so count = next prefix - prefix).
+// Repairs: segment k's entries go to their index-order rank (bitmap popcounts).
 template <class T>
-__global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archive, TileList rp, uint32_t tiles,
+__global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archive, SegList rp, Ranks rank,
                                                      const EncodeResult* __restrict__ res) {
     if (res->status) return;
-    const uint32_t k = blockIdx.x;
+    const uint32_t k = blockIdx.x, c = rp.count[k];
+    if (!c) return;
     const uint32_t np = res->np;
-    const uint32_t first = rp.count[k], last = k + 1 < tiles ? rp.count[k + 1] : np;
-    if (first == last) return;
     const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * sizeof(T), res->nq);
-    const uint32_t* idx = rp.index + uint64_t(k) * kRepairTile;
-    const T* val = static_cast<const T*>(rp.payload) + uint64_t(k) * kRepairTile;
+    const uint32_t* idx = rp.index + uint64_t(k) * rp.cap;
+    const T* val = static_cast<const T*>(rp.payload) + uint64_t(k) * rp.cap;
     uint32_t* oi = reinterpret_cast<uint32_t*>(archive + L.pi);
     T* ov = reinterpret_cast<T*>(archive + L.pv);
-    for (uint32_t j = threadIdx.x; j < last - first; j += 256) {
-        oi[first + j] = idx[j];
-        ov[first + j] = val[j];
+    for (uint32_t j = threadIdx.x; j < c; j += 256) {
+        const uint32_t i = idx[j];
+        const uint32_t pos = rank.of(i);
+        oi[pos] = i;
+        ov[pos] = val[j];
     }
 }
 
-__global__ void __launch_bounds__(256) place_qoutliers(uint8_t* __restrict__ archive, SideList qo,
-                                                       const uint32_t* __restrict__ seg, uint64_t repair_bytes_per,
-                                                       const EncodeResult* __restrict__ res) {
+__global__ void __launch_bounds__(256) place_qoutliers(uint8_t* __restrict__ archive, SideList qo, Ranks rank,
+                                                       uint64_t repair_bytes_per, const EncodeResult* __restrict__ res) {
     if (res->status) return;
     const uint32_t np = res->np, nq = res->nq;
     if (!nq) return;
@@ -240,28 +272,43 @@ __global__ void __launch_bounds__(256) place_qoutliers(uint8_t* __restrict__ arc
     const uint32_t stride = gridDim.x * 256u;
     for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < nq; j += stride) {
         const uint32_t idx = qo.index[j];
-        const uint32_t pos = sorted_position(qo.bitmap, seg, idx);
+        const uint32_t pos = rank.of(idx);
         reinterpret_cast<uint32_t*>(archive + L.qi)[pos] = idx;
         reinterpret_cast<uint32_t*>(archive + L.qv)[pos] = static_cast<const uint32_t*>(qo.payload)[j];
     }
 }
 
+// Scratch layout: [repair wpre | repair bsum | q wpre | q bsum], each part a
+// multiple of 64 words so that the 16-byte vector accesses stay aligned.
+namespace {
+uint64_t pad64(uint64_t w) { return (w + 63) & ~uint64_t(63); }
+}  // namespace
+uint64_t finish_scratch_words(uint32_t n) {
+    const uint64_t words = (uint64_t(n) + 31) / 32, blocks = (words + kPrefixWords - 1) / kPrefixWords;
+    return 2 * (pad64(words) + pad64(blocks));
+}
+
 template <class T>
-void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, TileList repairs, SideList qout, uint32_t* seg,
+void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SegList repairs, SideList qout, uint32_t* scratch,
                     uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz, double quantum, EncodeResult* result,
                     hipStream_t s) {
-    const uint32_t words = (n + 31) / 32, segs = (n + 1023) / 1024;
-    const uint32_t tiles = (n + kRepairTile - 1) / kRepairTile;
-    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, repairs.count, tiles, repairs.total,
+    const uint32_t words = (n + 31) / 32, blocks = (words + kPrefixWords - 1) / kPrefixWords;
+    const uint64_t half = pad64(words) + pad64(blocks);
+    const Ranks rr{repairs.bitmap, scratch, scratch + pad64(words)};
+    const Ranks qr{qout.bitmap, scratch + half, scratch + half + pad64(words)};
+    uint32_t* np_total = &result->pad;  // repair total from the bitmap scan
+    hipLaunchKernelGGL(word_prefix, dim3(blocks), dim3(256), 0, s, repairs.bitmap, words, const_cast<uint32_t*>(rr.wpre),
+                       const_cast<uint32_t*>(rr.bsum), static_cast<const uint32_t*>(nullptr));
+    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, const_cast<uint32_t*>(rr.bsum), blocks, np_total,
                        static_cast<const uint32_t*>(nullptr));
-    hipLaunchKernelGGL(segment_count, dim3((segs + 7) / 8), dim3(256), 0, s, qout.bitmap, words, segs, seg,
-                       qout.count);
-    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, seg, segs, static_cast<uint32_t*>(nullptr),
-                       qout.count);
-    hipLaunchKernelGGL((layout_kernel<T>), dim3(1), dim3(1), 0, s, archive, capacity, n, repairs.total, qout, flags,
-                       lx, lz, bz, quantum, result);
-    hipLaunchKernelGGL((place_repairs<T>), dim3(tiles), dim3(256), 0, s, archive, repairs, tiles, result);
-    hipLaunchKernelGGL(place_qoutliers, dim3(256), dim3(256), 0, s, archive, qout, seg, uint64_t(sizeof(T)), result);
+    hipLaunchKernelGGL(word_prefix, dim3(blocks), dim3(256), 0, s, qout.bitmap, words, const_cast<uint32_t*>(qr.wpre),
+                       const_cast<uint32_t*>(qr.bsum), qout.count);
+    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, const_cast<uint32_t*>(qr.bsum), blocks,
+                       static_cast<uint32_t*>(nullptr), qout.count);
+    hipLaunchKernelGGL((layout_kernel<T>), dim3(1), dim3(1), 0, s, archive, capacity, n, np_total, qout, flags, lx,
+                       lz, bz, quantum, result);
+    hipLaunchKernelGGL((place_repairs<T>), dim3(repairs.segments), dim3(256), 0, s, archive, repairs, rr, result);
+    hipLaunchKernelGGL(place_qoutliers, dim3(256), dim3(256), 0, s, archive, qout, qr, uint64_t(sizeof(T)), result);
 }
 
 // ------------------------------------------------------------------ decode patches
@@ -290,7 +337,9 @@ __global__ void rpatch_kernel(T* __restrict__ out, const uint32_t* __restrict__ 
         atomicOr(status, uint32_t(kCorrupt));
         return;
     }
-    out[i] = out[i] - e;  // cpu/src/codec.cpp:330
+    const T v = out[i] - e;  // cpu/src/codec.cpp:330
+    out[i] = v;
+    if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
 }
 
 template <class T>
@@ -323,8 +372,8 @@ void check_finite(const T* out, uint32_t n, uint32_t* status, hipStream_t s) {
 }
 
 #define WZ_INST(T)                                                                                                \
-    template void detect_repairs<T>(const T*, const T*, uint32_t, double, TileList, uint32_t*, hipStream_t);        \
-    template void finish_archive<T>(uint8_t*, uint64_t, uint32_t, TileList, SideList, uint32_t*, uint8_t, uint8_t, \
+    template void detect_repairs<T>(const T*, const T*, uint32_t, double, SegList, uint32_t*, hipStream_t);         \
+    template void finish_archive<T>(uint8_t*, uint64_t, uint32_t, SegList, SideList, uint32_t*, uint8_t, uint8_t,  \
                                     uint8_t, uint8_t, double, EncodeResult*, hipStream_t);                         \
     template void apply_qoutliers<T>(T*, const uint8_t*, const uint8_t*, uint32_t, uint32_t, double, uint32_t*,    \
                                      hipStream_t);                                                                \

@@ -20,55 +20,10 @@
 #include "common.hpp"
 #include "fused.hpp"
 #include "launch.hpp"
+#include "lifting.hpp"
 #include "profile.hpp"
 
 namespace wz {
-
-// Forward lifting of P pairs in registers (cpu/src/transform.cpp:136-145).
-template <class T, int P>
-WZ_D void lift_forward(T (&e)[P], T (&o)[P]) {
-    using C = Lift<T>;
-#pragma unroll
-    for (int j = 0; j < P - 1; ++j) o[j] = fmadd(C::a, e[j] + e[j + 1], o[j]);
-#pragma unroll
-    for (int j = 1; j < P - 1; ++j) e[j] = fmadd(C::b, o[j - 1] + o[j], e[j]);
-#pragma unroll
-    for (int j = 1; j < P - 2; ++j) o[j] = fmadd(C::g, e[j] + e[j + 1], o[j]);
-#pragma unroll
-    for (int j = 2; j < P - 2; ++j) e[j] = fmadd(C::d, o[j - 1] + o[j], e[j]);
-#pragma unroll
-    for (int j = 2; j < P - 2; ++j) {
-        e[j] = e[j] * C::k;
-        o[j] = o[j] * (-C::ik);
-    }
-}
-
-// Inverse lifting of P pairs; Folded contracts the two tile-end sums
-// (cpu/src/transform.cpp:117-135).
-template <class T, int P, bool Folded>
-WZ_D void lift_inverse(T (&e)[P], T (&o)[P]) {
-    using C = Lift<T>;
-    const T first_odd = o[0], last_odd = o[P - 1];
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        e[j] = e[j] * C::ik;
-        o[j] = o[j] * (-C::k);
-    }
-#pragma unroll
-    for (int j = 1; j < P; ++j) {
-        T s;
-        if (Folded && j == 1) s = fmadd(first_odd, -C::k, o[1]);
-        else if (Folded && j == P - 1) s = fmadd(last_odd, -C::k, o[P - 2]);
-        else s = o[j - 1] + o[j];
-        e[j] = fmadd(-C::d, s, e[j]);
-    }
-#pragma unroll
-    for (int j = 1; j < P - 1; ++j) o[j] = fmadd(-C::g, e[j] + e[j + 1], o[j]);
-#pragma unroll
-    for (int j = 2; j < P - 1; ++j) e[j] = fmadd(-C::b, o[j - 1] + o[j], e[j]);
-#pragma unroll
-    for (int j = 2; j < P - 2; ++j) o[j] = fmadd(-C::a, e[j] + e[j + 1], o[j]);
-}
 
 template <class T>
 WZ_D bool finite_value(T v) {
@@ -232,6 +187,18 @@ uint32_t z_segment(Dims n) {
     return uint32_t(sz < 16 ? 16 : sz);
 }
 
+// Leading levels whose box fills the GPU with fused workgroups (>= 512 of
+// them at the smallest z segment); coarser boxes are cheaper as axis passes.
+unsigned fused_levels(Dims n, unsigned levels) {
+    unsigned f = 0;
+    for (; f < levels; ++f) {
+        const uint64_t wgs = uint64_t((n.nx + 63) / 64) * ((n.ny + 31) / 32) * ((n.nz + 15) / 16);
+        if (wgs < 512) break;
+        n = shrink(n, true, true);
+    }
+    return f;
+}
+
 // Compact boxes of levels >= 1 live in `scratch`: level l's box at offset 0
 // for odd l and after the largest level-1 box for even l (two never-aliasing
 // halves, each <= n/8 + lines).
@@ -280,7 +247,8 @@ void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels
 }
 
 template <class T>
-void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, hipStream_t s) {
+void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, const Level0Emit<T>* emit,
+                   hipStream_t s) {
     Dims sizes[5], n = full;
     for (unsigned l = 0; l < levels; ++l) { sizes[l] = n; n = shrink(n, true, true); }
     const T* lo = coeff;
@@ -288,7 +256,7 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, hip
     static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
     for (int l = int(levels) - 1; l >= 0; --l) {
         StageClock clock(names[l], s);
-        InvLevel<T> a;
+        InvLevel<T> a{};
         a.hi = coeff;
         a.hpy = full.nx;
         a.hpz = uint64_t(full.nx) * full.ny;
@@ -306,11 +274,30 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, hip
             a.opz = uint64_t(sizes[l].nx) * sizes[l].ny;
         }
         a.sz = z_segment(sizes[l]);
+        a.emit = kEmitStore;
+        if (l == 0 && emit) {
+            a.emit = emit->mode;
+            a.status = emit->status;
+            a.orig = emit->orig;
+            a.bound = emit->bound;
+            a.rep = emit->rep;
+            if (emit->mode == kEmitRepair && (full.nx & 31u))  // atomics path needs a clean bitmap
+                zero_words(emit->rep.bitmap, (uint64_t(full.nx) * full.ny * full.nz + 31) / 32, s);
+        }
         launch_inv_level<T>(a, s);
         lo = a.out;
         lpy = a.opy;
         lpz = a.opz;
     }
+}
+
+void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, uint32_t* segments, uint32_t* cap) {
+    *segments = 0;
+    *cap = 0;
+    if (!dyadic || !lxy || !fused_enabled() || !fused_levels(full, lxy)) return;
+    const uint32_t sz = z_segment(full);
+    *segments = uint32_t(inv_segments(full, sz));
+    *cap = 64u * 32u * sz;
 }
 
 // Full multi-level transform (cpu/src/transform.cpp:409-456).
@@ -325,18 +312,20 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
         if (input != coeff) copy_box<T>(input, coeff, full, full, s);
         return;
     }
-    if (dyadic && fused_enabled()) {
-        if (input == coeff) {  // the level-0 kernel must not read what it writes
-            copy_box<T>(input, tmp1, full, full, s);
-            input = tmp1;
-        }
-        fused_forward<T>(input, coeff, tmp0, full, lxy, status, s);
-        return;
-    }
     if (dyadic) {
+        const unsigned fused = fused_enabled() ? fused_levels(full, lxy) : 0u;
         Dims act = full;
+        if (fused) {
+            if (input == coeff) {  // the level-0 kernel must not read what it writes
+                copy_box<T>(input, tmp1, full, full, s);
+                input = tmp1;
+            }
+            fused_forward<T>(input, coeff, tmp0, full, fused, status, s);  // leaves LLL in coeff
+            p.first = false;
+            for (unsigned l = 0; l < fused; ++l) act = shrink(act, true, true);
+        }
         static const int yxz[3] = {1, 0, 2};
-        for (unsigned l = 0; l < lxy; ++l) {
+        for (unsigned l = fused; l < lxy; ++l) {
             p.level(full, act, yxz, 3, false, l);
             act = shrink(act, true, true);
         }
@@ -357,28 +346,29 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
 }
 
 template <class T>
-void transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       hipStream_t s) {
+bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       hipStream_t s, const Level0Emit<T>* emit) {
     Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s};
     if (lxy == 0 && lz == 0) {
         if (output != coeff) copy_box<T>(coeff, output, full, full, s);
-        return;
-    }
-    if (dyadic && fused_enabled()) {
-        if (output == coeff) {  // the level-0 kernel must not overwrite what it reads
-            fused_inverse<T>(coeff, tmp1, tmp0, full, lxy, s);
-            copy_box<T>(tmp1, coeff, full, full, s);
-        } else {
-            fused_inverse<T>(coeff, output, tmp0, full, lxy, s);
-        }
-        return;
+        return false;
     }
     if (dyadic) {
+        const unsigned fused = fused_enabled() ? fused_levels(full, lxy) : 0u;
         Dims sizes[5], act = full;
         for (unsigned l = 0; l < lxy; ++l) { sizes[l] = act; act = shrink(act, true, true); }
         static const int zxy[3] = {2, 0, 1};
-        for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, sizes[l], zxy, 3, l == 0, unsigned(l));
-        return;
+        // coarse levels as passes in coeff; then the fused levels (coarse LLL read from coeff)
+        for (int l = int(lxy) - 1; l >= int(fused); --l)
+            p.level(full, sizes[l], zxy, 3, !fused && l == 0, unsigned(l));
+        if (!fused) return false;
+        if (output == coeff) {  // the level-0 kernel must not overwrite what it reads
+            fused_inverse<T>(coeff, tmp1, tmp0, full, fused, emit, s);
+            if (!emit || emit->mode != kEmitRepair) copy_box<T>(tmp1, coeff, full, full, s);
+        } else {
+            fused_inverse<T>(coeff, output, tmp0, full, fused, emit, s);
+        }
+        return emit != nullptr;
     }
     Dims xy[5], zs[5], act = full;
     for (unsigned l = 0; l < lxy; ++l) { xy[l] = act; act = shrink(act, true, false); }
@@ -388,14 +378,16 @@ void transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
     static const int zonly[1] = {2};
     for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, xy[l], xyo, 2, lz == 0 && l == 0, unsigned(l));
     for (int l = int(lz) - 1; l >= 0; --l) p.level(full, zs[l], zonly, 1, l == 0, unsigned(l));
+    return false;
 }
 
 template void transform_forward<float>(const float*, float*, float*, float*, Dims, unsigned, unsigned, bool,
                                        uint32_t*, hipStream_t);
 template void transform_forward<double>(const double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
                                         uint32_t*, hipStream_t);
-template void transform_inverse<float>(float*, float*, float*, float*, Dims, unsigned, unsigned, bool, hipStream_t);
-template void transform_inverse<double>(double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
-                                        hipStream_t);
+template bool transform_inverse<float>(float*, float*, float*, float*, Dims, unsigned, unsigned, bool, hipStream_t,
+                                       const Level0Emit<float>*);
+template bool transform_inverse<double>(double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
+                                        hipStream_t, const Level0Emit<double>*);
 
 }  // namespace wz

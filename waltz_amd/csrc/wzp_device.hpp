@@ -1,0 +1,308 @@
+// Device pieces of the WZP stream shared by the single-field and batched
+// kernels: block scans, byte loads, the decoder's sample sinks and the
+// per-chunk decoder (cpu/src/wzp_cpu.cpp:595-949).
+#pragma once
+
+#include "bits.hpp"
+#include "common.hpp"
+#include "launch.hpp"
+#include "pairbrick.hpp"
+
+namespace wz {
+
+namespace {
+constexpr uint32_t kThreads = 256;
+
+WZ_D uint32_t wave_inclusive(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan for 256 threads; `sh` holds >= 4 words.
+WZ_D uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const uint32_t inc = wave_inclusive(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 63u) sh[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kThreads / 64; ++i) {
+        const uint32_t s = sh[i];
+        before += i < w ? s : 0u;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return before + inc - v;
+}
+
+WZ_D uint32_t load24(const uint8_t* p) { return uint32_t(p[0]) | (uint32_t(p[1]) << 8) | (uint32_t(p[2]) << 16); }
+WZ_D uint32_t load32(const uint8_t* p) { return load24(p) | (uint32_t(p[3]) << 24); }
+
+constexpr unsigned long long kFlagAggregate = 1ull << 62;
+constexpr unsigned long long kFlagInclusive = 2ull << 62;
+constexpr unsigned long long kValueMask = (1ull << 62) - 1;
+constexpr uint32_t kSpinLimit = 1u << 26;
+
+WZ_D uint32_t chunk_elements(uint32_t n, uint32_t c) {
+    const uint32_t rest = n - c * kChunk;
+    return rest < kChunk ? rest : kChunk;
+}
+}  // namespace
+
+// Four consecutive samples of one row (16-byte aligned in the paired-brick
+// geometry: rows start at multiples of 4 samples).
+WZ_D void load4(const float* p, float (&v)[4]) {
+    const float4 x = *reinterpret_cast<const float4*>(p);
+    v[0] = x.x;
+    v[1] = x.y;
+    v[2] = x.z;
+    v[3] = x.w;
+}
+WZ_D void load4(const double* p, double (&v)[4]) {
+    const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+WZ_D void store4(float* p, const float (&v)[4]) { *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]); }
+WZ_D void store4(double* p, const double (&v)[4]) {
+    reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+
+// Decoder outputs. store() takes thread t's magnitudes [32t, 32t + 32) of
+// chunk c and their sign word.
+struct MagSink {  // packed magnitude / sign arrays
+    uint16_t* mag;
+    uint32_t* sign;
+    WZ_D void store(uint32_t c, uint32_t t, uint32_t E, const uint32_t (&m)[32], uint32_t sw) const {
+        const uint32_t first = t * 32;
+        uint16_t* p = mag + uint64_t(c) * kChunk + first;
+        if (first < E) sign[uint64_t(c) * (kChunk / 32) + t] = sw;
+        if (first + 32 <= E) {
+            uint4* v = reinterpret_cast<uint4*>(p);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                v[q] = make_uint4(m[8 * q] | (m[8 * q + 1] << 16), m[8 * q + 2] | (m[8 * q + 3] << 16),
+                                  m[8 * q + 4] | (m[8 * q + 5] << 16), m[8 * q + 6] | (m[8 * q + 7] << 16));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (first + i < E) p[i] = uint16_t(m[i]);
+        }
+    }
+};
+
+// Dequantizes straight into the coefficient volume (paired-brick geometry),
+// as dequantize_kernel does per sample (cpu/src/codec.cpp:146-147).
+template <class T>
+struct BrickSink {
+    T* coeff;
+    uint32_t nx, ny;
+    T q;
+    WZ_D void store(uint32_t c, uint32_t t, uint32_t, const uint32_t (&m)[32], uint32_t sw) const {
+        uint32_t X, Y, Z;
+        pair_subblock(nx, ny, c, t, X, Y, Z);
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) {
+            const uint32_t dy = r & 3u, dz = r >> 2;
+            T d[4];
+#pragma unroll
+            for (uint32_t dx = 0; dx < 4; ++dx) {
+                const uint32_t i = subblock_rank(dx, dy, dz);
+                const T v = T(m[i]) * q;
+                d[dx] = ((sw >> i) & 1u) ? -v : v;
+            }
+            store4(coeff + (uint64_t(Z + dz) * ny + (Y + dy)) * nx + X, d);
+        }
+    }
+};
+
+// Decodes chunk c of a WZP stream of `bytes` bytes holding n magnitudes;
+// called by all 256 threads of a workgroup.
+template <class Sink>
+WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t n, uint32_t chunks, uint32_t groups,
+                       uint32_t c, const Sink& sink, uint32_t* __restrict__ status) {
+    __shared__ uint32_t tr[kChunk / 2];
+    __shared__ uint32_t ib32[(kChunkBytes + kSignSlot) / 4 + 4];  // staged chunk payload
+    __shared__ uint32_t b1w[kThreads / 4];  // B1 as 64 words (256 bytes)
+    __shared__ uint32_t b2w[8];             // B2 as 8 words (32 bytes)
+    __shared__ uint32_t sh[8];
+    __shared__ uint32_t info[4];            // offset, ok
+
+    const uint32_t t = threadIdx.x;
+    const uint32_t E = chunk_elements(n, c);
+    const uint32_t first = t * 32;
+    const uint64_t header = uint64_t(groups) * 4 + uint64_t(chunks) * 3;
+    const uint8_t* meta = in + uint64_t(groups) * 4;
+    const uint32_t mc = load24(meta + uint64_t(c) * 3);
+    const uint32_t code = mc & kRawCode, sc = mc >> 14;
+    const uint32_t raw_bytes = 2 * E;
+    const bool raw = code == kRawCode;
+    const uint32_t mbytes = code == 0 ? 0u : raw ? raw_bytes : code;
+    const uint32_t sbytes = code == 0 ? 0u : sc ? sc : kSignSlot;
+    if (t == 0) {
+        const uint32_t g = c / kGroup;
+        uint64_t off = load32(in + uint64_t(g) * 4);
+        uint32_t ok = 1;
+        for (uint32_t j = g * kGroup; j < c; ++j) {
+            const uint32_t mj = load24(meta + uint64_t(j) * 3);
+            const uint32_t cj = mj & kRawCode, sj = mj >> 14;
+            if (!cj) continue;
+            off += (cj == kRawCode ? 2 * chunk_elements(n, j) : cj) + (sj ? sj : kSignSlot);
+        }
+        if (code == 0 && sc != 0) ok = 0;
+        if (code && !raw && (code < 4 || code + 2 >= raw_bytes)) ok = 0;
+        if (code && sbytes > (E + 7) / 8) ok = 0;
+        if (header + off + mbytes + sbytes > bytes) ok = 0;
+        info[0] = uint32_t(off);
+        info[1] = ok;
+        if (!ok) atomicOr(status, uint32_t(kCorrupt));
+    }
+    __syncthreads();
+    if (!info[1]) return;
+    if (code == 0) {
+        uint32_t zero[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) zero[i] = 0;
+        sink.store(c, t, E, zero, 0u);
+        return;
+    }
+    // stage the chunk payload [g0, g1) in LDS with aligned word loads; the
+    // words never leave the stream [in, in + bytes)
+    const uintptr_t g0 = reinterpret_cast<uintptr_t>(in + header + info[0]), g1 = g0 + mbytes + sbytes;
+    const uintptr_t a0 = g0 & ~uintptr_t(3), limit = reinterpret_cast<uintptr_t>(in + bytes);
+    for (uintptr_t a = a0 + 4u * t; a < g1; a += 4u * kThreads) {
+        uint32_t v;
+        if (a >= reinterpret_cast<uintptr_t>(in) && a + 4 <= limit) {
+            v = *reinterpret_cast<const uint32_t*>(a);
+        } else {
+            v = 0;
+            for (uint32_t k = 0; k < 4; ++k)
+                if (a + k >= reinterpret_cast<uintptr_t>(in) && a + k < limit)
+                    v |= uint32_t(*reinterpret_cast<const uint8_t*>(a + k)) << (8 * k);
+        }
+        ib32[(a - a0) >> 2] = v;
+    }
+    __syncthreads();
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(ib32) + (g0 - a0);
+    uint32_t m[32];
+    bool bad = false;
+    if (raw) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            m[i] = first + i < E ? uint32_t(src[2 * (first + i)]) | (uint32_t(src[2 * (first + i) + 1]) << 8) : 0u;
+    } else {
+        // top-down: B3 -> B2 -> B1 -> B0 -> bytes, each level's changed bytes
+        // located by popcount ranks (cpu/src/wzp_cpu.cpp:595-660)
+        const uint32_t b3 = load32(src + code - 4);
+        const uint32_t K3 = popc32(b3);
+        const int32_t pos3 = int32_t(code) - 4 - int32_t(K3);
+        if (pos3 < 0) bad = true;
+        if (!bad && t < 32) {
+            const uint32_t r = popc32(t == 31 ? b3 : b3 & ((2u << t) - 1u));
+            const uint32_t v = r ? uint32_t(src[pos3 + int32_t(r) - 1]) : 0u;
+            reinterpret_cast<uint8_t*>(b2w)[t] = uint8_t(v);
+        }
+        __syncthreads();
+        uint32_t K2 = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) K2 += popc32(b2w[i]);
+        const int32_t pos2 = pos3 - int32_t(K2);
+        if (pos2 < 0) bad = true;
+        if (!bad) {
+            const uint32_t wd = t >> 5, bit = t & 31u;
+            uint32_t r = popc32(bit == 31 ? b2w[wd] : b2w[wd] & ((2u << bit) - 1u));
+            for (uint32_t i = 0; i < wd; ++i) r += popc32(b2w[i]);
+            const uint32_t v = r ? uint32_t(src[pos2 + int32_t(r) - 1]) : 0u;
+            reinterpret_cast<uint8_t*>(b1w)[t] = uint8_t(v);
+        }
+        __syncthreads();
+        const uint32_t b1t = reinterpret_cast<const uint8_t*>(b1w)[t];
+        uint32_t K1;
+        const uint32_t e1 = block_exclusive(popc32(b1t), sh, K1);
+        const int32_t pos1 = pos2 - int32_t(K1);
+        if (pos1 < 0) bad = true;
+        unsigned long long b0t = 0;
+        if (!bad) {
+            uint32_t r = e1;
+            uint32_t cur = r ? uint32_t(src[pos1 + int32_t(r) - 1]) : 0u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if ((b1t >> k) & 1u) cur = src[pos1 + int32_t(r++)];
+                b0t |= (unsigned long long)(cur & 0xFFu) << (8 * k);
+            }
+        }
+        {
+            const int32_t valid = int32_t(raw_bytes) - int32_t(64 * t);
+            const unsigned long long keep = valid <= 0 ? 0ull : valid >= 64 ? ~0ull : (1ull << valid) - 1ull;
+            if (b0t & ~keep) bad = true;
+        }
+        uint32_t K0;
+        const uint32_t e0 = block_exclusive(popc64(b0t), sh, K0);
+        if (pos1 - int32_t(K0) != 0) bad = true;
+        if (!bad) {
+            uint32_t r = e0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((b0t >> (4 * q + k)) & 1ull) word |= uint32_t(src[r++]) << (8 * k);
+                tr[16 * t + q] = word;
+            }
+        }
+        __syncthreads();
+        const uint16_t* tr16 = reinterpret_cast<const uint16_t*>(tr);
+        const uint32_t blocks = E / 16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t b = 2 * t + h;
+            if (b < blocks) {
+                uint32_t p[16];
+#pragma unroll
+                for (int pl = 0; pl < 16; ++pl) p[pl] = tr16[pl * blocks + b];
+                bit2_inverse(p);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) m[16 * h + j] = p[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const uint32_t i = first + 16 * h + j;
+                    m[16 * h + j] = i < E ? uint32_t(tr16[i]) : 0u;
+                }
+            }
+        }
+    }
+    uint32_t nzmask = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) nzmask |= uint32_t(m[i] != 0) << i;
+    uint32_t NNZ;
+    const uint32_t es = block_exclusive(popc32(nzmask), sh, NNZ);
+    if (NNZ == 0 || (NNZ + 7) / 8 != sbytes) bad = true;
+    if (bad) {
+        if (t == 0 || (threadIdx.x & 63u) == 0) atomicOr(status, uint32_t(kCorrupt));
+        return;
+    }
+    const uint8_t* ps = src + mbytes;
+    const uint32_t cnt = popc32(nzmask);
+    uint32_t bits = 0;
+    if (cnt) {
+        const uint32_t b = es >> 3, sh8 = es & 7u;
+        const uint32_t need = (sh8 + cnt + 7) >> 3;  // <= 5 bytes
+        unsigned long long v = 0;
+        for (uint32_t i = 0; i < need; ++i) v |= (unsigned long long)ps[b + i] << (8 * i);
+        bits = uint32_t(v >> sh8) & (cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u));
+    }
+    if (t == 0 && (NNZ & 7u) && (ps[sbytes - 1] >> (NNZ & 7u))) atomicOr(status, uint32_t(kCorrupt));
+    sink.store(c, t, E, m, expand_bits(bits, nzmask));
+}
+
+}  // namespace wz

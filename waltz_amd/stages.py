@@ -25,6 +25,19 @@ def transform(volume, levels_xy, levels_z, dyadic, inverse, out=None):
     return out
 
 
+def transform_adjoint(volume, levels_xy, levels_z, dyadic, of_inverse, out=None):
+    """Transpose of transform() with the same geometry: of_inverse=False maps a
+    coefficient gradient to samples (DWT^T), True maps a sample gradient to
+    coefficients (IDWT^T)."""
+    volume = volume.contiguous()
+    out = torch.empty_like(volume) if out is None else out
+    nx, ny, nz = _dims(volume)
+    check(lib().waltz_amd_transform_adjoint(_W[volume.dtype], int(of_inverse), volume.data_ptr(), out.data_ptr(), nx,
+                                            ny, nz, levels_xy, levels_z, int(dyadic), _stream(volume.device)),
+          "transform_adjoint")
+    return out
+
+
 def quantize(coeff, bz, quantum):
     """Returns (dequantized, mag u16 as int16 view, sign words as int32, q indices, q values).
 
