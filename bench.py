@@ -30,6 +30,7 @@ def parse():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-only", action="store_true", help="few calls, for rocprofv3 runs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and autograd legs")
     return ap.parse_args()
 
 
@@ -53,6 +54,118 @@ def stage_alg_bytes(name, n, w, wzp_bytes, sizes):
         "decode_dequantize": wzp_bytes + w * n,      # read stream, write coefficients
     }
     return table.get(name)
+
+
+def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps):
+    """256 float32 chunks of uneven shape (seeded mix of 32^3, 64x64x32,
+    96x64x48, 128x128x64; each its own k^-5/3 field), ABS 1e-3: decoded one
+    archive per call through the plain API, then in one batched pass (eager
+    and replayed as a captured graph) straight into one contiguous tensor."""
+    import numpy as np
+    from waltz_amd import batch
+    rng = np.random.default_rng(20261202 + 4)
+    kinds = [(32, 32, 32), (32, 64, 64), (48, 64, 96), (64, 128, 128)]  # (z, y, x)
+    shapes = [kinds[k] for k in rng.integers(0, len(kinds), 256)]
+    archives = []
+    for j, s in enumerate(shapes):
+        f = synthetic.grf(s, slope=-5.0 / 3.0, seed=20261202 + 4 + 1000 * (j + 1))
+        archives.append(waltz_amd.compress(f, abs_error=1e-3).clone())
+    packed, spans = batch.pack(archives)
+    dec = batch.BatchDecoder(shapes, torch.float32)
+    out_plain = torch.empty(dec.numel, dtype=torch.float32, device="cuda")
+    views = dec.views(out_plain)
+
+    def plain():
+        for a, s, v in zip(archives, shapes, views):
+            waltz_amd.decompress(a, shape=s, dtype=torch.float32, out=v)
+
+    out_batch = torch.empty_like(out_plain)
+    plain()
+    dec.decode(packed, spans, out=out_batch)
+    identical = bool(torch.equal(out_plain, out_batch))
+    graph = torch.cuda.CUDAGraph()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        with torch.cuda.graph(graph, stream=stream):
+            dec.decode(packed, spans, out=out_batch, check=False)
+    torch.cuda.current_stream().wait_stream(stream)
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return statistics.median(ts)
+
+    t_plain = timed(plain, 3)
+    t_batch = timed(lambda: dec.decode(packed, spans, out=out_batch), 5)
+    t_graph = timed(graph.replay, 5)
+    dec.check()
+    identical = identical and bool(torch.equal(out_plain, out_batch))
+    nbytes = dec.numel * 4
+    g_batch, g_graph, g_plain = nbytes / t_batch / 1e9, nbytes / t_graph / 1e9, nbytes / t_plain / 1e9
+    return {"chunks": len(shapes), "samples": dec.numel, "archive_bytes": int(packed.numel()),
+            "per_chunk_calls_GBps": round(g_plain, 2), "batched_GBps": round(g_batch, 2),
+            "graph_replay_GBps": round(g_graph, 2), "batched_over_per_chunk": round(g_batch / g_plain, 2),
+            "graph_over_per_chunk": round(g_graph / g_plain, 2),
+            "graph_over_single_field_decompress": round(g_graph / single_field_decompress_gbps, 3),
+            "bit_identical_to_per_chunk_decode": identical}
+
+
+def autograd_leg(torch, waltz_amd, synthetic):
+    """Batch of 8 256^3 float32 k^-5/3 fields at REL 1e-3: loss =
+    MSE(IDWT(dequant(q)), field + N(0, 0.05^2)), backward to the coefficients
+    through the adjoint; timed beside the plain decode of the same archives."""
+    from waltz_amd import autograd, stages
+    shape = (256, 256, 256)
+    lxy, lz, dyadic = autograd.default_levels(shape)
+    items = []
+    for b in range(8):
+        f = synthetic.grf(shape, slope=-5.0 / 3.0, seed=20261202 + 5 + 1000 * (b + 1))
+        bound = 1e-3 * (float(f.max()) - float(f.min()))
+        archive = waltz_amd.compress(f, abs_error=bound).clone()
+        quantum = 1.5 * bound
+        bz = int(archive[3])
+        coeff = stages.quantize(stages.transform(f, lxy, lz, dyadic, inverse=False), bz, quantum)[0]
+        g = torch.Generator(device="cuda").manual_seed(20261202 + 5 + b)
+        target = f + 0.05 * torch.randn(shape, generator=g, device="cuda")
+        items.append((archive, coeff, target))
+
+    def step():
+        total = 0.0
+        for _, coeff, target in items:
+            c = coeff.detach().requires_grad_()
+            loss = torch.nn.functional.mse_loss(autograd.idwt(c, (lxy, lz, dyadic)), target)
+            loss.backward()
+            total += float(loss.detach())
+        return total
+
+    def decode():
+        for archive, _, _ in items:
+            waltz_amd.decompress(archive, shape=shape, dtype=torch.float32)
+
+    step()
+    decode()
+    ts, td = [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        decode()
+        torch.cuda.synchronize()
+        td.append(time.perf_counter() - t)
+    nbytes = 8 * 256 ** 3 * 4
+    t_step, t_dec = statistics.median(ts), statistics.median(td)
+    return {"batch": 8, "shape_zyx": list(shape), "forward_backward_ms": round(t_step * 1e3, 3),
+            "forward_backward_GBps": round(nbytes / t_step / 1e9, 2), "plain_decode_ms": round(t_dec * 1e3, 3),
+            "forward_backward_over_plain_decode": round(t_dec / t_step, 3)}
 
 
 def main():
@@ -205,6 +318,14 @@ def main():
                      "archive_bytes": len(arch2_host), "max_abs_error_fp64": max_err2},
         "stages_ms": stage_table, "roofline": roofline,
     }
+
+    if not args.no_extra:
+        log("batched leg ...")
+        result["batched"] = batched_leg(torch, waltz_amd, synthetic, d_gbps)
+        log("batched:", json.dumps(result["batched"]))
+        log("autograd leg ...")
+        result["autograd"] = autograd_leg(torch, waltz_amd, synthetic)
+        log("autograd:", json.dumps(result["autograd"]))
 
     # CPU leg: byte identity with the reference CPU codec + its timing on this host
     if not args.no_cpu:

@@ -40,6 +40,20 @@ int waltz_amd_transform(int width, int inverse, const void* input, void* output,
 int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, void* output, uint32_t nx, uint32_t ny,
                                 uint32_t nz, unsigned levels_xy, unsigned levels_z, int dyadic, void* stream);
 
+/* Bit-neutral launch tuning of the dyadic schedule (no reference
+ * counterpart): autotune times the fused level kernels of both directions at
+ * each candidate z segment for this geometry and type on this device and keeps
+ * the fastest (process-wide, per geometry); set_tuning fixes the number of
+ * fused levels and their five z segments (positive multiples of 16) for one
+ * direction; clear_tuning restores the built-in heuristic; tuning_report
+ * writes the table as JSON (returns its full length). Results never depend on
+ * these choices. */
+int waltz_amd_autotune(int width, uint32_t nx, uint32_t ny, uint32_t nz, void* stream);
+int waltz_amd_set_tuning(int width, uint32_t nx, uint32_t ny, uint32_t nz, int inverse, unsigned fused_levels,
+                         const uint32_t* z_segments);
+void waltz_amd_clear_tuning(void);
+size_t waltz_amd_tuning_report(char* buf, size_t cap);
+
 /* Quantization + Morton packing (cpu/src/codec.cpp:198-234): coeff is
  * replaced by dequantized values; mag[n], sign[ceil(n/32)] packed order;
  * q-outliers sorted by index into qidx/qval (capacity n), count in *nq. */

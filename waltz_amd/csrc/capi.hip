@@ -3,6 +3,7 @@
 // over the pipelines in codec.hip; exceptions become -1 / diagnostics here.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -260,6 +261,39 @@ int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, vo
         wz::hip_check(hipGetLastError(), "adjoint launch");
         wz::hip_check(hipStreamSynchronize(s), "transform adjoint");
     });
+}
+
+int waltz_amd_autotune(int width, uint32_t nx, uint32_t ny, uint32_t nz, void* stream) {
+    return guarded([&] {
+        if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
+        const wz::Dims d = dims_of(nx, ny, nz);
+        if (width == 4) wz::autotune<float>(d, as_stream(stream));
+        else wz::autotune<double>(d, as_stream(stream));
+    });
+}
+
+int waltz_amd_set_tuning(int width, uint32_t nx, uint32_t ny, uint32_t nz, int inverse, unsigned fused_levels,
+                         const uint32_t* z_segments) {
+    return guarded([&] {
+        if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
+        if (!z_segments) throw std::invalid_argument("null z segments");
+        wz::Tuning t{};
+        t.fused = fused_levels;
+        for (int l = 0; l < 5; ++l) t.sz[l] = z_segments[l];
+        wz::set_tuning(dims_of(nx, ny, nz), unsigned(width), inverse != 0, t);
+    });
+}
+
+void waltz_amd_clear_tuning(void) { wz::clear_tuning(); }
+
+size_t waltz_amd_tuning_report(char* buf, size_t cap) {
+    const std::string r = wz::tuning_report();
+    if (buf && cap) {
+        const size_t k = std::min(cap - 1, r.size());
+        std::memcpy(buf, r.data(), k);
+        buf[k] = 0;
+    }
+    return r.size();
 }
 
 int waltz_amd_quantize(int width, void* coeff, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bz, double quantum,

@@ -172,7 +172,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     const uint32_t nq = qcap(n);
     const uint32_t rtiles = (n + kRepairTile - 1) / kRepairTile;
     uint32_t fsegs = 0, fcap = 0;  // repair segments of a fused finest inverse level
-    level0_repair_segments(g.d, g.lx, g.dyadic, &fsegs, &fcap);
+    level0_repair_segments(g.d, g.lx, g.dyadic, sizeof(T), &fsegs, &fcap);
     const uint64_t rslots = std::max<uint64_t>(uint64_t(rtiles) * kRepairTile, uint64_t(fsegs) * fcap);
     const uint32_t rsegs = std::max(rtiles, fsegs);
 

@@ -1,16 +1,16 @@
 // Fused one-launch-per-level CDF 9/7 lifting for the dyadic schedule.
 //
 // Forward level (order Y, X, Z; cpu/src/transform.cpp:425-430): a workgroup of
-// 320 threads (5 wave64s) owns a 64 x 32 output tile in XY and one z segment.
-// For every input plane of the segment (plus a 4-sample reflected halo on each
-// side) it loads the 72 x 40 reflected XY footprint into LDS, lifts Y (tile 8)
-// and X (tile 8) from LDS, and feeds the result into a per-position Z lifting
-// pipeline held in registers (5 values per position, the steps of
-// cpu/src/transform.cpp:136-145 in streaming order). Forward lifting is
-// bit-neutral to tiling, so every coefficient equals the reference's. Outputs of
-// each z pair are staged in LDS and written as 128-byte row segments: the LLL
-// band to a compact next-level box, the seven other bands to the coefficient
-// volume at their split-band positions.
+// 576 threads (9 wave64s) owns a 64 x 32 output tile in XY and one z segment.
+// For every pair of input planes of the segment (plus a 4-sample reflected halo
+// on each side) it loads the two 72 x 40 reflected XY footprints into LDS,
+// lifts Y (tile 8) and X (tile 8) from LDS, and feeds the result into a
+// per-position Z lifting pipeline held in registers (5 values per position,
+// the steps of cpu/src/transform.cpp:136-145 in streaming order). Forward
+// lifting is bit-neutral to tiling, so every coefficient equals the
+// reference's. Outputs of each z pair are staged in LDS and written with
+// 16-byte stores: the LLL band to a compact next-level box, the seven other
+// bands to the coefficient volume at their split-band positions.
 //
 // Inverse level (order Z, X, Y; cpu/src/transform.cpp:420-424) streams Z over
 // the 72 x 40 footprint of split coefficients in canonical tiles of 16 (f32) or
@@ -29,9 +29,9 @@ namespace {
 constexpr int TX = 64, TY = 32;            // output tile
 constexpr int FW = TX + 8, FH = TY + 8;    // reflected footprint
 constexpr int PA = FW + 1;                 // LDS pitch (odd: spreads banks)
-constexpr int THREADS = 320;
-constexpr int SLOTS = FW * FH / THREADS;   // 9 footprint samples per thread
-static_assert(FW * FH == SLOTS * THREADS, "footprint must tile the workgroup");
+constexpr int FT = 576;                    // threads (9 wave64s) per workgroup
+constexpr int FSLOTS = FW * FH / FT;       // 5 footprint samples per thread and plane
+static_assert(FW * FH == FSLOTS * FT, "footprint must tile the workgroup");
 
 // Workgroup -> tile, XCD-aware. The dispatcher deals consecutive workgroups
 // round-robin to the 8 XCDs, each with its own L2; XCD k is given the k-th
@@ -99,13 +99,39 @@ WZ_D void inv_lift(T (&e)[P], T (&o)[P]) {
     for (int j = 2; j < P - 2; ++j) o[j] = fmadd(-C::a, e[j] + e[j + 1], o[j]);
 }
 
+// Four consecutive staged values to a 16-byte aligned destination.
+WZ_D void store_vec4(float* dst, const float* v) {
+    *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(v);
+}
+WZ_D void store_vec4(double* dst, const double* v) {
+    reinterpret_cast<double2*>(dst)[0] = reinterpret_cast<const double2*>(v)[0];
+    reinterpret_cast<double2*>(dst)[1] = reinterpret_cast<const double2*>(v)[1];
+}
+
 }  // namespace
 
+// Forward level, 576 threads (9 wave64s) per 64 x 32 x sz tile; one step per
+// z pair (two planes):
+//   1. every thread stores its 5 footprint samples of both planes (loaded one
+//      step ahead) into A;
+//   2. Y: one (plane, column, 16-row window) per thread, into B;
+//   3. X: one (plane, row, 16-column window) per thread (512 of them), into X
+//      (aliasing A), low band at columns 0..31 and high band at 32..63;
+//   4. Z: 512 threads each stream 4 columns of a row through the Z lifting
+//      pipeline (five registers per column) and stage the two z bands of the
+//      finished pair in C (aliasing B);
+//   5. the staged rows are written with 16-byte stores: the LLL band to the
+//      next level's box, the seven other bands to the coefficient volume.
+constexpr int CP = TX + 4;             // row pitch of the X and C stages (16-byte rows)
+
 template <class T>
-__global__ void __launch_bounds__(THREADS) fwd_level_kernel(FwdLevel<T> a) {
-    __shared__ T A[FH * PA];            // input plane footprint
-    __shared__ T B[TY * PA];            // after Y: rows 0..15 low pairs, 16..31 high pairs
-    __shared__ T Cst[2 * TY * (TX + 1)]; // outputs of one z pair: [zband][row][x-low 32 | x-high 32]
+__global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
+    constexpr int ASZ = 2 * FH * PA, BSZ = 2 * TY * PA;
+    static_assert(2 * TY * CP <= ASZ && 2 * TY * CP <= BSZ, "stage aliasing");
+    __shared__ __attribute__((aligned(16))) T A[ASZ];  // two input planes; then X results [plane][row][CP]
+    __shared__ __attribute__((aligned(16))) T B[BSZ];  // after Y [plane][row][PA]; then C [zband][row][CP]
+    T* const Xs = A;
+    T* const Cs = B;
     const uint32_t t = threadIdx.x;
     Tile tl;
     if (!tile_of(tile_grid(a.n, a.sz), tl)) return;
@@ -113,100 +139,112 @@ __global__ void __launch_bounds__(THREADS) fwd_level_kernel(FwdLevel<T> a) {
     const Dims n = a.n;
     const uint32_t hx = half_len(n.nx), hy = half_len(n.ny), hz = half_len(n.nz);
 
-    uint32_t loff[SLOTS];
+    uint32_t loff[FSLOTS], aoff[FSLOTS];
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        const uint32_t idx = t + THREADS * s, r = idx / FW, c = idx % FW;
+    for (int s = 0; s < FSLOTS; ++s) {
+        const uint32_t idx = t + FT * s, r = idx / FW, c = idx % FW;
         loff[s] = reflect_pos(int32_t(y0) - 4 + int32_t(r), n.ny) * uint32_t(a.spy) +
                   reflect_pos(int32_t(x0) - 4 + int32_t(c), n.nx);
+        aoff[s] = r * PA + c;
     }
-    // X-pass / Z-pipeline ownership (threads < 256): row r, x tile u (4 pairs)
-    const uint32_t r = t & 31u, u = t >> 5;
-    T E[8], O[8], O1[8], E2[8], O3[8];
+    // Z ownership (threads < 512): row zr, columns 4 zq .. 4 zq + 3 of the X stage
+    const uint32_t zr = t & 31u, zq = (t >> 5) & 15u;
+    const bool zown = t < 512;
+    T E[4], O[4], O1[4], E2[4], O3[4];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) E[k] = O[k] = O1[k] = E2[k] = O3[k] = T(0);
+    for (int k = 0; k < 4; ++k) E[k] = O[k] = O1[k] = E2[k] = O3[k] = T(0);
+
+    // 16-byte stores need 4-aligned rows and band offsets in both destinations
+    const bool vec_hi = (a.hpy & 3u) == 0 && (a.hpz & 3u) == 0 && (hx & 3u) == 0;
+    const bool vec_lo = (a.lpy & 3u) == 0 && (a.lpz & 3u) == 0;
 
     const int npairs = int(a.sz / 2) + 4;
-    // software pipeline, two planes deep: while plane p is lifted, the
-    // footprints of planes p+1 and p+2 are in flight (even planes in `pe`,
-    // odd planes in `po`)
-    T pe[SLOTS], po[SLOTS];
+    // the two planes of the next step are in flight while this step runs
+    T pe[FSLOTS], po[FSLOTS];
     {
-        const T* plane0 = a.src + uint64_t(reflect_pos(int32_t(z0) - 4, n.nz)) * a.spz;
-        const T* plane1 = a.src + uint64_t(reflect_pos(int32_t(z0) - 3, n.nz)) * a.spz;
+        const T* p0 = a.src + uint64_t(reflect_pos(int32_t(z0) - 4, n.nz)) * a.spz;
+        const T* p1 = a.src + uint64_t(reflect_pos(int32_t(z0) - 3, n.nz)) * a.spz;
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
-            pe[s] = plane0[loff[s]];
-            po[s] = plane1[loff[s]];
+        for (int s = 0; s < FSLOTS; ++s) {
+            pe[s] = p0[loff[s]];
+            po[s] = p1[loff[s]];
         }
     }
     for (int i = 0; i < npairs; ++i) {
-        T ev[8], ov[8];
+        // 1. footprints of planes 2i, 2i+1 into A; next step's loads issued
+        bool ok = true;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            T cur[SLOTS];
-            T (&buf)[SLOTS] = h ? po : pe;
+        for (int s = 0; s < FSLOTS; ++s) {
+            ok = ok && __builtin_isfinite(pe[s]) && __builtin_isfinite(po[s]);
+            A[aoff[s]] = pe[s];
+            A[FH * PA + aoff[s]] = po[s];
+        }
+        if (i < npairs - 1) {
+            const int32_t pn = int32_t(z0) - 4 + 2 * (i + 1);
+            const T* p0 = a.src + uint64_t(reflect_pos(pn, n.nz)) * a.spz;
+            const T* p1 = a.src + uint64_t(reflect_pos(pn + 1, n.nz)) * a.spz;
 #pragma unroll
-            for (int s = 0; s < SLOTS; ++s) cur[s] = buf[s];
-            if (i < npairs - 1) {  // plane 2(i+1)+h of the segment
-                const int32_t pn = int32_t(z0) - 4 + 2 * (i + 1) + h;
-                const T* plane = a.src + uint64_t(reflect_pos(pn, n.nz)) * a.spz;
-#pragma unroll
-                for (int s = 0; s < SLOTS; ++s) buf[s] = plane[loff[s]];
-            }
-            bool ok = true;
-#pragma unroll
-            for (int s = 0; s < SLOTS; ++s) {
-                const uint32_t idx = t + THREADS * s;
-                ok = ok && __builtin_isfinite(cur[s]);
-                A[(idx / FW) * PA + idx % FW] = cur[s];
-            }
-            if (a.status && !ok) atomicOr(a.status, uint32_t(kNonFiniteInput));
-            __syncthreads();
-            if (t < 4 * FW) {  // Y: column c, rows 8s..8s+15 -> 4 low + 4 high pairs
-                const uint32_t c = t % FW, s = t / FW;
-                T e[8], o[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    e[j] = A[(8 * s + 2 * j) * PA + c];
-                    o[j] = A[(8 * s + 2 * j + 1) * PA + c];
-                }
-                fwd_lift<T, 8>(e, o);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    B[(4 * s + j) * PA + c] = e[j + 2];
-                    B[(16 + 4 * s + j) * PA + c] = o[j + 2];
-                }
-            }
-            __syncthreads();
-            if (t < 256) {  // X: row r, columns 8u..8u+15 -> 4 low + 4 high pairs
-                T e[8], o[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    e[j] = B[r * PA + 8 * u + 2 * j];
-                    o[j] = B[r * PA + 8 * u + 2 * j + 1];
-                }
-                fwd_lift<T, 8>(e, o);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    (h ? ov : ev)[q] = e[q + 2];
-                    (h ? ov : ev)[4 + q] = o[q + 2];
-                }
+            for (int s = 0; s < FSLOTS; ++s) {
+                pe[s] = p0[loff[s]];
+                po[s] = p1[loff[s]];
             }
         }
-        // Z pipeline: pair i of the segment (cpu/src/transform.cpp:136-145, streamed)
-        if (t < 256) {
-            using C = Lift<T>;
+        if (a.status && !ok) atomicOr(a.status, uint32_t(kNonFiniteInput));
+        __syncthreads();
+        {  // 2. Y: plane h, column c, rows 8s..8s+15 -> 4 low + 4 high pairs
+            const uint32_t h = t / (4 * FW), rem = t % (4 * FW), c = rem % FW, sg = rem / FW;
+            const T* src = A + h * FH * PA + c;
+            T e[8], o[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int j = 0; j < 8; ++j) {
+                e[j] = src[(8 * sg + 2 * j) * PA];
+                o[j] = src[(8 * sg + 2 * j + 1) * PA];
+            }
+            fwd_lift<T, 8>(e, o);
+            T* dst = B + h * TY * PA + c;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                dst[(4 * sg + j) * PA] = e[j + 2];
+                dst[(16 + 4 * sg + j) * PA] = o[j + 2];
+            }
+        }
+        __syncthreads();
+        if (t < 512) {  // 3. X: plane h, row r, columns 8u..8u+15 -> 4 low + 4 high pairs
+            const uint32_t h = t >> 8, r = t & 31u, u = (t >> 5) & 7u;
+            const T* src = B + h * TY * PA + r * PA + 8 * u;
+            T e[8], o[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                e[j] = src[2 * j];
+                o[j] = src[2 * j + 1];
+            }
+            fwd_lift<T, 8>(e, o);
+            T* dst = Xs + h * TY * CP + r * CP + 4 * u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                dst[q] = e[q + 2];
+                dst[32 + q] = o[q + 2];
+            }
+        }
+        __syncthreads();
+        // 4. Z pipeline: pair i of the segment (cpu/src/transform.cpp:136-145, streamed)
+        if (zown) {
+            using C = Lift<T>;
+            T ev[4], ov[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ev[k] = Xs[zr * CP + 4 * zq + k];
+                ov[k] = Xs[TY * CP + zr * CP + 4 * zq + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
                 const T o1 = fmadd(C::a, E[k] + ev[k], O[k]);   // o1[i-1]
                 const T e2 = fmadd(C::b, O1[k] + o1, E[k]);     // e2[i-1]
                 const T o3 = fmadd(C::g, E2[k] + e2, O1[k]);    // o3[i-2]
                 const T e4 = fmadd(C::d, O3[k] + o3, E2[k]);    // e4[i-2]
                 if (i >= 4) {
-                    const uint32_t col = (k < 4 ? 0u : 32u) + 4 * u + (k & 3);
-                    Cst[r * (TX + 1) + col] = e4 * C::k;
-                    Cst[(TY + r) * (TX + 1) + col] = o3 * (-C::ik);
+                    Cs[zr * CP + 4 * zq + k] = e4 * C::k;
+                    Cs[TY * CP + zr * CP + 4 * zq + k] = o3 * (-C::ik);
                 }
                 E[k] = ev[k];
                 O[k] = ov[k];
@@ -215,29 +253,53 @@ __global__ void __launch_bounds__(THREADS) fwd_level_kernel(FwdLevel<T> a) {
                 O3[k] = o3;
             }
         }
-        if (i >= 4) {
-            __syncthreads();
-            const uint32_t iz = z0 / 2 + uint32_t(i - 4);
-            const uint32_t lane = t & 63u, xb = lane >> 5, ix = x0 / 2 + (lane & 31u);
-            const bool xok = xb ? 2 * ix + 1 < n.nx : 2 * ix < n.nx;
-            for (uint32_t job = t >> 6; job < 2 * TY; job += THREADS / 64) {
-                const uint32_t zb = job / TY, rr = job % TY, yb = rr >> 4, iy = y0 / 2 + (rr & 15u);
-                const bool ok = xok && (yb ? 2 * iy + 1 < n.ny : 2 * iy < n.ny) && (zb ? 2 * iz + 1 < n.nz : 2 * iz < n.nz);
-                if (!ok) continue;
-                const T v = Cst[job * (TX + 1) + lane];
-                if (!(xb | yb | zb)) {
-                    a.lo[uint64_t(iz) * a.lpz + uint64_t(iy) * a.lpy + ix] = v;
-                } else {
-                    const uint64_t X = xb ? hx + ix : ix, Y = yb ? hy + iy : iy, Z = zb ? hz + iz : iz;
-                    a.hi[Z * a.hpz + Y * a.hpy + X] = v;
-                }
+        __syncthreads();  // X (in A) consumed; C (in B) complete
+        if (i < 4) continue;
+        // 5. rows of the finished pair: 64 rows (z band, y row) x 16 groups of 4
+        const uint32_t iz = z0 / 2 + uint32_t(i - 4);
+        for (uint32_t w = t; w < 2 * TY * 16; w += FT) {
+            const uint32_t row = w >> 4, q = w & 15u;
+            const uint32_t zb = row / TY, rr = row % TY, yb = rr >> 4, iy = y0 / 2 + (rr & 15u);
+            const uint32_t xb = q >> 3, ix = x0 / 2 + 4 * (q & 7u);
+            if (!(zb ? 2 * iz + 1 < n.nz : 2 * iz < n.nz) || !(yb ? 2 * iy + 1 < n.ny : 2 * iy < n.ny)) continue;
+            const T* v = Cs + row * CP + 4 * q;
+            const bool low = !(xb | yb | zb);
+            T* dst;
+            if (low) {
+                dst = a.lo + uint64_t(iz) * a.lpz + uint64_t(iy) * a.lpy + ix;
+            } else {
+                const uint64_t X = xb ? hx + ix : ix, Y = yb ? hy + iy : iy, Z = zb ? hz + iz : iz;
+                dst = a.hi + Z * a.hpz + Y * a.hpy + X;
+            }
+            // samples 2*ix+xb .. of the level box exist up to nx
+            const bool full = 2 * (ix + 3) + xb < n.nx;
+            if (full && (low ? vec_lo : vec_hi)) {
+                store_vec4(dst, v);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (2 * (ix + k) + xb < n.nx) dst[k] = v[k];
             }
         }
     }
 }
 
+// Inverse level, 576 threads per 64 x 32 x sz tile; one step per z pair:
+// every thread streams 5 footprint columns through the Z synthesis (with the
+// tile-end fold side chain) and stages the two reconstructed planes in Zp;
+// X then runs on 640 (plane, row, 8-sample) items and Y on 512 (plane,
+// column, 8-sample) items, one per thread, whose lanes are 64 consecutive x
+// for coalesced emission. For the repair emitter each Y thread's 8 original
+// samples are loaded one step ahead.
+// Waves per SIMD the register budget must allow: two workgroups per CU for
+// the float store variants (<= 96 VGPRs); the others take what they need.
 template <class T, int Emit>
-__global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
+constexpr int inv_waves() {
+    return Emit != kEmitRepair && sizeof(T) == 4 ? 5 : 1;
+}
+
+template <class T, int Emit>
+__global__ void __launch_bounds__(FT, (inv_waves<T, Emit>())) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
     __shared__ T Zp[2 * FH * PA];                // two reconstructed z planes (split XY)
     __shared__ T Xp[2 * FH * (TX + 1)];          // after X: rows still split in Y
@@ -255,19 +317,19 @@ __global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
 
     // per footprint slot: offsets of its (cx, cy) in the coefficient volume and
     // in the coarse LLL box (used when cx < hx, cy < hy and cz < hz)
-    uint32_t ohi[SLOTS], olo[SLOTS];
+    uint32_t ohi[FSLOTS], olo[FSLOTS];
     uint32_t lowmask = 0;
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        const uint32_t idx = t + THREADS * s, r = idx / FW, c = idx % FW;
+    for (int s = 0; s < FSLOTS; ++s) {
+        const uint32_t idx = t + FT * s, r = idx / FW, c = idx % FW;
         const uint32_t cy = split_pos(int32_t(y0) - 4 + int32_t(r), n.ny);
         const uint32_t cx = split_pos(int32_t(x0) - 4 + int32_t(c), n.nx);
         ohi[s] = cy * uint32_t(a.hpy) + cx;
         olo[s] = cy * uint32_t(a.lpy) + cx;
         if (cx < hx && cy < hy) lowmask |= 1u << s;
     }
-    // Issue the 2 x SLOTS coefficient loads of pair g (even and odd z sample).
-    auto issue = [&](int32_t g, T (&ev)[SLOTS], T (&ov)[SLOTS]) {
+    // Issue the 2 x FSLOTS coefficient loads of pair g (even and odd z sample).
+    auto issue = [&](int32_t g, T (&ev)[FSLOTS], T (&ov)[FSLOTS]) {
         const uint32_t ze = split_pos(2 * g, n.nz), zo = split_pos(2 * g + 1, n.nz);
         const T* he = a.hi + uint64_t(ze) * a.hpz;
         const T* ho = a.hi + uint64_t(zo) * a.hpz;
@@ -275,34 +337,63 @@ __global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
         const T* lo = a.lo + uint64_t(zo) * a.lpz;
         const uint32_t le_ok = ze < hz ? lowmask : 0u, lo_ok = zo < hz ? lowmask : 0u;
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
+        for (int s = 0; s < FSLOTS; ++s) {
             const T* pe = ((le_ok >> s) & 1u) ? le + olo[s] : he + ohi[s];
             const T* po = ((lo_ok >> s) & 1u) ? lo + olo[s] : ho + ohi[s];
             ev[s] = *pe;
             ov[s] = *po;
         }
     };
+    // Y item of this thread (t < 512): plane yp, column yx, y tile yt; a wave
+    // is one (plane, tile) and its lanes are 64 consecutive x
+    const bool yown = t < 2 * TX * (TY / 8);
+    const uint32_t yp = t / (TX * (TY / 8)), yx = t % TX, yt = (t / TX) % (TY / 8);
+    const uint32_t X = x0 + yx;
+    // original samples of the Y item for step g (repair emitter)
+    auto fetch_orig = [&](int32_t g, T (&ref)[8]) {
+        const uint32_t Z = 2 * uint32_t(g - 2) + yp;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t Y = y0 + 8 * yt + j;
+            ref[j] = yown && X < n.nx && Z < n.nz && Y < n.ny
+                         ? a.orig[uint64_t(Z) * a.opz + uint64_t(Y) * a.opy + X]
+                         : T(0);
+        }
+    };
     // streaming state per slot: OS = os[g-1], EA = eA[g-1], OB = oB[g-2],
     // EC = eC[g-2]; SF rotates through o_raw[B-2], eA'[B-1], oB'[B-1], eC'[B]
-    T OS[SLOTS], EA[SLOTS], OB[SLOTS], EC[SLOTS], SF[SLOTS];
+    T OS[FSLOTS], EA[FSLOTS], OB[FSLOTS], EC[FSLOTS], SF[FSLOTS];
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) OS[s] = EA[s] = OB[s] = EC[s] = SF[s] = T(0);
+    for (int s = 0; s < FSLOTS; ++s) OS[s] = EA[s] = OB[s] = EC[s] = SF[s] = T(0);
 
     const int32_t g0 = int32_t(zs / 2) - 2, g1 = int32_t((zs + a.sz) / 2) + 1;
-    T nev[SLOTS], nov[SLOTS];
+    auto emits = [&](int32_t g) { return g - 2 >= int32_t(zs / 2); };
+    // float repair emitter: originals one step ahead (double: at use, for registers)
+    constexpr bool prefetch = Emit == kEmitRepair && sizeof(T) == 4;
+    T nev[FSLOTS], nov[FSLOTS], nref[8], cref[8];
     issue(g0, nev, nov);
+    if constexpr (prefetch) {
+        if (emits(g0)) fetch_orig(g0, nref);
+    }
     for (int32_t g = g0; g <= g1; ++g) {
-        T cev[SLOTS], cov[SLOTS];
+        T cev[FSLOTS], cov[FSLOTS];
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
+        for (int s = 0; s < FSLOTS; ++s) {
             cev[s] = nev[s];
             cov[s] = nov[s];
         }
         if (g < g1) issue(g + 1, nev, nov);  // in flight during this step
-        const int phase = ((g % H) + H) % H;
-        const bool emit = g - 2 >= int32_t(zs / 2);
+        const bool emit = emits(g);
+        if constexpr (prefetch) {
+            if (emit) {
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
+                for (int j = 0; j < 8; ++j) cref[j] = nref[j];
+            }
+            if (g < g1 && emits(g + 1)) fetch_orig(g + 1, nref);
+        }
+        const int phase = ((g % H) + H) % H;
+#pragma unroll
+        for (int s = 0; s < FSLOTS; ++s) {
             const T oraw = cov[s];
             const T es = cev[s] * C::ik, os = oraw * (-C::k);
             const T eA = fmadd(-C::d, OS[s] + os, es);   // eA[g]
@@ -326,7 +417,7 @@ __global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
             }
             if (phase == H - 2) SF[s] = oraw;  // o_raw[B-2] for the next tile's fold
             if (emit) {
-                const uint32_t idx = t + THREADS * s, row = idx / FW, col = idx % FW;
+                const uint32_t idx = t + FT * s, row = idx / FW, col = idx % FW;
                 Zp[row * PA + col] = even;
                 Zp[FH * PA + row * PA + col] = odd;
             }
@@ -338,7 +429,7 @@ __global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
         if (!emit) continue;
         __syncthreads();
         // X (unfolded, tile 8): plane p, row rr, x tile xt -> 8 samples
-        for (uint32_t w = t; w < 2 * FH * (TX / 8); w += THREADS) {
+        for (uint32_t w = t; w < 2 * FH * (TX / 8); w += FT) {
             const uint32_t p = w / (FH * (TX / 8)), rem = w % (FH * (TX / 8));
             const uint32_t rr = rem % FH, xt = rem / FH;
             const T* src = Zp + p * FH * PA + rr * PA + 8 * xt;
@@ -357,104 +448,84 @@ __global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
             }
         }
         __syncthreads();
-        // Y (folded, tile 8): plane p, column x, y tile yt -> 8 samples. A wave
-        // is one (p, yt) and its lanes are 64 consecutive x (64 | 256, 64 | 320).
-        const uint32_t zbase = 2 * uint32_t(g - 2);
-        for (uint32_t w = t; w < 2 * TX * (TY / 8); w += THREADS) {
-            const uint32_t p = w / (TX * (TY / 8)), rem = w % (TX * (TY / 8));
-            const uint32_t x = rem % TX, yt = rem / TX;
-            const T* src = Xp + p * FH * (TX + 1) + 8 * yt * (TX + 1) + x;
-            const uint32_t X = x0 + x, Z = zbase + p;
-            const bool xz = X < n.nx && Z < n.nz;
-            // the original samples of this column (repair emit): all eight
-            // loads issued before the lifting, none waits behind a store
-            // (float; for double the registers are not there and each load
-            // is issued where it is used)
-            constexpr bool hoist = Emit == kEmitRepair && sizeof(T) == 4;
-            T ref[8];
-            if constexpr (hoist) {
+        if (!yown) continue;
+        // Y (folded, tile 8) of this thread's column -> 8 samples
+        const uint32_t Z = 2 * uint32_t(g - 2) + yp;
+        const bool xz = X < n.nx && Z < n.nz;
+        const T* src = Xp + yp * FH * (TX + 1) + 8 * yt * (TX + 1) + yx;
+        T e[8], o[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t Y = y0 + 8 * yt + j;
-                    ref[j] = xz && Y < n.ny ? a.orig[uint64_t(Z) * a.opz + uint64_t(Y) * a.opy + X] : T(0);
+        for (int j = 0; j < 8; ++j) {
+            e[j] = src[(2 * j) * (TX + 1)];
+            o[j] = src[(2 * j + 1) * (TX + 1)];
+        }
+        inv_lift<T, 8, true>(e, o);
+        if constexpr (Emit != kEmitRepair) {
+            bool fin = true;
+            if (xz) {
+                T* dst = a.out + uint64_t(Z) * a.opz + X;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t Y = y0 + 8 * yt + 2 * j;
+                    if (Y < n.ny) {
+                        dst[uint64_t(Y) * a.opy] = e[j + 2];
+                        fin = fin && __builtin_isfinite(e[j + 2]);
+                    }
+                    if (Y + 1 < n.ny) {
+                        dst[uint64_t(Y + 1) * a.opy] = o[j + 2];
+                        fin = fin && __builtin_isfinite(o[j + 2]);
+                    }
                 }
             }
-            T e[8], o[8];
+            if (Emit == kEmitStoreScreen && __ballot(!fin) && (t & 63u) == 0)
+                atomicOr(a.status, uint32_t(kNonFiniteOutput));
+        } else {
+            if constexpr (!prefetch) fetch_orig(g, cref);
+            // decide the column's 8 samples, then reserve all of the wave's
+            // records with one LDS atomic and write them
+            const uint32_t lane = t & 63u;
+            const unsigned long long below = (1ull << lane) - 1ull;
+            const T nb = T(a.bound);
+            bool ok = true;
+            T err[8];
+            uint32_t pbits = 0, before[8], total = 0;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                e[j] = src[(2 * j) * (TX + 1)];
-                o[j] = src[(2 * j + 1) * (TX + 1)];
-            }
-            inv_lift<T, 8, true>(e, o);
-            if constexpr (Emit != kEmitRepair) {
-                bool fin = true;
-                if (xz) {
-                    T* dst = a.out + uint64_t(Z) * a.opz + X;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const uint32_t Y = y0 + 8 * yt + 2 * j;
-                        if (Y < n.ny) {
-                            dst[uint64_t(Y) * a.opy] = e[j + 2];
-                            fin = fin && __builtin_isfinite(e[j + 2]);
-                        }
-                        if (Y + 1 < n.ny) {
-                            dst[uint64_t(Y + 1) * a.opy] = o[j + 2];
-                            fin = fin && __builtin_isfinite(o[j + 2]);
-                        }
+                const uint32_t Y = y0 + 8 * yt + j;
+                const T v = (j & 1) ? o[j / 2 + 2] : e[j / 2 + 2];
+                bool patch = false;
+                err[j] = T(0);
+                if (xz && Y < n.ny) patch = repair_decision<T>(v, cref[j], nb, a.bound, &err[j], &ok);
+                const unsigned long long m = __ballot(patch);
+                pbits |= uint32_t(patch) << j;
+                before[j] = total + uint32_t(__popcll(m & below));
+                total += uint32_t(__popcll(m));
+                if (rowwords) {  // 64 consecutive x at a 32-aligned row start: own two words
+                    if (lane == 0 && Z < n.nz && Y < n.ny) {
+                        const uint64_t w0 = ((uint64_t(Z) * n.ny + Y) * n.nx + x0) >> 5;
+                        a.rep.bitmap[w0] = uint32_t(m);
+                        if (x0 + 32 < n.nx) a.rep.bitmap[w0 + 1] = uint32_t(m >> 32);
                     }
+                } else if (patch) {
+                    const uint64_t at = (uint64_t(Z) * n.ny + Y) * n.nx + X;
+                    atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
                 }
-                if (Emit == kEmitStoreScreen && __ballot(!fin) && (t & 63u) == 0)
-                    atomicOr(a.status, uint32_t(kNonFiniteOutput));
-            } else {
-                // decide the column's 8 samples, then reserve all of the
-                // wave's records with one LDS atomic and write them
-                const uint32_t lane = t & 63u;
-                const unsigned long long below = (1ull << lane) - 1ull;
-                const T nb = T(a.bound);
-                bool ok = true;
-                T err[8];
-                uint32_t pbits = 0, before[8], total = 0;
+            }
+            if (total) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&seg_fill, total);
+                base = __shfl(base, 0, 64);
+                const uint64_t seg0 = uint64_t(wg) * a.rep.cap + base;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    const uint32_t Y = y0 + 8 * yt + j;
-                    const T v = (j & 1) ? o[j / 2 + 2] : e[j / 2 + 2];
-                    bool patch = false;
-                    err[j] = T(0);
-                    if (xz && Y < n.ny) {
-                        const T r = hoist ? ref[j] : a.orig[uint64_t(Z) * a.opz + uint64_t(Y) * a.opy + X];
-                        patch = repair_decision<T>(v, r, nb, a.bound, &err[j], &ok);
-                    }
-                    const unsigned long long m = __ballot(patch);
-                    pbits |= uint32_t(patch) << j;
-                    before[j] = total + uint32_t(__popcll(m & below));
-                    total += uint32_t(__popcll(m));
-                    if (rowwords) {  // 64 consecutive x at a 32-aligned row start: own two words
-                        if (lane == 0 && Z < n.nz && Y < n.ny) {
-                            const uint64_t w0 = ((uint64_t(Z) * n.ny + Y) * n.nx + x0) >> 5;
-                            a.rep.bitmap[w0] = uint32_t(m);
-                            if (x0 + 32 < n.nx) a.rep.bitmap[w0 + 1] = uint32_t(m >> 32);
-                        }
-                    } else if (patch) {
-                        const uint64_t at = (uint64_t(Z) * n.ny + Y) * n.nx + X;
-                        atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
+                    if ((pbits >> j) & 1u) {
+                        const uint32_t Y = y0 + 8 * yt + j;
+                        a.rep.index[seg0 + before[j]] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
+                        static_cast<T*>(a.rep.payload)[seg0 + before[j]] = err[j];
                     }
                 }
-                if (total) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&seg_fill, total);
-                    base = __shfl(base, 0, 64);
-                    const uint64_t seg0 = uint64_t(wg) * a.rep.cap + base;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if ((pbits >> j) & 1u) {
-                            const uint32_t Y = y0 + 8 * yt + j;
-                            a.rep.index[seg0 + before[j]] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
-                            static_cast<T*>(a.rep.payload)[seg0 + before[j]] = err[j];
-                        }
-                    }
-                }
-                if (__ballot(!ok) && lane == 0) atomicOr(a.status, uint32_t(kRepairBound));
             }
+            if (__ballot(!ok) && lane == 0) atomicOr(a.status, uint32_t(kRepairBound));
         }
     }
     if constexpr (Emit == kEmitRepair) {
@@ -467,17 +538,17 @@ __global__ void __launch_bounds__(THREADS) inv_level_kernel(InvLevel<T> a) {
 template <class T>
 void launch_fwd_level(const FwdLevel<T>& a, hipStream_t s) {
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
-    hipLaunchKernelGGL((fwd_level_kernel<T>), grid, dim3(THREADS), 0, s, a);
+    hipLaunchKernelGGL((fwd_level_kernel<T>), grid, dim3(FT), 0, s, a);
 }
 template <class T>
 void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
     if (a.emit == kEmitRepair)
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(THREADS), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(FT), 0, s, a);
     else if (a.emit == kEmitStoreScreen)
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(THREADS), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(FT), 0, s, a);
     else
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore>), grid, dim3(THREADS), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore>), grid, dim3(FT), 0, s, a);
 }
 
 template void launch_fwd_level<float>(const FwdLevel<float>&, hipStream_t);

@@ -45,7 +45,24 @@ void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsig
                        bool of_inverse, hipStream_t s);
 // Segments and per-segment capacity the fused finest inverse level uses for
 // repairs (0 segments when that level is not fused for this geometry).
-void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, uint32_t* segments, uint32_t* cap);
+void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width, uint32_t* segments,
+                            uint32_t* cap);
+
+// Bit-neutral launch choices of the dyadic schedule for one geometry, type
+// and direction: leading levels run as fused kernels, and their z segments
+// (positive multiples of 16).
+struct Tuning {
+    unsigned fused;
+    uint32_t sz[5];
+};
+Tuning tuning_for(Dims full, unsigned lxy, unsigned width, bool inverse);
+Tuning heuristic_tuning(Dims full, unsigned lxy);
+void set_tuning(Dims full, unsigned width, bool inverse, const Tuning& t);
+void clear_tuning();
+std::string tuning_report();
+// Measures the candidates on this device and keeps the fastest (autotune.hip).
+template <class T>
+void autotune(Dims full, hipStream_t s);
 
 // quantize.hip
 struct BrickGeometry {

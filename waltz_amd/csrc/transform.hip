@@ -16,6 +16,10 @@
 // is a fully coalesced 256 B (f32) / 512 B (f64) wave access; X lines map lanes
 // to consecutive tiles of one row (overlapping 64 B windows served from L1).
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 #include "common.hpp"
 #include "fused.hpp"
@@ -168,14 +172,6 @@ struct Passes {
 
 // ------------------------------------------------------------------ fused dyadic schedule
 namespace {
-bool fused_enabled() {
-    static const bool on = [] {
-        const char* e = std::getenv("WALTZ_FUSED");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
-
 // z segment per workgroup: enough segments for ~1024 workgroups, a multiple of
 // 16 (the canonical f32 inverse Z tile, and twice the f64 one). Bit-neutral.
 uint32_t z_segment(Dims n) {
@@ -199,6 +195,30 @@ unsigned fused_levels(Dims n, unsigned levels) {
     return f;
 }
 
+// ------------------------------------------------------------------ tuning
+// Launch choices of the dyadic schedule that do not change a single bit: how
+// many leading levels run as fused kernels and each fused level's z segment.
+// Per (geometry, type, direction): measured by autotune(), set explicitly, or
+// the heuristic above.
+struct TuneKey {
+    uint32_t nx, ny, nz, width, inverse;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(nx, ny, nz, width, inverse) < std::tie(o.nx, o.ny, o.nz, o.width, o.inverse);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Tuning> g_tune;
+
+Tuning heuristic_impl(Dims full, unsigned lxy) {
+    Tuning t{};
+    t.fused = fused_levels(full, lxy);
+    Dims n = full;
+    for (unsigned l = 0; l < 5; ++l) {
+        t.sz[l] = z_segment(n);
+        n = shrink(n, true, true);
+    }
+    return t;
+}
 // Compact boxes of levels >= 1 live in `scratch`: level l's box at offset 0
 // for odd l and after the largest level-1 box for even l (two never-aliasing
 // halves, each <= n/8 + lines).
@@ -210,8 +230,46 @@ T* low_box(T* scratch, Dims full, unsigned l) {
 }
 }  // namespace
 
+Tuning tuning_for(Dims full, unsigned lxy, unsigned width, bool inverse) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    const auto it = g_tune.find(TuneKey{full.nx, full.ny, full.nz, width, inverse ? 1u : 0u});
+    Tuning t = it != g_tune.end() ? it->second : heuristic_impl(full, lxy);
+    if (t.fused > lxy) t.fused = lxy;
+    return t;
+}
+
+Tuning heuristic_tuning(Dims full, unsigned lxy) { return heuristic_impl(full, lxy); }
+
+void clear_tuning() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.clear();
+}
+
+void set_tuning(Dims full, unsigned width, bool inverse, const Tuning& t) {
+    for (unsigned l = 0; l < 5; ++l)
+        if (t.sz[l] < 16 || t.sz[l] % 16) throw std::invalid_argument("z segment must be a positive multiple of 16");
+    if (t.fused > 5) throw std::invalid_argument("at most five fused levels");
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[TuneKey{full.nx, full.ny, full.nz, width, inverse ? 1u : 0u}] = t;
+}
+
+std::string tuning_report() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    std::string out = "[";
+    for (const auto& [k, t] : g_tune) {
+        if (out.size() > 1) out += ",";
+        out += "{\"shape_xyz\":[" + std::to_string(k.nx) + "," + std::to_string(k.ny) + "," + std::to_string(k.nz) +
+               "],\"width\":" + std::to_string(k.width) + ",\"inverse\":" + (k.inverse ? "true" : "false") +
+               ",\"fused_levels\":" + std::to_string(t.fused) + ",\"z_segment\":[";
+        for (unsigned l = 0; l < 5; ++l) out += (l ? "," : "") + std::to_string(t.sz[l]);
+        out += "]}";
+    }
+    return out + "]";
+}
+
 template <class T>
-void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels, uint32_t* status, hipStream_t s) {
+void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels, const Tuning& tu, uint32_t* status,
+                   hipStream_t s) {
     const T* src = input;
     uint64_t spy = full.nx, spz = uint64_t(full.nx) * full.ny;
     Dims n = full;
@@ -236,7 +294,7 @@ void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels
             a.lpy = h.nx;
             a.lpz = uint64_t(h.nx) * h.ny;
         }
-        a.sz = z_segment(n);
+        a.sz = tu.sz[l];
         a.status = l == 0 ? status : nullptr;
         launch_fwd_level<T>(a, s);
         src = a.lo;
@@ -247,8 +305,8 @@ void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels
 }
 
 template <class T>
-void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, const Level0Emit<T>* emit,
-                   hipStream_t s) {
+void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, const Tuning& tu,
+                   const Level0Emit<T>* emit, hipStream_t s) {
     Dims sizes[5], n = full;
     for (unsigned l = 0; l < levels; ++l) { sizes[l] = n; n = shrink(n, true, true); }
     const T* lo = coeff;
@@ -273,7 +331,7 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
             a.opy = sizes[l].nx;
             a.opz = uint64_t(sizes[l].nx) * sizes[l].ny;
         }
-        a.sz = z_segment(sizes[l]);
+        a.sz = tu.sz[l];
         a.emit = kEmitStore;
         if (l == 0 && emit) {
             a.emit = emit->mode;
@@ -281,6 +339,11 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
             a.orig = emit->orig;
             a.bound = emit->bound;
             a.rep = emit->rep;
+            if (emit->mode == kEmitRepair) {  // the segment layout the caller allocated fixes the z segment
+                a.sz = emit->rep.cap / (64u * 32u);
+                if (a.sz % 16 || inv_segments(full, a.sz) != emit->rep.segments)
+                    throw std::logic_error("repair segments do not match the finest level's launch");
+            }
             if (emit->mode == kEmitRepair && (full.nx & 31u))  // atomics path needs a clean bitmap
                 zero_words(emit->rep.bitmap, (uint64_t(full.nx) * full.ny * full.nz + 31) / 32, s);
         }
@@ -291,11 +354,14 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
     }
 }
 
-void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, uint32_t* segments, uint32_t* cap) {
+void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width, uint32_t* segments,
+                            uint32_t* cap) {
     *segments = 0;
     *cap = 0;
-    if (!dyadic || !lxy || !fused_enabled() || !fused_levels(full, lxy)) return;
-    const uint32_t sz = z_segment(full);
+    if (!dyadic || !lxy) return;
+    const Tuning tu = tuning_for(full, lxy, width, true);
+    if (!tu.fused) return;
+    const uint32_t sz = tu.sz[0];
     *segments = uint32_t(inv_segments(full, sz));
     *cap = 64u * 32u * sz;
 }
@@ -313,14 +379,15 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
         return;
     }
     if (dyadic) {
-        const unsigned fused = fused_enabled() ? fused_levels(full, lxy) : 0u;
+        const Tuning tu = tuning_for(full, lxy, sizeof(T), false);
+        const unsigned fused = tu.fused;
         Dims act = full;
         if (fused) {
             if (input == coeff) {  // the level-0 kernel must not read what it writes
                 copy_box<T>(input, tmp1, full, full, s);
                 input = tmp1;
             }
-            fused_forward<T>(input, coeff, tmp0, full, fused, status, s);  // leaves LLL in coeff
+            fused_forward<T>(input, coeff, tmp0, full, fused, tu, status, s);  // leaves LLL in coeff
             p.first = false;
             for (unsigned l = 0; l < fused; ++l) act = shrink(act, true, true);
         }
@@ -354,7 +421,8 @@ bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
         return false;
     }
     if (dyadic) {
-        const unsigned fused = fused_enabled() ? fused_levels(full, lxy) : 0u;
+        const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
+        const unsigned fused = tu.fused;
         Dims sizes[5], act = full;
         for (unsigned l = 0; l < lxy; ++l) { sizes[l] = act; act = shrink(act, true, true); }
         static const int zxy[3] = {2, 0, 1};
@@ -363,10 +431,10 @@ bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
             p.level(full, sizes[l], zxy, 3, !fused && l == 0, unsigned(l));
         if (!fused) return false;
         if (output == coeff) {  // the level-0 kernel must not overwrite what it reads
-            fused_inverse<T>(coeff, tmp1, tmp0, full, fused, emit, s);
+            fused_inverse<T>(coeff, tmp1, tmp0, full, fused, tu, emit, s);
             if (!emit || emit->mode != kEmitRepair) copy_box<T>(tmp1, coeff, full, full, s);
         } else {
-            fused_inverse<T>(coeff, output, tmp0, full, fused, emit, s);
+            fused_inverse<T>(coeff, output, tmp0, full, fused, tu, emit, s);
         }
         return emit != nullptr;
     }

@@ -23,6 +23,19 @@
 
 namespace wz {
 
+// v_perm_b32 selector that packs the bytes of a word whose bits are set in
+// the 4-bit mask m, in order, into the low bytes (selector 4 = a zero byte of
+// the first operand for the rest).
+WZ_HD uint32_t compaction_selector(uint32_t m) {
+    uint32_t sel = 0x04040404u, k = 0;
+    for (uint32_t b = 0; b < 4; ++b)
+        if ((m >> b) & 1u) {
+            sel = (sel & ~(0xFFu << (8 * k))) | (b << (8 * k));
+            ++k;
+        }
+    return sel;
+}
+
 // Encoder inputs. load() gives thread t of chunk c the chunk's magnitudes
 // [32t, 32t + 32) (zero past the chunk's E elements) and returns their sign word.
 struct MagSource {  // packed magnitude / sign arrays
@@ -106,8 +119,9 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
                                                          uint8_t* __restrict__ out, uint64_t capacity, WzpScratch w,
                                                          unsigned long long* __restrict__ total,
                                                          uint32_t* __restrict__ status) {
-    __shared__ uint32_t tr[kChunk / 2];                // BIT_2 planes, 16 KiB
-    __shared__ uint32_t ob32[(kChunkBytes + kSignSlot) / 4 + 2];  // assembled payload
+    __shared__ __attribute__((aligned(16))) uint32_t tr[kChunk / 2];  // BIT_2 planes, 16 KiB
+    __shared__ __attribute__((aligned(16))) uint32_t ob32[(kChunkBytes + kSignSlot) / 4 + 4];  // payload
+    __shared__ uint32_t csel[16];                      // byte-compaction selectors
     __shared__ unsigned long long b0s[kThreads];       // B0, 2048 bytes
     __shared__ __attribute__((aligned(8))) uint8_t b1s[kThreads];
     __shared__ __attribute__((aligned(8))) uint8_t b2s[32];
@@ -120,6 +134,9 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     const uint32_t t = threadIdx.x;
     if (t == 0) chunk_sh = atomicAdd(w.ticket, 1u);
     signbuf[t] = 0;
+    if (t < 16) csel[t] = compaction_selector(t);
+    for (uint32_t i = t; i < (kChunkBytes + kSignSlot) / 16 + 1; i += kThreads)
+        reinterpret_cast<uint4*>(ob32)[i] = make_uint4(0, 0, 0, 0);
     if (t == 0) signbuf[kThreads] = 0;
     __syncthreads();
     const uint32_t c = chunk_sh;
@@ -159,9 +176,19 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     __syncthreads();
 
     // 3. RZE bitmaps: B0 (nonzero bytes), B1/B2/B3 (changed bytes of the level below)
+    // this thread's 64 transformed bytes stay in registers for the payload
+    uint32_t tw[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = reinterpret_cast<const uint4*>(tr)[4 * t + q];
+        tw[4 * q] = v.x;
+        tw[4 * q + 1] = v.y;
+        tw[4 * q + 2] = v.z;
+        tw[4 * q + 3] = v.w;
+    }
     unsigned long long b0t = 0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) b0t |= (unsigned long long)nonzero_bytes4(tr[16 * t + q]) << (4 * q);
+    for (int q = 0; q < 16; ++q) b0t |= (unsigned long long)nonzero_bytes4(tw[q]) << (4 * q);
     {
         const int32_t valid = int32_t(raw_bytes) - int32_t(64 * t);
         if (valid <= 0) b0t = 0;
@@ -285,15 +312,26 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
         for (int i = 0; i < 32; ++i)
             if (first + i < E) ob16[first + i] = uint16_t(m[i]);
     } else {
-        {  // nonzero bytes in order
-            uint32_t k = e0;
-            const uint8_t* bytes = reinterpret_cast<const uint8_t*>(tr) + 64 * t;
-            unsigned long long bits = b0t;
-            while (bits) {
-                const uint32_t j = uint32_t(__builtin_ctzll(bits));
-                ob[k++] = bytes[j];
-                bits &= bits - 1;
+        {  // nonzero bytes in order: each word's nonzero bytes packed by one
+           // byte permute, gathered in an 8-byte accumulator and OR-ed into
+           // the zeroed payload a word at a time (a lane's two edge words are
+           // shared with its neighbours; the bytes are disjoint)
+            uint32_t wi = e0 >> 2, nacc = e0 & 3u;
+            unsigned long long acc = 0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const uint32_t m = uint32_t(b0t >> (4 * q)) & 15u;
+                const uint32_t packed = __builtin_amdgcn_perm(0u, tw[q], csel[m]);
+                acc |= (unsigned long long)packed << (8 * nacc);
+                nacc += popc32(m);
+                if (nacc >= 4) {
+                    atomicOr(&ob32[wi], uint32_t(acc));
+                    acc >>= 32;
+                    nacc -= 4;
+                    ++wi;
+                }
             }
+            if (nacc) atomicOr(&ob32[wi], uint32_t(acc));
         }
         {  // changed B0 bytes
             uint32_t k = K0 + e1, bits = b1t;

@@ -125,13 +125,26 @@ struct BrickSink {
     }
 };
 
+// v_perm_b32 selector that spreads the low bytes of a word to the byte
+// positions whose bits are set in the 4-bit mask m (selector 4 = a zero byte
+// of the first operand elsewhere).
+WZ_HD uint32_t expansion_selector(uint32_t m) {
+    uint32_t sel = 0, k = 0;
+    for (uint32_t b = 0; b < 4; ++b) {
+        const uint32_t v = ((m >> b) & 1u) ? k++ : 4u;
+        sel |= v << (8 * b);
+    }
+    return sel;
+}
+
 // Decodes chunk c of a WZP stream of `bytes` bytes holding n magnitudes;
 // called by all 256 threads of a workgroup.
 template <class Sink>
 WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t n, uint32_t chunks, uint32_t groups,
                        uint32_t c, const Sink& sink, uint32_t* __restrict__ status) {
-    __shared__ uint32_t tr[kChunk / 2];
+    __shared__ __attribute__((aligned(16))) uint32_t tr[kChunk / 2];
     __shared__ uint32_t ib32[(kChunkBytes + kSignSlot) / 4 + 4];  // staged chunk payload
+    __shared__ uint32_t esel[16];           // byte-expansion selectors
     __shared__ uint32_t b1w[kThreads / 4];  // B1 as 64 words (256 bytes)
     __shared__ uint32_t b2w[8];             // B2 as 8 words (32 bytes)
     __shared__ uint32_t sh[8];
@@ -148,6 +161,7 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
     const bool raw = code == kRawCode;
     const uint32_t mbytes = code == 0 ? 0u : raw ? raw_bytes : code;
     const uint32_t sbytes = code == 0 ? 0u : sc ? sc : kSignSlot;
+    if (t < 16) esel[t] = expansion_selector(t);
     if (t == 0) {
         const uint32_t g = c / kGroup;
         uint64_t off = load32(in + uint64_t(g) * 4);
@@ -249,15 +263,32 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
         const uint32_t e0 = block_exclusive(popc64(b0t), sh, K0);
         if (pos1 - int32_t(K0) != 0) bad = true;
         if (!bad) {
-            uint32_t r = e0;
+            // this lane's nonzero bytes src[e0, e0 + popcount(b0t)), spread
+            // word by word with byte permutes from an 8-byte window of
+            // aligned loads (never past the staging buffer)
+            const uint32_t byte0 = uint32_t(src - reinterpret_cast<const uint8_t*>(ib32)) + e0;
+            uint32_t wi = byte0 >> 2;
+            const uint32_t sh0 = byte0 & 3u;
+            unsigned long long acc = (((unsigned long long)ib32[wi + 1] << 32) | ib32[wi]) >> (8 * sh0);
+            uint32_t navail = 8 - sh0;
+            wi += 2;
+            uint32_t tw[16];
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if ((b0t >> (4 * q + k)) & 1ull) word |= uint32_t(src[r++]) << (8 * k);
-                tr[16 * t + q] = word;
+                if (navail < 4) {
+                    acc |= (unsigned long long)ib32[wi++] << (8 * navail);
+                    navail += 4;
+                }
+                const uint32_t m = uint32_t(b0t >> (4 * q)) & 15u;
+                tw[q] = __builtin_amdgcn_perm(0u, uint32_t(acc), esel[m]);
+                const uint32_t c = popc32(m);
+                acc >>= 8 * c;
+                navail -= c;
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                reinterpret_cast<uint4*>(tr)[4 * t + q] = make_uint4(tw[4 * q], tw[4 * q + 1], tw[4 * q + 2],
+                                                                      tw[4 * q + 3]);
         }
         __syncthreads();
         const uint16_t* tr16 = reinterpret_cast<const uint16_t*>(tr);

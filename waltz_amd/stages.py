@@ -77,3 +77,30 @@ def wzp_decode(stream, n):
     check(lib().waltz_amd_wzp_decode(stream.data_ptr(), stream.numel(), n, mag.data_ptr(), sign.data_ptr(),
                                      _stream(dev)), "wzp_decode")
     return mag[:n], sign[:(n + 31) // 32]
+
+
+def autotune(shape, dtype, device=None):
+    """Time the fused lifting kernels' bit-neutral launch choices for this
+    (z, y, x) shape and dtype on the device and keep the fastest."""
+    nz, ny, nx = shape
+    dev = torch.device(device or "cuda")
+    check(lib().waltz_amd_autotune(_W[dtype], nx, ny, nz, _stream(dev)), "autotune")
+
+
+def set_tuning(shape, dtype, inverse, fused_levels, z_segments):
+    """Fix the fused level count and the five z segments for one direction."""
+    nz, ny, nx = shape
+    seg = (C.c_uint32 * 5)(*[int(v) for v in z_segments])
+    check(lib().waltz_amd_set_tuning(_W[dtype], nx, ny, nz, int(inverse), int(fused_levels), seg), "set_tuning")
+
+
+def clear_tuning():
+    lib().waltz_amd_clear_tuning()
+
+
+def tuning_report():
+    import json
+    cap = 1 << 16
+    buf = C.create_string_buffer(cap)
+    lib().waltz_amd_tuning_report(buf, cap)
+    return json.loads(buf.value.decode())
