@@ -1,10 +1,11 @@
 // Autotuner of the bit-neutral launch choices of the dyadic schedule (no
-// reference counterpart). For one geometry and type it times every fused level
-// kernel of both directions at each candidate z segment on this device, on a
-// synthetic finite field, and keeps the fastest per level; the number of fused
-// levels stays the heuristic's. Every candidate produces the same bits (the
-// segment only partitions the z range among workgroups), so the choice
-// changes speed only.
+// reference counterpart). For one geometry and type it times, on this device
+// and a synthetic finite field, every level's fused kernel of both directions
+// at each candidate z segment, keeps the fastest per level, and then picks the
+// number of leading levels to run fused (the rest as axis passes) from whole
+// transforms. Every candidate produces the same bits (a z segment only
+// partitions the z range among workgroups; fused and pass forms are bitwise
+// equal), so the choice changes speed only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,7 +80,6 @@ void autotune(Dims full, hipStream_t s) {
     const unsigned lxy = dwt_levels(full.nx < full.ny ? full.nx : full.ny), lz = dwt_levels(full.nz);
     if (lxy != lz || lxy == 0) return;  // the plane schedule runs axis passes only
     const Tuning base = heuristic_tuning(full, lxy);
-    if (!base.fused) return;
     const uint64_t n = uint64_t(full.nx) * full.ny * full.nz;
     DeviceBuffer in(n * sizeof(T)), coeff(n * sizeof(T)), low(n * sizeof(T));
     hipLaunchKernelGGL((tune_fill<T>), dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, static_cast<T*>(in.p), n);
@@ -88,7 +88,7 @@ void autotune(Dims full, hipStream_t s) {
     Timer timer;
     Tuning fw = base, iv = base;
     Dims box = full;
-    for (unsigned l = 0; l < base.fused; ++l) {
+    for (unsigned l = 0; l < lxy; ++l) {
         const Dims h{half_len(box.nx), half_len(box.ny), half_len(box.nz)};
         float best_f = 1e30f, best_i = 1e30f;
         for (const uint32_t sz : candidates(box.nz)) {
@@ -130,6 +130,34 @@ void autotune(Dims full, hipStream_t s) {
         }
         box = h;
     }
+    // then the number of fused levels: whole transforms with 0 .. lxy leading
+    // levels fused (coarser boxes underfill the GPU but save launches)
+    DeviceBuffer spare(n * sizeof(T));
+    T* const x = static_cast<T*>(in.p);
+    T* const c = static_cast<T*>(coeff.p);
+    T* const t0 = static_cast<T*>(low.p);
+    T* const t1 = static_cast<T*>(spare.p);
+    float best_f = 1e30f, best_i = 1e30f;
+    unsigned fuse_f = base.fused, fuse_i = base.fused;
+    for (unsigned f = 0; f <= lxy; ++f) {
+        Tuning tf = fw, ti = iv;
+        tf.fused = ti.fused = f;
+        set_tuning(full, sizeof(T), false, tf);
+        set_tuning(full, sizeof(T), true, ti);
+        const float a =
+            timer.best_ms([&] { transform_forward<T>(x, c, t0, t1, full, lxy, lz, true, nullptr, s); }, s, 2);
+        const float b = timer.best_ms([&] { transform_inverse<T>(c, x, t0, t1, full, lxy, lz, true, s); }, s, 2);
+        if (a < best_f) {
+            best_f = a;
+            fuse_f = f;
+        }
+        if (b < best_i) {
+            best_i = b;
+            fuse_i = f;
+        }
+    }
+    fw.fused = fuse_f;
+    iv.fused = fuse_i;
     hip_check(hipGetLastError(), "autotune launch");
     hip_check(hipStreamSynchronize(s), "autotune");
     set_tuning(full, sizeof(T), false, fw);

@@ -313,7 +313,8 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
     uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
     static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
     for (int l = int(levels) - 1; l >= 0; --l) {
-        StageClock clock(names[l], s);
+        const bool repairs = l == 0 && emit && emit->mode == kEmitRepair;
+        StageClock clock(repairs ? "inverse_L0_repairs" : names[l], s);
         InvLevel<T> a{};
         a.hi = coeff;
         a.hpy = full.nx;
