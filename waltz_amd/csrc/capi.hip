@@ -8,6 +8,8 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
+#include <mutex>
 
 #include "../../include/waltz.hpp"
 #include "../../include/waltz_amd.h"
@@ -41,6 +43,35 @@ wz::Dims dims_of(const WALTZ::Config& c) {
 }
 
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// Two full-volume scratch arrays for the stage entry points, kept per device
+// and grown on demand (calls are synchronous; the lock serialises them).
+struct StageScratch {
+    std::mutex mu;
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes) {
+        if (bytes > cap) {
+            if (p) hipFree(p);
+            p = nullptr;
+            cap = 0;
+            wz::hip_check(hipMalloc(&p, bytes), "stage scratch");
+            cap = bytes;
+        }
+        return p;
+    }
+};
+
+StageScratch& stage_scratch() {
+    static std::mutex m;
+    static std::vector<std::unique_ptr<StageScratch>> all;
+    int dev = 0;
+    wz::hip_check(hipGetDevice(&dev), "device");
+    std::lock_guard<std::mutex> g(m);
+    if (all.size() <= size_t(dev)) all.resize(size_t(dev) + 1);
+    if (!all[size_t(dev)]) all[size_t(dev)] = std::make_unique<StageScratch>();
+    return *all[size_t(dev)];
+}
 
 // Order-preserving integer images of floating-point values for atomic min/max.
 __device__ __forceinline__ uint32_t ordered(float v) {
@@ -199,11 +230,10 @@ int waltz_amd_transform(int width, int inverse, const void* input, void* output,
         const wz::Dims d = dims_of(nx, ny, nz);
         const uint64_t n = uint64_t(nx) * ny * nz;
         hipStream_t s = as_stream(stream);
-        void* tmp = nullptr;
-        wz::hip_check(hipMalloc(&tmp, 2 * n * size_t(width) + 512), "transform scratch");
-        uint32_t* status = nullptr;
-        wz::hip_check(hipMalloc(reinterpret_cast<void**>(&status), 4), "status");
-        wz::zero_words(status, 1, s);
+        StageScratch& sc = stage_scratch();
+        std::lock_guard<std::mutex> hold(sc.mu);
+        void* tmp = sc.get(2 * ((n * size_t(width) + 255) & ~size_t(255)));
+        uint32_t* status = nullptr;  // the stage transform does not screen its input
         char* t0 = static_cast<char*>(tmp);
         char* t1 = t0 + ((n * size_t(width) + 255) & ~size_t(255));
         if (width == 4) {
@@ -231,9 +261,8 @@ int waltz_amd_transform(int width, int inverse, const void* input, void* output,
                                               lz, dyadic != 0, status, s);
             }
         }
+        wz::hip_check(hipGetLastError(), "transform launch");
         wz::hip_check(hipStreamSynchronize(s), "transform");
-        hipFree(tmp);
-        hipFree(status);
     });
 }
 
@@ -247,9 +276,9 @@ int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, vo
         hipStream_t s = as_stream(stream);
         if (input != output)
             wz::hip_check(hipMemcpyAsync(output, input, n * size_t(width), hipMemcpyDeviceToDevice, s), "copy");
-        void* tmp = nullptr;
-        wz::hip_check(hipMalloc(&tmp, 2 * n * size_t(width) + 512), "adjoint scratch");
-        std::unique_ptr<void, hipError_t (*)(void*)> hold(tmp, hipFree);
+        StageScratch& sc = stage_scratch();
+        std::lock_guard<std::mutex> hold(sc.mu);
+        void* tmp = sc.get(2 * ((n * size_t(width) + 255) & ~size_t(255)));
         char* t0 = static_cast<char*>(tmp);
         char* t1 = t0 + ((n * size_t(width) + 255) & ~size_t(255));
         if (width == 4)

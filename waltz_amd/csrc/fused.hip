@@ -291,15 +291,8 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 // column, 8-sample) items, one per thread, whose lanes are 64 consecutive x
 // for coalesced emission. For the repair emitter each Y thread's 8 original
 // samples are loaded one step ahead.
-// Waves per SIMD the register budget must allow: two workgroups per CU for
-// the float store variants (<= 96 VGPRs); the others take what they need.
 template <class T, int Emit>
-constexpr int inv_waves() {
-    return Emit != kEmitRepair && sizeof(T) == 4 ? 5 : 1;
-}
-
-template <class T, int Emit>
-__global__ void __launch_bounds__(FT, (inv_waves<T, Emit>())) inv_level_kernel(InvLevel<T> a) {
+__global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
     __shared__ T Zp[2 * FH * PA];                // two reconstructed z planes (split XY)
     __shared__ T Xp[2 * FH * (TX + 1)];          // after X: rows still split in Y
@@ -428,21 +421,22 @@ __global__ void __launch_bounds__(FT, (inv_waves<T, Emit>())) inv_level_kernel(I
         }
         if (!emit) continue;
         __syncthreads();
-        // X (unfolded, tile 8): plane p, row rr, x tile xt -> 8 samples
-        for (uint32_t w = t; w < 2 * FH * (TX / 8); w += FT) {
-            const uint32_t p = w / (FH * (TX / 8)), rem = w % (FH * (TX / 8));
+        // X (unfolded, so any tile width gives the same bits; 16 samples per
+        // item keeps the 320 items in one round): plane p, row rr, tile xt
+        for (uint32_t w = t; w < 2 * FH * (TX / 16); w += FT) {
+            const uint32_t p = w / (FH * (TX / 16)), rem = w % (FH * (TX / 16));
             const uint32_t rr = rem % FH, xt = rem / FH;
-            const T* src = Zp + p * FH * PA + rr * PA + 8 * xt;
-            T e[8], o[8];
+            const T* src = Zp + p * FH * PA + rr * PA + 16 * xt;
+            T e[12], o[12];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < 12; ++j) {
                 e[j] = src[2 * j];
                 o[j] = src[2 * j + 1];
             }
-            inv_lift<T, 8, false>(e, o);
-            T* dst = Xp + p * FH * (TX + 1) + rr * (TX + 1) + 8 * xt;
+            inv_lift<T, 12, false>(e, o);
+            T* dst = Xp + p * FH * (TX + 1) + rr * (TX + 1) + 16 * xt;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < 8; ++j) {
                 dst[2 * j] = e[j + 2];
                 dst[2 * j + 1] = o[j + 2];
             }
