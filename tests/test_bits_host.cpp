@@ -83,6 +83,26 @@ int main() {
         bit2_inverse(p);
         for (int j = 0; j < 16; ++j) if (p[j] != m[j]) ++bad;
 
+        {  // the two-block form against two one-block transforms
+            uint32_t m2[32], a[16], b[16], planes[16], back[32];
+            for (int j = 0; j < 32; ++j) {
+                m2[j] = rnd() & 0xFFFFu;
+                if (iter % 5 == 0) m2[j] &= (1u << (rnd() % 17)) - 1u;
+            }
+            for (int j = 0; j < 16; ++j) {
+                a[j] = m2[j];
+                b[j] = m2[16 + j];
+            }
+            bit2_forward(a);
+            bit2_forward(b);
+            bit2_forward_pair(m2, planes);
+            for (int pl = 0; pl < 16; ++pl)
+                if (planes[pl] != (a[pl] | (b[pl] << 16))) ++bad;
+            bit2_inverse_pair(planes, back);
+            for (int j = 0; j < 32; ++j)
+                if (back[j] != m2[j]) ++bad;
+        }
+
         const uint32_t v = rnd(), mask = rnd() & rnd();
         uint32_t c = 0, k = 0;
         for (int i = 0; i < 32; ++i)

@@ -79,6 +79,7 @@ template <class T>
 void autotune(Dims full, hipStream_t s) {
     const unsigned lxy = dwt_levels(full.nx < full.ny ? full.nx : full.ny), lz = dwt_levels(full.nz);
     if (lxy != lz || lxy == 0) return;  // the plane schedule runs axis passes only
+    if (!fused_plane_fits(full.nx, full.ny, sizeof(T))) return;  // so do planes of 4 GiB and more
     const Tuning base = heuristic_tuning(full, lxy);
     const uint64_t n = uint64_t(full.nx) * full.ny * full.nz;
     DeviceBuffer in(n * sizeof(T)), coeff(n * sizeof(T)), low(n * sizeof(T));

@@ -16,21 +16,22 @@ namespace wz {
 WZB_HD uint32_t popc32(uint32_t v) { return uint32_t(__builtin_popcount(v)); }
 WZB_HD uint32_t popc64(uint64_t v) { return uint32_t(__builtin_popcountll(v)); }
 #if defined(__clang__)
-WZB_HD uint32_t bitrev16(uint32_t v) { return __builtin_bitreverse32(v) >> 16; }
+WZB_HD uint32_t bitrev32(uint32_t v) { return __builtin_bitreverse32(v); }
 #else
-WZB_HD uint32_t bitrev16(uint32_t v) {
+WZB_HD uint32_t bitrev32(uint32_t v) {
     uint32_t r = 0;
-    for (int i = 0; i < 16; ++i) r |= ((v >> i) & 1u) << (15 - i);
+    for (int i = 0; i < 32; ++i) r |= ((v >> i) & 1u) << (31 - i);
     return r;
 }
 #endif
+WZB_HD uint32_t bitrev16(uint32_t v) { return bitrev32(v) >> 16; }
 
 // In-place transpose of a 16x16 bit matrix held as 16 rows of 16 bits:
 // afterwards row c bit r holds what row r bit c held (Hacker's Delight
 // block-swap, four rounds of masked exchanges).
 WZB_HD void transpose16(uint32_t (&a)[16]) {
 #define WZ_SWAP(J, M)                                                   \
-    for (int k = 0; k < 16; ++k)                                        \
+    _Pragma("unroll") for (int k = 0; k < 16; ++k)                                        \
         if (!(k & (J))) {                                               \
             const uint32_t t = ((a[k] >> (J)) ^ a[k + (J)]) & (M);      \
             a[k + (J)] ^= t;                                            \
@@ -40,6 +41,24 @@ WZB_HD void transpose16(uint32_t (&a)[16]) {
     WZ_SWAP(4, 0x0F0Fu)
     WZ_SWAP(2, 0x3333u)
     WZ_SWAP(1, 0x5555u)
+#undef WZ_SWAP
+}
+
+// Two independent 16x16 transposes, one per 16-bit half of each word: the
+// same masked exchanges with the masks repeated in both halves (no exchange
+// crosses bit 16, so the halves never mix).
+WZB_HD void transpose16x2(uint32_t (&a)[16]) {
+#define WZ_SWAP(J, M)                                                   \
+    _Pragma("unroll") for (int k = 0; k < 16; ++k)                                        \
+        if (!(k & (J))) {                                               \
+            const uint32_t t = ((a[k] >> (J)) ^ a[k + (J)]) & (M);      \
+            a[k + (J)] ^= t;                                            \
+            a[k] ^= t << (J);                                           \
+        }
+    WZ_SWAP(8, 0x00FF00FFu)
+    WZ_SWAP(4, 0x0F0F0F0Fu)
+    WZ_SWAP(2, 0x33333333u)
+    WZ_SWAP(1, 0x55555555u)
 #undef WZ_SWAP
 }
 
@@ -58,6 +77,29 @@ WZB_HD void bit2_inverse(uint32_t (&m)[16]) {
     for (int p = 0; p < 16; ++p) a[p] = m[p];
     transpose16(a);
     for (int q = 0; q < 16; ++q) m[15 - q] = bitrev16(a[q]);
+}
+
+// BIT_2 of two consecutive blocks m[0..15] and m[16..31] (16-bit magnitudes):
+// the low half of planes[p] is plane p of the first block, the high half
+// plane p of the second. One 32-bit reversal serves both halves:
+// bitrev32(hi << 16 | lo) = bitrev16(hi) | bitrev16(lo) << 16.
+WZB_HD void bit2_forward_pair(const uint32_t (&m)[32], uint32_t (&planes)[16]) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) planes[q] = bitrev32((m[15 - q] << 16) | m[31 - q]);
+    transpose16x2(planes);
+}
+// Inverse of bit2_forward_pair.
+WZB_HD void bit2_inverse_pair(const uint32_t (&planes)[16], uint32_t (&m)[32]) {
+    uint32_t a[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) a[p] = planes[p];
+    transpose16x2(a);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t r = bitrev32(a[q]);
+        m[15 - q] = r >> 16;
+        m[31 - q] = r & 0xFFFFu;
+    }
 }
 
 // Gather the bits of v selected by mask into the low end, in order (x86 pext).

@@ -74,29 +74,66 @@ WZ_D void fwd_lift(T (&e)[P], T (&o)[P]) {
     }
 }
 
-template <class T, int P, bool Folded>
-WZ_D void inv_lift(T (&e)[P], T (&o)[P]) {
+// Raw buffer access with a wave-uniform base (a 4 GiB window: stride 0, no
+// range limit, dword data format) and a 32-bit per-lane byte offset: no
+// per-lane 64-bit address arithmetic.
+WZ_D __amdgpu_buffer_rsrc_t window(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+template <class T>
+WZ_D T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+WZ_D float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(off), 0, 0));
+}
+template <>
+WZ_D double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
+}
+WZ_D void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(off), 0, 0);
+}
+WZ_D void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, int(off), 0, 0);
+}
+
+// inv_lift streamed left to right: ld(k, even, odd) supplies pair k and
+// out(j, even, odd) receives finished pair j (2 <= j <= P-3) as soon as it is
+// final, so only four lifting values stay live instead of 2P. Every output is
+// the same sequence of operations on the same operands as in inv_lift, hence
+// the same bits.
+template <class T, int P, bool Folded, class Ld, class Out>
+WZ_D void inv_lift_stream(Ld&& ld, Out&& out) {
     using C = Lift<T>;
-    const T first_odd = o[0], last_odd = o[P - 1];
+    T first_odd = T(0), po0 = T(0), pe1 = T(0), po2 = T(0), pe3 = T(0);
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
-        e[j] = e[j] * C::ik;
-        o[j] = o[j] * (-C::k);
-    }
-#pragma unroll
-    for (int j = 1; j < P; ++j) {
+    for (int k = 0; k < P; ++k) {
+        T E, O;
+        ld(k, E, O);
+        if (k == 0) {  // e[0] is never read; o[0] only through the sum (or the fold)
+            if (Folded) first_odd = O;
+            else po0 = O * (-C::k);
+            continue;
+        }
+        const T o0 = O * (-C::k), e0 = E * C::ik;
         T s;
-        if (Folded && j == 1) s = fmadd(first_odd, -C::k, o[1]);
-        else if (Folded && j == P - 1) s = fmadd(last_odd, -C::k, o[P - 2]);
-        else s = o[j - 1] + o[j];
-        e[j] = fmadd(-C::d, s, e[j]);
+        if (Folded && k == 1) s = fmadd(first_odd, -C::k, o0);
+        else if (Folded && k == P - 1) s = fmadd(O, -C::k, po0);
+        else s = po0 + o0;
+        const T e1 = fmadd(-C::d, s, e0);                 // e1[k]
+        if (k >= 2) {
+            const T o2 = fmadd(-C::g, pe1 + e1, po0);     // o2[k-1]
+            if (k >= 3) {
+                const T e3 = fmadd(-C::b, po2 + o2, pe1); // e3[k-1]
+                if (k >= 4) out(k - 2, pe3, fmadd(-C::a, pe3 + e3, po2));  // o4[k-2]
+                pe3 = e3;
+            }
+            po2 = o2;
+        }
+        po0 = o0;
+        pe1 = e1;
     }
-#pragma unroll
-    for (int j = 1; j < P - 1; ++j) o[j] = fmadd(-C::g, e[j] + e[j + 1], o[j]);
-#pragma unroll
-    for (int j = 2; j < P - 1; ++j) e[j] = fmadd(-C::b, o[j - 1] + o[j], e[j]);
-#pragma unroll
-    for (int j = 2; j < P - 2; ++j) o[j] = fmadd(-C::a, e[j] + e[j + 1], o[j]);
 }
 
 // Four consecutive staged values to a 16-byte aligned destination.
@@ -286,11 +323,14 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 
 // Inverse level, 576 threads per 64 x 32 x sz tile; one step per z pair:
 // every thread streams 5 footprint columns through the Z synthesis (with the
-// tile-end fold side chain) and stages the two reconstructed planes in Zp;
-// X then runs on 640 (plane, row, 8-sample) items and Y on 512 (plane,
-// column, 8-sample) items, one per thread, whose lanes are 64 consecutive x
-// for coalesced emission. For the repair emitter each Y thread's 8 original
-// samples are loaded one step ahead.
+// tile-end fold side chain), refilling each column's registers with the next
+// pair as soon as they are consumed, and stages the two reconstructed planes
+// in Zp; X then runs on 320 (plane, row, 16-sample) items and Y on 512
+// (plane, column, 8-sample) items, one per thread, whose lanes are 64
+// consecutive x for coalesced emission; both lift their line streamed. Loads
+// and stores go through buffer windows at wave-uniform plane or row bases.
+// For the repair emitter each Y thread's 8 original samples are loaded as
+// the step begins.
 template <class T, int Emit>
 __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
@@ -308,49 +348,75 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     const bool rowwords = (n.nx & 31u) == 0;  // bitmap words never straddle rows
     if (t == 0) seg_fill = 0;
 
-    // per footprint slot: offsets of its (cx, cy) in the coefficient volume and
-    // in the coarse LLL box (used when cx < hx, cy < hy and cz < hz)
-    uint32_t ohi[FSLOTS], olo[FSLOTS];
-    uint32_t lowmask = 0;
+    // Footprint positions (row r, column c of the 72 x 40 window) in slot
+    // order: first the 36 x 20 positions with even r and c (which read the
+    // coarse LLL box in even z planes), then the rest row pair by row pair.
+    // Waves then read from one base pointer (wave-uniform) with 32-bit byte
+    // offsets; only the one wave that straddles the two groups selects per lane.
+    constexpr uint32_t kLow = (FW / 2) * (FH / 2);
+    uint32_t eoff[FSLOTS], ooff[FSLOTS], zoff[FSLOTS];  // even-plane, odd-plane, Zp offsets
+    uint32_t low_lanes = 0, lo_wave = 0, hi_wave = 0;   // per slot: lane / whole-wave classes
 #pragma unroll
     for (int s = 0; s < FSLOTS; ++s) {
-        const uint32_t idx = t + FT * s, r = idx / FW, c = idx % FW;
+        const uint32_t idx = t + FT * s;
+        uint32_t r, c;
+        if (idx < kLow) {
+            r = 2 * (idx / (FW / 2));
+            c = 2 * (idx % (FW / 2));
+        } else {
+            const uint32_t j = idx - kLow, k = j % (FW / 2 + FW);
+            r = 2 * (j / (FW / 2 + FW)) + (k < FW / 2 ? 0u : 1u);
+            c = k < FW / 2 ? 2 * k + 1 : k - FW / 2;
+        }
         const uint32_t cy = split_pos(int32_t(y0) - 4 + int32_t(r), n.ny);
         const uint32_t cx = split_pos(int32_t(x0) - 4 + int32_t(c), n.nx);
-        ohi[s] = cy * uint32_t(a.hpy) + cx;
-        olo[s] = cy * uint32_t(a.lpy) + cx;
-        if (cx < hx && cy < hy) lowmask |= 1u << s;
+        const bool low = cx < hx && cy < hy;
+        const uint32_t ohi = (cy * uint32_t(a.hpy) + cx) * uint32_t(sizeof(T));
+        eoff[s] = low ? (cy * uint32_t(a.lpy) + cx) * uint32_t(sizeof(T)) : ohi;
+        ooff[s] = ohi;
+        zoff[s] = r * PA + c;
+        const unsigned long long lw = __ballot(low);
+        low_lanes |= uint32_t(low) << s;
+        lo_wave |= uint32_t(lw == ~0ull) << s;
+        hi_wave |= uint32_t(lw == 0ull) << s;
     }
-    // Issue the 2 x FSLOTS coefficient loads of pair g (even and odd z sample).
-    auto issue = [&](int32_t g, T (&ev)[FSLOTS], T (&ov)[FSLOTS]) {
+    auto at = [](const T* base, uint32_t off) {
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+    };
+    // Sources of pair g: the coefficient volume's planes of the even and odd
+    // z sample and the LLL box's even plane (an even z sample always lies in
+    // the low half, an odd one never)
+    struct PairSrc {
+        const T *he, *ho, *le;
+    };
+    auto pair_src = [&](int32_t g) {
         const uint32_t ze = split_pos(2 * g, n.nz), zo = split_pos(2 * g + 1, n.nz);
-        const T* he = a.hi + uint64_t(ze) * a.hpz;
-        const T* ho = a.hi + uint64_t(zo) * a.hpz;
-        const T* le = a.lo + uint64_t(ze) * a.lpz;
-        const T* lo = a.lo + uint64_t(zo) * a.lpz;
-        const uint32_t le_ok = ze < hz ? lowmask : 0u, lo_ok = zo < hz ? lowmask : 0u;
-#pragma unroll
-        for (int s = 0; s < FSLOTS; ++s) {
-            const T* pe = ((le_ok >> s) & 1u) ? le + olo[s] : he + ohi[s];
-            const T* po = ((lo_ok >> s) & 1u) ? lo + olo[s] : ho + ohi[s];
-            ev[s] = *pe;
-            ov[s] = *po;
-        }
+        return PairSrc{a.hi + uint64_t(ze) * a.hpz, a.hi + uint64_t(zo) * a.hpz, a.lo + uint64_t(ze) * a.lpz};
+    };
+    auto load_slot = [&](const PairSrc& q, int s, T& ev, T& ov) {
+        if (((lo_wave | hi_wave) >> s) & 1u)
+            ev = buf_load<T>(window(((lo_wave >> s) & 1u) ? q.le : q.he), eoff[s]);
+        else
+            ev = at(((low_lanes >> s) & 1u) ? q.le : q.he, eoff[s]);
+        ov = buf_load<T>(window(q.ho), ooff[s]);
     };
     // Y item of this thread (t < 512): plane yp, column yx, y tile yt; a wave
     // is one (plane, tile) and its lanes are 64 consecutive x
-    const bool yown = t < 2 * TX * (TY / 8);
-    const uint32_t yp = t / (TX * (TY / 8)), yx = t % TX, yt = (t / TX) % (TY / 8);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const bool yown = wave < 2 * (TY / 8);
+    const uint32_t yp = wave / (TY / 8), yt = wave % (TY / 8), yx = t & 63u;
     const uint32_t X = x0 + yx;
+    const bool xin = X < n.nx;
+    // row Y of plane Z of a full-size field (wave-uniform) and this lane's byte offset in it
+    auto row_of = [&](auto* base, uint32_t Z, uint32_t Y) { return base + uint64_t(Z) * a.opz + uint64_t(Y) * a.opy; };
+    const uint32_t xoff = X * uint32_t(sizeof(T));
     // original samples of the Y item for step g (repair emitter)
     auto fetch_orig = [&](int32_t g, T (&ref)[8]) {
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const uint32_t Y = y0 + 8 * yt + j;
-            ref[j] = yown && X < n.nx && Z < n.nz && Y < n.ny
-                         ? a.orig[uint64_t(Z) * a.opz + uint64_t(Y) * a.opy + X]
-                         : T(0);
+            ref[j] = yown && xin && Z < n.nz && Y < n.ny ? buf_load<T>(window(row_of(a.orig, Z, Y)), xoff) : T(0);
         }
     };
     // streaming state per slot: OS = os[g-1], EA = eA[g-1], OB = oB[g-2],
@@ -361,34 +427,32 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
 
     const int32_t g0 = int32_t(zs / 2) - 2, g1 = int32_t((zs + a.sz) / 2) + 1;
     auto emits = [&](int32_t g) { return g - 2 >= int32_t(zs / 2); };
-    // float repair emitter: originals one step ahead (double: at use, for registers)
-    constexpr bool prefetch = Emit == kEmitRepair && sizeof(T) == 4;
-    T nev[FSLOTS], nov[FSLOTS], nref[8], cref[8];
-    issue(g0, nev, nov);
-    if constexpr (prefetch) {
-        if (emits(g0)) fetch_orig(g0, nref);
+    // float repair emitter: a step's originals are loaded as the step begins,
+    // ahead of the next pair's coefficients, so they arrive during the Z and X
+    // passes and the in-order memory counter never makes a column wait for
+    // them (double: at use, for registers)
+    constexpr bool early = Emit == kEmitRepair && sizeof(T) == 4;
+    T ref[8];
+    // ve/vo hold pair g's coefficients; each slot's registers are refilled
+    // with pair g+1 as soon as the slot has consumed them (in flight during
+    // the rest of the step)
+    T ve[FSLOTS], vo[FSLOTS];
+    {
+        const PairSrc q = pair_src(g0);
+#pragma unroll
+        for (int s = 0; s < FSLOTS; ++s) load_slot(q, s, ve[s], vo[s]);
     }
     for (int32_t g = g0; g <= g1; ++g) {
-        T cev[FSLOTS], cov[FSLOTS];
-#pragma unroll
-        for (int s = 0; s < FSLOTS; ++s) {
-            cev[s] = nev[s];
-            cov[s] = nov[s];
-        }
-        if (g < g1) issue(g + 1, nev, nov);  // in flight during this step
         const bool emit = emits(g);
-        if constexpr (prefetch) {
-            if (emit) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) cref[j] = nref[j];
-            }
-            if (g < g1 && emits(g + 1)) fetch_orig(g + 1, nref);
+        if constexpr (early) {
+            if (emit) fetch_orig(g, ref);
         }
+        const PairSrc nq = pair_src(g < g1 ? g + 1 : g);
         const int phase = ((g % H) + H) % H;
 #pragma unroll
         for (int s = 0; s < FSLOTS; ++s) {
-            const T oraw = cov[s];
-            const T es = cev[s] * C::ik, os = oraw * (-C::k);
+            const T oraw = vo[s];
+            const T es = ve[s] * C::ik, os = oraw * (-C::k);
             const T eA = fmadd(-C::d, OS[s] + os, es);   // eA[g]
             const T oB = fmadd(-C::g, EA[s] + eA, OS[s]); // oB[g-1]
             const T eC = fmadd(-C::b, OB[s] + oB, EA[s]); // eC[g-1]
@@ -410,14 +474,16 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             }
             if (phase == H - 2) SF[s] = oraw;  // o_raw[B-2] for the next tile's fold
             if (emit) {
-                const uint32_t idx = t + FT * s, row = idx / FW, col = idx % FW;
-                Zp[row * PA + col] = even;
-                Zp[FH * PA + row * PA + col] = odd;
+                Zp[zoff[s]] = even;
+                Zp[FH * PA + zoff[s]] = odd;
             }
             OS[s] = os;
             EA[s] = eA;
             OB[s] = oB;
             EC[s] = eC;
+            // the slot's registers take the next pair once nothing reads them
+            // (the last step reloads its own pair, unused)
+            load_slot(nq, s, ve[s], vo[s]);
         }
         if (!emit) continue;
         __syncthreads();
@@ -427,73 +493,77 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             const uint32_t p = w / (FH * (TX / 16)), rem = w % (FH * (TX / 16));
             const uint32_t rr = rem % FH, xt = rem / FH;
             const T* src = Zp + p * FH * PA + rr * PA + 16 * xt;
-            T e[12], o[12];
-#pragma unroll
-            for (int j = 0; j < 12; ++j) {
-                e[j] = src[2 * j];
-                o[j] = src[2 * j + 1];
-            }
-            inv_lift<T, 12, false>(e, o);
             T* dst = Xp + p * FH * (TX + 1) + rr * (TX + 1) + 16 * xt;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                dst[2 * j] = e[j + 2];
-                dst[2 * j + 1] = o[j + 2];
-            }
+            inv_lift_stream<T, 12, false>(
+                [&](int k, T& e, T& o) {
+                    e = src[2 * k];
+                    o = src[2 * k + 1];
+                },
+                [&](int j, T e, T o) {
+                    dst[2 * (j - 2)] = e;
+                    dst[2 * (j - 2) + 1] = o;
+                });
         }
         __syncthreads();
         if (!yown) continue;
         // Y (folded, tile 8) of this thread's column -> 8 samples
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
-        const bool xz = X < n.nx && Z < n.nz;
+        const bool xz = xin && Z < n.nz;
         const T* src = Xp + yp * FH * (TX + 1) + 8 * yt * (TX + 1) + yx;
-        T e[8], o[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            e[j] = src[(2 * j) * (TX + 1)];
-            o[j] = src[(2 * j + 1) * (TX + 1)];
-        }
-        inv_lift<T, 8, true>(e, o);
+        auto ld = [&](int k, T& e, T& o) {
+            e = src[(2 * k) * (TX + 1)];
+            o = src[(2 * k + 1) * (TX + 1)];
+        };
         if constexpr (Emit != kEmitRepair) {
             bool fin = true;
-            if (xz) {
-                T* dst = a.out + uint64_t(Z) * a.opz + X;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t Y = y0 + 8 * yt + 2 * j;
-                    if (Y < n.ny) {
-                        dst[uint64_t(Y) * a.opy] = e[j + 2];
-                        fin = fin && __builtin_isfinite(e[j + 2]);
-                    }
-                    if (Y + 1 < n.ny) {
-                        dst[uint64_t(Y + 1) * a.opy] = o[j + 2];
-                        fin = fin && __builtin_isfinite(o[j + 2]);
-                    }
+            auto put = [&](uint32_t j, T v) {
+                const uint32_t Y = y0 + 8 * yt + j;
+                if (xz && Y < n.ny) {
+                    buf_store(v, window(row_of(a.out, Z, Y)), xoff);
+                    fin = fin && __builtin_isfinite(v);
                 }
-            }
+            };
+            inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
+                put(2 * (j - 2), e);
+                put(2 * (j - 2) + 1, o);
+            });
             if (Emit == kEmitStoreScreen && __ballot(!fin) && (t & 63u) == 0)
                 atomicOr(a.status, uint32_t(kNonFiniteOutput));
         } else {
-            if constexpr (!prefetch) fetch_orig(g, cref);
-            // decide the column's 8 samples, then reserve all of the wave's
-            // records with one LDS atomic and write them
+            if constexpr (!early) fetch_orig(g, ref);
+            // decide each sample as the lifting finishes it; the rare rows
+            // with records reserve the wave's entries with one LDS atomic
             const uint32_t lane = t & 63u;
             const unsigned long long below = (1ull << lane) - 1ull;
             const T nb = T(a.bound);
             bool ok = true;
-            T err[8];
-            uint32_t pbits = 0, before[8], total = 0;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            auto decide = [&](uint32_t j, T v, T orig) {
                 const uint32_t Y = y0 + 8 * yt + j;
-                const T v = (j & 1) ? o[j / 2 + 2] : e[j / 2 + 2];
+                const bool inb = xz && Y < n.ny;
                 bool patch = false;
-                err[j] = T(0);
-                if (xz && Y < n.ny) patch = repair_decision<T>(v, cref[j], nb, a.bound, &err[j], &ok);
+                T err = T(0);
+                if constexpr (sizeof(T) == 4) {
+                    // the same float operations as repair_decision; its exact
+                    // bound test is settled by the rounded difference d of
+                    // the final value and the original: |d| < the largest
+                    // float <= bound implies |exact difference| < bound
+                    // (round to nearest is monotonic), which also implies
+                    // every operand was finite. Other lanes (ties at the
+                    // bound, non-finite samples) take the full test.
+                    const T e = v - orig;
+                    const bool p = __builtin_fabs(e) > nb;
+                    const T d = p ? (v - e) - orig : e;
+                    const bool fast = __builtin_fabs(d) < a.bound_lo;
+                    if (__ballot(inb && !fast)) {
+                        if (inb) patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
+                    } else {
+                        patch = inb && p;
+                        err = inb ? e : T(0);
+                    }
+                } else if (inb) {
+                    patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
+                }
                 const unsigned long long m = __ballot(patch);
-                pbits |= uint32_t(patch) << j;
-                before[j] = total + uint32_t(__popcll(m & below));
-                total += uint32_t(__popcll(m));
                 if (rowwords) {  // 64 consecutive x at a 32-aligned row start: own two words
                     if (lane == 0 && Z < n.nz && Y < n.ny) {
                         const uint64_t w0 = ((uint64_t(Z) * n.ny + Y) * n.nx + x0) >> 5;
@@ -504,21 +574,21 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                     const uint64_t at = (uint64_t(Z) * n.ny + Y) * n.nx + X;
                     atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
                 }
-            }
-            if (total) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&seg_fill, total);
-                base = __shfl(base, 0, 64);
-                const uint64_t seg0 = uint64_t(wg) * a.rep.cap + base;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if ((pbits >> j) & 1u) {
-                        const uint32_t Y = y0 + 8 * yt + j;
-                        a.rep.index[seg0 + before[j]] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
-                        static_cast<T*>(a.rep.payload)[seg0 + before[j]] = err[j];
+                if (m) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&seg_fill, uint32_t(__popcll(m)));
+                    base = __shfl(base, 0, 64);
+                    if (patch) {
+                        const uint64_t at = uint64_t(wg) * a.rep.cap + base + uint32_t(__popcll(m & below));
+                        a.rep.index[at] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
+                        static_cast<T*>(a.rep.payload)[at] = err;
                     }
                 }
-            }
+            };
+            inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
+                decide(2 * (j - 2), e, ref[2 * (j - 2)]);
+                decide(2 * (j - 2) + 1, o, ref[2 * (j - 2) + 1]);
+            });
             if (__ballot(!ok) && lane == 0) atomicOr(a.status, uint32_t(kRepairBound));
         }
     }
@@ -536,6 +606,10 @@ void launch_fwd_level(const FwdLevel<T>& a, hipStream_t s) {
 }
 template <class T>
 void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
+    // the kernel addresses a plane with 32-bit byte offsets
+    if (!fused_plane_fits(a.hpy, a.n.ny, sizeof(T)) || !fused_plane_fits(a.lpy, half_len(a.n.ny), sizeof(T)) ||
+        !fused_plane_fits(a.opy, a.n.ny, sizeof(T)))
+        throw std::logic_error("fused inverse level: plane exceeds 4 GiB");
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
     if (a.emit == kEmitRepair)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(FT), 0, s, a);
@@ -545,6 +619,32 @@ void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore>), grid, dim3(FT), 0, s, a);
 }
 
+template <class T>
+uint32_t repair_z_segment(Dims n, uint32_t tuned) {
+    // workgroups of the repair emitter one CU holds at once (registers, LDS)
+    static const uint32_t per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, inv_level_kernel<T, kEmitRepair>, FT, 0) != hipSuccess ||
+            b < 1)
+            b = 1;
+        return uint32_t(b);
+    }();
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "device");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    const uint64_t resident = uint64_t(cus) * per_cu;
+    const uint32_t top = (n.nz + 15) & ~15u;
+    uint32_t sz = tuned;
+    auto tiles = [&](uint32_t z) {
+        const Dims g = tile_grid(n, z);
+        return uint64_t(g.nx) * g.ny * g.nz;
+    };
+    while (sz < top && tiles(sz) > resident) sz = sz * 2 < top ? sz * 2 : top;
+    return sz;
+}
+
+template uint32_t repair_z_segment<float>(Dims, uint32_t);
+template uint32_t repair_z_segment<double>(Dims, uint32_t);
 template void launch_fwd_level<float>(const FwdLevel<float>&, hipStream_t);
 template void launch_fwd_level<double>(const FwdLevel<double>&, hipStream_t);
 template void launch_inv_level<float>(const InvLevel<float>&, hipStream_t);

@@ -18,6 +18,12 @@ struct FwdLevel {
     uint32_t* status;   // non-null: screen the input for non-finite samples
 };
 
+// Whether `rows` rows of pitch `pitch` elements of `width` bytes stay below
+// 4 GiB (the inverse level kernel's 32-bit byte offsets within a plane).
+inline bool fused_plane_fits(uint64_t pitch, uint64_t rows, uint64_t width) {
+    return pitch * rows * width < (uint64_t(1) << 32);
+}
+
 enum InvEmit : int {
     kEmitStore = 0,        // write the reconstruction
     kEmitStoreScreen = 1,  // write it and flag non-finite samples
@@ -38,6 +44,7 @@ struct InvLevel {
     uint32_t* status;   // screen / strict-bound failures
     const T* orig;      // kEmitRepair: the original field
     double bound;
+    float bound_lo;     // float repair emitter: the largest float <= bound
     SegList rep;        // kEmitRepair: one segment per workgroup, cap = 64*32*sz
 };
 
@@ -50,5 +57,10 @@ template <class T>
 void launch_fwd_level(const FwdLevel<T>& a, hipStream_t s);
 template <class T>
 void launch_inv_level(const InvLevel<T>& a, hipStream_t s);
+// z segment of the finest level's repair emitter for box n: the tuned segment,
+// doubled while the tiles outnumber the workgroups the device holds at once
+// (a multiple of 16 either way)
+template <class T>
+uint32_t repair_z_segment(Dims n, uint32_t tuned);
 
 }  // namespace wz

@@ -242,7 +242,10 @@ struct Ranks {
     }
 };
 
-// Repairs: segment k's entries go to their index-order rank (bitmap popcounts).
+// Repairs: segment k's entries go to their index-order rank (bitmap popcounts);
+// gridDim.y workgroups share a segment so that enough index -> rank -> store
+// chains are in flight.
+constexpr uint32_t kPlaceSlices = 8;
 template <class T>
 __global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archive, SegList rp, Ranks rank,
                                                      const EncodeResult* __restrict__ res) {
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archi
     const T* val = static_cast<const T*>(rp.payload) + uint64_t(k) * rp.cap;
     uint32_t* oi = reinterpret_cast<uint32_t*>(archive + L.pi);
     T* ov = reinterpret_cast<T*>(archive + L.pv);
-    for (uint32_t j = threadIdx.x; j < c; j += 256) {
+    for (uint32_t j = blockIdx.y * 256u + threadIdx.x; j < c; j += 256u * gridDim.y) {
         const uint32_t i = idx[j];
         const uint32_t pos = rank.of(i);
         oi[pos] = i;
@@ -307,7 +310,8 @@ void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SegList rep
                        static_cast<uint32_t*>(nullptr), qout.count);
     hipLaunchKernelGGL((layout_kernel<T>), dim3(1), dim3(1), 0, s, archive, capacity, n, np_total, qout, flags, lx,
                        lz, bz, quantum, result);
-    hipLaunchKernelGGL((place_repairs<T>), dim3(repairs.segments), dim3(256), 0, s, archive, repairs, rr, result);
+    hipLaunchKernelGGL((place_repairs<T>), dim3(repairs.segments, kPlaceSlices), dim3(256), 0, s, archive, repairs, rr,
+                       result);
     hipLaunchKernelGGL(place_qoutliers, dim3(256), dim3(256), 0, s, archive, qout, qr, uint64_t(sizeof(T)), result);
 }
 

@@ -15,6 +15,7 @@
 // Y and Z lines map consecutive lanes to consecutive x, so every register load
 // is a fully coalesced 256 B (f32) / 512 B (f64) wave access; X lines map lanes
 // to consecutive tiles of one row (overlapping 64 B windows served from L1).
+#include <cmath>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -235,6 +236,7 @@ Tuning tuning_for(Dims full, unsigned lxy, unsigned width, bool inverse) {
     const auto it = g_tune.find(TuneKey{full.nx, full.ny, full.nz, width, inverse ? 1u : 0u});
     Tuning t = it != g_tune.end() ? it->second : heuristic_impl(full, lxy);
     if (t.fused > lxy) t.fused = lxy;
+    if (!fused_plane_fits(full.nx, full.ny, width)) t.fused = 0;  // planes of 4 GiB and more: axis passes
     return t;
 }
 
@@ -339,6 +341,8 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
             a.status = emit->status;
             a.orig = emit->orig;
             a.bound = emit->bound;
+            a.bound_lo = float(emit->bound);
+            if (double(a.bound_lo) > emit->bound) a.bound_lo = std::nextafter(a.bound_lo, 0.0f);
             a.rep = emit->rep;
             if (emit->mode == kEmitRepair) {  // the segment layout the caller allocated fixes the z segment
                 a.sz = emit->rep.cap / (64u * 32u);
@@ -362,7 +366,7 @@ void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width
     if (!dyadic || !lxy) return;
     const Tuning tu = tuning_for(full, lxy, width, true);
     if (!tu.fused) return;
-    const uint32_t sz = tu.sz[0];
+    const uint32_t sz = width == 4 ? repair_z_segment<float>(full, tu.sz[0]) : repair_z_segment<double>(full, tu.sz[0]);
     *segments = uint32_t(inv_segments(full, sz));
     *cap = 64u * 32u * sz;
 }

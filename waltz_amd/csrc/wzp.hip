@@ -78,16 +78,22 @@ struct BrickSource {
         pair_subblock(nx, ny, c, t, X, Y, Z);
         uint32_t sw = 0;
         bool bad = false;
+        // all eight rows in flight at once (the write-backs and q-outlier
+        // atomics below would otherwise hold each load until the previous
+        // row is done)
+        T v[8][4];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+            load4(coeff + (uint64_t(Z + (r >> 2)) * ny + (Y + (r & 3u))) * nx + X, v[r]);
 #pragma unroll
         for (uint32_t r = 0; r < 8; ++r) {
             const uint32_t dy = r & 3u, dz = r >> 2;
             const uint64_t at = (uint64_t(Z + dz) * ny + (Y + dy)) * nx + X;
-            T v[4], d[4];
-            load4(coeff + at, v);
+            T d[4];
 #pragma unroll
             for (uint32_t dx = 0; dx < 4; ++dx) {
                 // cpu/src/quantize_avx2.cpp:45-62: finite, |scaled| < 2^63, llrint, INT32 saturation
-                const T scaled = v[dx] * inv;
+                const T scaled = v[r][dx] * inv;
                 if (!__builtin_isfinite(scaled) || !(__builtin_fabs(scaled) < T(9223372036854775808.0))) bad = true;
                 const T rounded = __builtin_rint(scaled);
                 const bool negative = rounded < T(0);
@@ -119,8 +125,11 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
                                                          uint8_t* __restrict__ out, uint64_t capacity, WzpScratch w,
                                                          unsigned long long* __restrict__ total,
                                                          uint32_t* __restrict__ status) {
-    __shared__ __attribute__((aligned(16))) uint32_t tr[kChunk / 2];  // BIT_2 planes, 16 KiB
-    __shared__ __attribute__((aligned(16))) uint32_t ob32[(kChunkBytes + kSignSlot) / 4 + 4];  // payload
+    // BIT_2 planes (16 KiB) and, once every thread holds its share of them in
+    // registers, the assembled payload: one LDS buffer
+    __shared__ __attribute__((aligned(16))) uint32_t stage[(kChunkBytes + kSignSlot) / 4 + 4];
+    uint32_t* const tr = stage;
+    uint32_t* const ob32 = stage;
     __shared__ uint32_t csel[16];                      // byte-compaction selectors
     __shared__ unsigned long long b0s[kThreads];       // B0, 2048 bytes
     __shared__ __attribute__((aligned(8))) uint8_t b1s[kThreads];
@@ -135,8 +144,6 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     if (t == 0) chunk_sh = atomicAdd(w.ticket, 1u);
     signbuf[t] = 0;
     if (t < 16) csel[t] = compaction_selector(t);
-    for (uint32_t i = t; i < (kChunkBytes + kSignSlot) / 16 + 1; i += kThreads)
-        reinterpret_cast<uint4*>(ob32)[i] = make_uint4(0, 0, 0, 0);
     if (t == 0) signbuf[kThreads] = 0;
     __syncthreads();
     const uint32_t c = chunk_sh;
@@ -155,21 +162,36 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     // 2. BIT_2 planes into LDS (plane p of block b at u16 index p*blocks+b)
     uint16_t* tr16 = reinterpret_cast<uint16_t*>(tr);
     const uint32_t blocks = E / 16;
+    if (2 * t + 1 < blocks) {  // both blocks whole: one two-block transform
+        uint32_t pp[16];
+        bit2_forward_pair(m, pp);
+        if (!(blocks & 1u)) {  // the pair's u16 slots share one aligned word
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t b = 2 * t + h;
-        if (b < blocks) {
-            uint32_t p[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) p[j] = m[16 * h + j];
-            bit2_forward(p);
-#pragma unroll
-            for (int pl = 0; pl < 16; ++pl) tr16[pl * blocks + b] = uint16_t(p[pl]);
+            for (int pl = 0; pl < 16; ++pl) tr[pl * (blocks >> 1) + t] = pp[pl];
         } else {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const uint32_t i = first + 16 * h + j;
-                if (i < E) tr16[i] = uint16_t(m[16 * h + j]);
+            for (int pl = 0; pl < 16; ++pl) {
+                tr16[pl * blocks + 2 * t] = uint16_t(pp[pl]);
+                tr16[pl * blocks + 2 * t + 1] = uint16_t(pp[pl] >> 16);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t b = 2 * t + h;
+            if (b < blocks) {
+                uint32_t p[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) p[j] = m[16 * h + j];
+                bit2_forward(p);
+#pragma unroll
+                for (int pl = 0; pl < 16; ++pl) tr16[pl * blocks + b] = uint16_t(p[pl]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const uint32_t i = first + 16 * h + j;
+                    if (i < E) tr16[i] = uint16_t(m[16 * h + j]);
+                }
             }
         }
     }
@@ -224,6 +246,12 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     const uint32_t stored = raw ? raw_bytes : rze;
     const uint32_t sign_bytes = (NNZ + 7) / 8;
     const uint32_t chunk_bytes = NNZ ? stored + sign_bytes : 0u;
+    // the planes are in registers (tw) since the B0 barrier: clear the buffer
+    // for the OR-assembled payload (a raw chunk re-reads its planes instead;
+    // the barrier after the lookback orders both before the payload writes)
+    if (!raw)
+        for (uint32_t i = t; i < (kChunkBytes + kSignSlot) / 16 + 1; i += kThreads)
+            reinterpret_cast<uint4*>(ob32)[i] = make_uint4(0, 0, 0, 0);
 
     // 4. compacted signs of the nonzero magnitudes, LSB first, into LDS
     {
@@ -306,11 +334,26 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     if (!NNZ) return;
     // 7. assemble the chunk payload in LDS
     uint8_t* ob = reinterpret_cast<uint8_t*>(ob32);
-    if (raw) {
+    if (raw) {  // LE u16 magnitudes in place: whole blocks from their staged
+                // planes; the samples past the last whole block are staged
+                // verbatim at their own positions already
+        const uint16_t* tr16c = reinterpret_cast<const uint16_t*>(tr);
+        uint32_t pp[16], mr[32];
+        if (!(blocks & 1u)) {  // (indices stay inside the buffer for every thread)
+#pragma unroll
+            for (int pl = 0; pl < 16; ++pl) pp[pl] = tr[pl * (blocks >> 1) + t];
+        } else {
+#pragma unroll
+            for (int pl = 0; pl < 16; ++pl)
+                pp[pl] = uint32_t(tr16c[pl * blocks + 2 * t]) | (uint32_t(tr16c[pl * blocks + 2 * t + 1]) << 16);
+        }
+        bit2_inverse_pair(pp, mr);
+        const uint32_t whole = 2 * t + 1 < blocks ? 32u : (2 * t < blocks ? 16u : 0u);
+        __syncthreads();  // all planes read before the buffer is overwritten
         uint16_t* ob16 = reinterpret_cast<uint16_t*>(ob32);
 #pragma unroll
         for (int i = 0; i < 32; ++i)
-            if (first + i < E) ob16[first + i] = uint16_t(m[i]);
+            if (uint32_t(i) < whole) ob16[first + i] = uint16_t(mr[i]);
     } else {
         {  // nonzero bytes in order: each word's nonzero bytes packed by one
            // byte permute, gathered in an 8-byte accumulator and OR-ed into

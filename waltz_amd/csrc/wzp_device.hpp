@@ -139,11 +139,46 @@ WZ_HD uint32_t expansion_selector(uint32_t m) {
 
 // Decodes chunk c of a WZP stream of `bytes` bytes holding n magnitudes;
 // called by all 256 threads of a workgroup.
+// The 32 magnitudes [32t, 32t + 32) of a chunk of E from its staged BIT_2
+// planes (plane p of block b at u16 index p * blocks + b, the tail past the
+// last whole block verbatim); zero past E.
+WZ_D void bit2_unstage(const uint32_t* tr, uint32_t E, uint32_t t, uint32_t (&m)[32]) {
+    const uint16_t* tr16 = reinterpret_cast<const uint16_t*>(tr);
+    const uint32_t blocks = E / 16, first = 32 * t;
+    // every thread runs the two-block transform (indices stay inside the
+    // 8192-entry buffer even for threads past the last whole block) ...
+    uint32_t pp[16];
+    if (!(blocks & 1u)) {
+#pragma unroll
+        for (int pl = 0; pl < 16; ++pl) pp[pl] = tr[pl * (blocks >> 1) + t];
+    } else {
+#pragma unroll
+        for (int pl = 0; pl < 16; ++pl)
+            pp[pl] = uint32_t(tr16[pl * blocks + 2 * t]) | (uint32_t(tr16[pl * blocks + 2 * t + 1]) << 16);
+    }
+    bit2_inverse_pair(pp, m);
+    // ... and threads whose second block is not whole take the samples past
+    // the last whole block verbatim (selects only: m stays in registers)
+    if (2 * t + 1 >= blocks) {
+        const bool whole0 = 2 * t < blocks;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const uint32_t i = first + j;
+            const uint32_t raw = i < E ? uint32_t(tr16[i]) : 0u;
+            m[j] = (j < 16 && whole0) ? m[j] : raw;
+        }
+    }
+}
+
 template <class Sink>
 WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t n, uint32_t chunks, uint32_t groups,
                        uint32_t c, const Sink& sink, uint32_t* __restrict__ status) {
-    __shared__ __attribute__((aligned(16))) uint32_t tr[kChunk / 2];
-    __shared__ uint32_t ib32[(kChunkBytes + kSignSlot) / 4 + 4];  // staged chunk payload
+    // the staged chunk payload and, once every lane holds its bytes in
+    // registers, the BIT_2 planes: one LDS buffer; the sign bytes are copied
+    // aside before it is reused
+    __shared__ __attribute__((aligned(16))) uint32_t ib32[(kChunkBytes + kSignSlot) / 4 + 4];
+    uint32_t* const tr = ib32;
+    __shared__ uint32_t sg32[kSignSlot / 4];  // sign bytes
     __shared__ uint32_t esel[16];           // byte-expansion selectors
     __shared__ uint32_t b1w[kThreads / 4];  // B1 as 64 words (256 bytes)
     __shared__ uint32_t b2w[8];             // B2 as 8 words (32 bytes)
@@ -189,24 +224,42 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
         sink.store(c, t, E, zero, 0u);
         return;
     }
-    // stage the chunk payload [g0, g1) in LDS with aligned word loads; the
-    // words never leave the stream [in, in + bytes)
-    const uintptr_t g0 = reinterpret_cast<uintptr_t>(in + header + info[0]), g1 = g0 + mbytes + sbytes;
-    const uintptr_t a0 = g0 & ~uintptr_t(3), limit = reinterpret_cast<uintptr_t>(in + bytes);
-    for (uintptr_t a = a0 + 4u * t; a < g1; a += 4u * kThreads) {
-        uint32_t v;
-        if (a >= reinterpret_cast<uintptr_t>(in) && a + 4 <= limit) {
-            v = *reinterpret_cast<const uint32_t*>(a);
-        } else {
-            v = 0;
-            for (uint32_t k = 0; k < 4; ++k)
-                if (a + k >= reinterpret_cast<uintptr_t>(in) && a + k < limit)
-                    v |= uint32_t(*reinterpret_cast<const uint8_t*>(a + k)) << (8 * k);
+    // stage the chunk payload in LDS with aligned word loads that never leave
+    // the stream [in, in + bytes); six words per thread are in flight per
+    // round (a typical chunk takes one or two rounds)
+    const uint8_t* const p0 = in + header + info[0];
+    const uint32_t mis = uint32_t(reinterpret_cast<uintptr_t>(p0) & 3u);
+    const uint32_t* const w0 = reinterpret_cast<const uint32_t*>(p0 - mis);
+    const uint32_t nwords = (mis + mbytes + sbytes + 3) / 4;
+    constexpr uint32_t kRound = 6;
+    if (reinterpret_cast<const uint8_t*>(w0) >= in &&
+        reinterpret_cast<const uint8_t*>(w0 + nwords) <= in + bytes) {  // the usual case
+        for (uint32_t base = t; base < nwords; base += kRound * kThreads) {
+            uint32_t v[kRound];
+#pragma unroll
+            for (uint32_t k = 0; k < kRound; ++k) {
+                const uint32_t i = base + k * kThreads;
+                v[k] = i < nwords ? w0[i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kRound; ++k) {
+                const uint32_t i = base + k * kThreads;
+                if (i < nwords) ib32[i] = v[k];
+            }
         }
-        ib32[(a - a0) >> 2] = v;
+    } else {  // a word at an end of the stream: bytes inside it only
+        for (uint32_t i = t; i < nwords; i += kThreads) {
+            const uint8_t* a = reinterpret_cast<const uint8_t*>(w0 + i);
+            uint32_t v = 0;
+            for (uint32_t k = 0; k < 4; ++k)
+                if (a + k >= in && a + k < in + bytes) v |= uint32_t(a[k]) << (8 * k);
+            ib32[i] = v;
+        }
     }
     __syncthreads();
-    const uint8_t* src = reinterpret_cast<const uint8_t*>(ib32) + (g0 - a0);
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(ib32) + mis;
+    uint8_t* const ps = reinterpret_cast<uint8_t*>(sg32);
+    for (uint32_t i = t; i < sbytes; i += kThreads) ps[i] = src[mbytes + i];
     uint32_t m[32];
     bool bad = false;
     if (raw) {
@@ -262,6 +315,7 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
         uint32_t K0;
         const uint32_t e0 = block_exclusive(popc64(b0t), sh, K0);
         if (pos1 - int32_t(K0) != 0) bad = true;
+        uint32_t tw[16];
         if (!bad) {
             // this lane's nonzero bytes src[e0, e0 + popcount(b0t)), spread
             // word by word with byte permutes from an 8-byte window of
@@ -272,7 +326,6 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
             unsigned long long acc = (((unsigned long long)ib32[wi + 1] << 32) | ib32[wi]) >> (8 * sh0);
             uint32_t navail = 8 - sh0;
             wi += 2;
-            uint32_t tw[16];
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 if (navail < 4) {
@@ -285,32 +338,16 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
                 acc >>= 8 * c;
                 navail -= c;
             }
+        }
+        __syncthreads();  // every lane's bytes are in registers: the buffer takes the planes
+        if (!bad) {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 reinterpret_cast<uint4*>(tr)[4 * t + q] = make_uint4(tw[4 * q], tw[4 * q + 1], tw[4 * q + 2],
                                                                       tw[4 * q + 3]);
         }
         __syncthreads();
-        const uint16_t* tr16 = reinterpret_cast<const uint16_t*>(tr);
-        const uint32_t blocks = E / 16;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const uint32_t b = 2 * t + h;
-            if (b < blocks) {
-                uint32_t p[16];
-#pragma unroll
-                for (int pl = 0; pl < 16; ++pl) p[pl] = tr16[pl * blocks + b];
-                bit2_inverse(p);
-#pragma unroll
-                for (int j = 0; j < 16; ++j) m[16 * h + j] = p[j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const uint32_t i = first + 16 * h + j;
-                    m[16 * h + j] = i < E ? uint32_t(tr16[i]) : 0u;
-                }
-            }
-        }
+        bit2_unstage(tr, E, t, m);
     }
     uint32_t nzmask = 0;
 #pragma unroll
@@ -322,7 +359,6 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
         if (t == 0 || (threadIdx.x & 63u) == 0) atomicOr(status, uint32_t(kCorrupt));
         return;
     }
-    const uint8_t* ps = src + mbytes;
     const uint32_t cnt = popc32(nzmask);
     uint32_t bits = 0;
     if (cnt) {
