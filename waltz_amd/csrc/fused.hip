@@ -354,7 +354,8 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     // Waves then read from one base pointer (wave-uniform) with 32-bit byte
     // offsets; only the one wave that straddles the two groups selects per lane.
     constexpr uint32_t kLow = (FW / 2) * (FH / 2);
-    uint32_t eoff[FSLOTS], ooff[FSLOTS], zoff[FSLOTS];  // even-plane, odd-plane, Zp offsets
+    uint32_t eoff[FSLOTS], ooff[FSLOTS];   // even-plane and odd-plane byte offsets
+    uint32_t zpk[(FSLOTS + 1) / 2] = {};     // Zp offsets, two 16-bit halves per word
     uint32_t low_lanes = 0, lo_wave = 0, hi_wave = 0;   // per slot: lane / whole-wave classes
 #pragma unroll
     for (int s = 0; s < FSLOTS; ++s) {
@@ -374,7 +375,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         const uint32_t ohi = (cy * uint32_t(a.hpy) + cx) * uint32_t(sizeof(T));
         eoff[s] = low ? (cy * uint32_t(a.lpy) + cx) * uint32_t(sizeof(T)) : ohi;
         ooff[s] = ohi;
-        zoff[s] = r * PA + c;
+        zpk[s / 2] |= (r * PA + c) << (16 * (s & 1));
         const unsigned long long lw = __ballot(low);
         low_lanes |= uint32_t(low) << s;
         lo_wave |= uint32_t(lw == ~0ull) << s;
@@ -474,8 +475,9 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             }
             if (phase == H - 2) SF[s] = oraw;  // o_raw[B-2] for the next tile's fold
             if (emit) {
-                Zp[zoff[s]] = even;
-                Zp[FH * PA + zoff[s]] = odd;
+                const uint32_t zo = (zpk[s / 2] >> (16 * (s & 1))) & 0xFFFFu;
+                Zp[zo] = even;
+                Zp[FH * PA + zo] = odd;
             }
             OS[s] = os;
             EA[s] = eA;
@@ -531,12 +533,14 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 atomicOr(a.status, uint32_t(kNonFiniteOutput));
         } else {
             if constexpr (!early) fetch_orig(g, ref);
-            // decide each sample as the lifting finishes it; the rare rows
-            // with records reserve the wave's entries with one LDS atomic
+            // decide each sample as the lifting finishes it; then the step's
+            // eight row masks go out as one 16-lane store of bitmap words and
+            // its records take one LDS reservation
             const uint32_t lane = t & 63u;
-            const unsigned long long below = (1ull << lane) - 1ull;
             const T nb = T(a.bound);
             bool ok = true;
+            unsigned long long m8[8];  // wave-uniform patch masks of the eight rows
+            T err8[8];
             auto decide = [&](uint32_t j, T v, T orig) {
                 const uint32_t Y = y0 + 8 * yt + j;
                 const bool inb = xz && Y < n.ny;
@@ -563,32 +567,48 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 } else if (inb) {
                     patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
                 }
-                const unsigned long long m = __ballot(patch);
-                if (rowwords) {  // 64 consecutive x at a 32-aligned row start: own two words
-                    if (lane == 0 && Z < n.nz && Y < n.ny) {
-                        const uint64_t w0 = ((uint64_t(Z) * n.ny + Y) * n.nx + x0) >> 5;
-                        a.rep.bitmap[w0] = uint32_t(m);
-                        if (x0 + 32 < n.nx) a.rep.bitmap[w0 + 1] = uint32_t(m >> 32);
-                    }
-                } else if (patch) {
+                m8[j] = __ballot(patch);
+                err8[j] = err;
+                if (!rowwords && patch) {
                     const uint64_t at = (uint64_t(Z) * n.ny + Y) * n.nx + X;
                     atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
-                }
-                if (m) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&seg_fill, uint32_t(__popcll(m)));
-                    base = __shfl(base, 0, 64);
-                    if (patch) {
-                        const uint64_t at = uint64_t(wg) * a.rep.cap + base + uint32_t(__popcll(m & below));
-                        a.rep.index[at] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
-                        static_cast<T*>(a.rep.payload)[at] = err;
-                    }
                 }
             };
             inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
                 decide(2 * (j - 2), e, ref[2 * (j - 2)]);
                 decide(2 * (j - 2) + 1, o, ref[2 * (j - 2) + 1]);
             });
+            if (rowwords) {  // 64 consecutive x at a 32-aligned row start: each row owns two words
+                uint32_t wv = 0;
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if ((lane >> 1) == uint32_t(r)) wv = (lane & 1u) ? uint32_t(m8[r] >> 32) : uint32_t(m8[r]);
+                const uint32_t Y = y0 + 8 * yt + (lane >> 1);
+                if (lane < 16 && Z < n.nz && Y < n.ny && ((lane & 1u) == 0 || x0 + 32 < n.nx))
+                    a.rep.bitmap[(((uint64_t(Z) * n.ny + Y) * n.nx + x0) >> 5) + (lane & 1u)] = wv;
+            }
+            uint32_t total = 0;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) total += uint32_t(__popcll(m8[r]));
+            if (total) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&seg_fill, total);
+                base = __builtin_amdgcn_readfirstlane(base);
+                const uint64_t seg0 = uint64_t(wg) * a.rep.cap;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const unsigned long long m = m8[r];
+                    if ((m >> lane) & 1ull) {
+                        // entries of lower lanes: mbcnt counts the mask bits below this lane
+                        const uint32_t mine =
+                            __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                        const uint32_t Y = y0 + 8 * yt + uint32_t(r);
+                        a.rep.index[seg0 + base + mine] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
+                        static_cast<T*>(a.rep.payload)[seg0 + base + mine] = err8[r];
+                    }
+                    base += uint32_t(__popcll(m));
+                }
+            }
             if (__ballot(!ok) && lane == 0) atomicOr(a.status, uint32_t(kRepairBound));
         }
     }

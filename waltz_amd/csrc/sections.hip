@@ -115,12 +115,10 @@ void detect_repairs(const T* original, const T* reconstructed, uint32_t n, doubl
 // wpre[word] + popcount of the lower bits of the word.
 constexpr uint32_t kPrefixWords = 8192;  // 256 threads x 32 words
 
-__global__ void __launch_bounds__(256) word_prefix(const uint32_t* __restrict__ bm, uint32_t words,
-                                                   uint32_t* __restrict__ wpre, uint32_t* __restrict__ bsum,
-                                                   const uint32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
+WZ_D void word_prefix_block(const uint32_t* __restrict__ bm, uint32_t words, uint32_t* __restrict__ wpre,
+                             uint32_t* __restrict__ bsum, uint32_t block) {
     __shared__ uint32_t sh[4];
-    const uint64_t w0 = uint64_t(blockIdx.x) * kPrefixWords + 32u * threadIdx.x;
+    const uint64_t w0 = uint64_t(block) * kPrefixWords + 32u * threadIdx.x;
     uint32_t c[32];
     if (w0 + 32 <= words) {
         const uint4* v = reinterpret_cast<const uint4*>(bm + w0);
@@ -163,14 +161,27 @@ __global__ void __launch_bounds__(256) word_prefix(const uint32_t* __restrict__ 
             run += c[i];
         }
     }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    if (threadIdx.x == 0) bsum[block] = total;
+}
+
+// Word prefixes of both flag bitmaps in one launch: workgroups [0, blocks)
+// take the repair bitmap, [blocks, 2 blocks) the q-outlier bitmap (skipped
+// when there are no q-outliers).
+__global__ void __launch_bounds__(256) word_prefix2(const uint32_t* __restrict__ rbm, uint32_t* __restrict__ rwpre,
+                                                    uint32_t* __restrict__ rbsum, const uint32_t* __restrict__ qbm,
+                                                    uint32_t* __restrict__ qwpre, uint32_t* __restrict__ qbsum,
+                                                    const uint32_t* __restrict__ qcount, uint32_t words,
+                                                    uint32_t blocks) {
+    if (blockIdx.x < blocks) {
+        word_prefix_block(rbm, words, rwpre, rbsum, blockIdx.x);
+    } else if (*qcount != 0) {
+        word_prefix_block(qbm, words, qwpre, qbsum, blockIdx.x - blocks);
+    }
 }
 
 // In-place exclusive scan of a[0..len) by one 1024-thread workgroup; the sum
 // goes to *total. Skipped (a untouched, *total untouched) when *gate == 0.
-__global__ void __launch_bounds__(1024) exclusive_scan(uint32_t* a, uint32_t len, uint32_t* total,
-                                                       const uint32_t* gate) {
-    if (gate && *gate == 0) return;
+WZ_D void exclusive_scan_block(uint32_t* a, uint32_t len, uint32_t* total) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry_sh;
     if (threadIdx.x == 0) carry_sh = 0;
@@ -193,12 +204,20 @@ __global__ void __launch_bounds__(1024) exclusive_scan(uint32_t* a, uint32_t len
         __syncthreads();
     }
     if (threadIdx.x == 0 && total) *total = carry_sh;
+    __syncthreads();
 }
 
+// One 1024-thread workgroup: scans both block sums (the repair total lands in
+// *np_dev), then thread 0 lays out the archive sections and writes the header.
 template <class T>
-__global__ void layout_kernel(uint8_t* __restrict__ archive, uint64_t capacity, uint32_t n,
-                              const uint32_t* __restrict__ np_dev, SideList qo, uint8_t flags, uint8_t lx, uint8_t lz,
-                              uint8_t bz, double quantum, EncodeResult* __restrict__ res) {
+__global__ void __launch_bounds__(1024) scan_layout_kernel(uint8_t* __restrict__ archive, uint64_t capacity,
+                                                           uint32_t n, uint32_t* rbsum, uint32_t* qbsum,
+                                                           uint32_t blocks, uint32_t* np_dev, SideList qo,
+                                                           uint8_t flags, uint8_t lx, uint8_t lz, uint8_t bz,
+                                                           double quantum, EncodeResult* __restrict__ res) {
+    exclusive_scan_block(rbsum, blocks, np_dev);
+    if (*qo.count != 0) exclusive_scan_block(qbsum, blocks, nullptr);
+    if (threadIdx.x != 0) return;
     const uint32_t np = *np_dev, nq = *qo.count;
     uint32_t st = res->status;
     if (nq > qo.capacity || (n > (1u << 20) && nq >= (1u << 20)) || (n > (1u << 22) && np >= (1u << 22)))
@@ -247,10 +266,9 @@ struct Ranks {
 // chains are in flight.
 constexpr uint32_t kPlaceSlices = 8;
 template <class T>
-__global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archive, SegList rp, Ranks rank,
-                                                     const EncodeResult* __restrict__ res) {
-    if (res->status) return;
-    const uint32_t k = blockIdx.x, c = rp.count[k];
+WZ_D void place_repairs(uint8_t* __restrict__ archive, SegList rp, Ranks rank, const EncodeResult* __restrict__ res,
+                        uint32_t k, uint32_t slice) {
+    const uint32_t c = rp.count[k];
     if (!c) return;
     const uint32_t np = res->np;
     const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * sizeof(T), res->nq);
@@ -258,7 +276,7 @@ __global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archi
     const T* val = static_cast<const T*>(rp.payload) + uint64_t(k) * rp.cap;
     uint32_t* oi = reinterpret_cast<uint32_t*>(archive + L.pi);
     T* ov = reinterpret_cast<T*>(archive + L.pv);
-    for (uint32_t j = blockIdx.y * 256u + threadIdx.x; j < c; j += 256u * gridDim.y) {
+    for (uint32_t j = slice * 256u + threadIdx.x; j < c; j += 256u * kPlaceSlices) {
         const uint32_t i = idx[j];
         const uint32_t pos = rank.of(i);
         oi[pos] = i;
@@ -266,19 +284,31 @@ __global__ void __launch_bounds__(256) place_repairs(uint8_t* __restrict__ archi
     }
 }
 
-__global__ void __launch_bounds__(256) place_qoutliers(uint8_t* __restrict__ archive, SideList qo, Ranks rank,
-                                                       uint64_t repair_bytes_per, const EncodeResult* __restrict__ res) {
-    if (res->status) return;
+constexpr uint32_t kPlaceQBlocks = 256;
+WZ_D void place_qoutliers(uint8_t* __restrict__ archive, SideList qo, Ranks rank, uint64_t repair_bytes_per,
+                          const EncodeResult* __restrict__ res, uint32_t block) {
     const uint32_t np = res->np, nq = res->nq;
     if (!nq) return;
     const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * repair_bytes_per, nq);
-    const uint32_t stride = gridDim.x * 256u;
-    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < nq; j += stride) {
+    const uint32_t stride = kPlaceQBlocks * 256u;
+    for (uint32_t j = block * 256u + threadIdx.x; j < nq; j += stride) {
         const uint32_t idx = qo.index[j];
         const uint32_t pos = rank.of(idx);
         reinterpret_cast<uint32_t*>(archive + L.qi)[pos] = idx;
         reinterpret_cast<uint32_t*>(archive + L.qv)[pos] = static_cast<const uint32_t*>(qo.payload)[j];
     }
+}
+
+// Both side channels to their index-order slots in one launch: workgroup
+// g < segments * kPlaceSlices places repairs of segment g / kPlaceSlices,
+// the remaining kPlaceQBlocks place q-outliers.
+template <class T>
+__global__ void __launch_bounds__(256) place_kernel(uint8_t* __restrict__ archive, SegList rp, Ranks rr, SideList qo,
+                                                    Ranks qr, const EncodeResult* __restrict__ res) {
+    if (res->status) return;
+    const uint32_t g = blockIdx.x, rblocks = rp.segments * kPlaceSlices;
+    if (g < rblocks) place_repairs<T>(archive, rp, rr, res, g / kPlaceSlices, g % kPlaceSlices);
+    else place_qoutliers(archive, qo, qr, uint64_t(sizeof(T)), res, g - rblocks);
 }
 
 // Scratch layout: [repair wpre | repair bsum | q wpre | q bsum], each part a
@@ -300,19 +330,14 @@ void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SegList rep
     const Ranks rr{repairs.bitmap, scratch, scratch + pad64(words)};
     const Ranks qr{qout.bitmap, scratch + half, scratch + half + pad64(words)};
     uint32_t* np_total = &result->pad;  // repair total from the bitmap scan
-    hipLaunchKernelGGL(word_prefix, dim3(blocks), dim3(256), 0, s, repairs.bitmap, words, const_cast<uint32_t*>(rr.wpre),
-                       const_cast<uint32_t*>(rr.bsum), static_cast<const uint32_t*>(nullptr));
-    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, const_cast<uint32_t*>(rr.bsum), blocks, np_total,
-                       static_cast<const uint32_t*>(nullptr));
-    hipLaunchKernelGGL(word_prefix, dim3(blocks), dim3(256), 0, s, qout.bitmap, words, const_cast<uint32_t*>(qr.wpre),
-                       const_cast<uint32_t*>(qr.bsum), qout.count);
-    hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, s, const_cast<uint32_t*>(qr.bsum), blocks,
-                       static_cast<uint32_t*>(nullptr), qout.count);
-    hipLaunchKernelGGL((layout_kernel<T>), dim3(1), dim3(1), 0, s, archive, capacity, n, np_total, qout, flags, lx,
-                       lz, bz, quantum, result);
-    hipLaunchKernelGGL((place_repairs<T>), dim3(repairs.segments, kPlaceSlices), dim3(256), 0, s, archive, repairs, rr,
-                       result);
-    hipLaunchKernelGGL(place_qoutliers, dim3(256), dim3(256), 0, s, archive, qout, qr, uint64_t(sizeof(T)), result);
+    hipLaunchKernelGGL(word_prefix2, dim3(2 * blocks), dim3(256), 0, s, repairs.bitmap, const_cast<uint32_t*>(rr.wpre),
+                       const_cast<uint32_t*>(rr.bsum), qout.bitmap, const_cast<uint32_t*>(qr.wpre),
+                       const_cast<uint32_t*>(qr.bsum), static_cast<const uint32_t*>(qout.count), words, blocks);
+    hipLaunchKernelGGL((scan_layout_kernel<T>), dim3(1), dim3(1024), 0, s, archive, capacity, n,
+                       const_cast<uint32_t*>(rr.bsum), const_cast<uint32_t*>(qr.bsum), blocks, np_total, qout, flags,
+                       lx, lz, bz, quantum, result);
+    hipLaunchKernelGGL((place_kernel<T>), dim3(repairs.segments * kPlaceSlices + kPlaceQBlocks), dim3(256), 0, s,
+                       archive, repairs, rr, qout, qr, result);
 }
 
 // ------------------------------------------------------------------ decode patches
