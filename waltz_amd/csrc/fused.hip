@@ -180,8 +180,9 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 #pragma unroll
     for (int s = 0; s < FSLOTS; ++s) {
         const uint32_t idx = t + FT * s, r = idx / FW, c = idx % FW;
-        loff[s] = reflect_pos(int32_t(y0) - 4 + int32_t(r), n.ny) * uint32_t(a.spy) +
-                  reflect_pos(int32_t(x0) - 4 + int32_t(c), n.nx);
+        loff[s] = (reflect_pos(int32_t(y0) - 4 + int32_t(r), n.ny) * uint32_t(a.spy) +
+                   reflect_pos(int32_t(x0) - 4 + int32_t(c), n.nx)) *
+                  uint32_t(sizeof(T));  // byte offset in the plane
         aoff[s] = r * PA + c;
     }
     // Z ownership (threads < 512): row zr, columns 4 zq .. 4 zq + 3 of the X stage
@@ -199,12 +200,12 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
     // the two planes of the next step are in flight while this step runs
     T pe[FSLOTS], po[FSLOTS];
     {
-        const T* p0 = a.src + uint64_t(reflect_pos(int32_t(z0) - 4, n.nz)) * a.spz;
-        const T* p1 = a.src + uint64_t(reflect_pos(int32_t(z0) - 3, n.nz)) * a.spz;
+        const auto w0 = window(a.src + uint64_t(reflect_pos(int32_t(z0) - 4, n.nz)) * a.spz);
+        const auto w1 = window(a.src + uint64_t(reflect_pos(int32_t(z0) - 3, n.nz)) * a.spz);
 #pragma unroll
         for (int s = 0; s < FSLOTS; ++s) {
-            pe[s] = p0[loff[s]];
-            po[s] = p1[loff[s]];
+            pe[s] = buf_load<T>(w0, loff[s]);
+            po[s] = buf_load<T>(w1, loff[s]);
         }
     }
     for (int i = 0; i < npairs; ++i) {
@@ -218,12 +219,12 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
         }
         if (i < npairs - 1) {
             const int32_t pn = int32_t(z0) - 4 + 2 * (i + 1);
-            const T* p0 = a.src + uint64_t(reflect_pos(pn, n.nz)) * a.spz;
-            const T* p1 = a.src + uint64_t(reflect_pos(pn + 1, n.nz)) * a.spz;
+            const auto w0 = window(a.src + uint64_t(reflect_pos(pn, n.nz)) * a.spz);
+            const auto w1 = window(a.src + uint64_t(reflect_pos(pn + 1, n.nz)) * a.spz);
 #pragma unroll
             for (int s = 0; s < FSLOTS; ++s) {
-                pe[s] = p0[loff[s]];
-                po[s] = p1[loff[s]];
+                pe[s] = buf_load<T>(w0, loff[s]);
+                po[s] = buf_load<T>(w1, loff[s]);
             }
         }
         if (a.status && !ok) atomicOr(a.status, uint32_t(kNonFiniteInput));
@@ -540,6 +541,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             const T nb = T(a.bound);
             bool ok = true;
             unsigned long long m8[8];  // wave-uniform patch masks of the eight rows
+            uint32_t pbits = 0;        // this lane's patched rows
             T err8[8];
             auto decide = [&](uint32_t j, T v, T orig) {
                 const uint32_t Y = y0 + 8 * yt + j;
@@ -568,6 +570,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                     patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
                 }
                 m8[j] = __ballot(patch);
+                pbits |= uint32_t(patch) << j;
                 err8[j] = err;
                 if (!rowwords && patch) {
                     const uint64_t at = (uint64_t(Z) * n.ny + Y) * n.nx + X;
@@ -578,14 +581,16 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 decide(2 * (j - 2), e, ref[2 * (j - 2)]);
                 decide(2 * (j - 2) + 1, o, ref[2 * (j - 2) + 1]);
             });
+            // sample index of this lane in row y0 + 8 yt of plane Z (indices fit 32 bits)
+            const uint32_t row0 = uint32_t((uint64_t(Z) * n.ny + y0 + 8 * yt) * n.nx) + X;
             if (rowwords) {  // 64 consecutive x at a 32-aligned row start: each row owns two words
-                uint32_t wv = 0;
+                uint32_t wv = 0;  // lane 2r + h holds word h of row r
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
                     if ((lane >> 1) == uint32_t(r)) wv = (lane & 1u) ? uint32_t(m8[r] >> 32) : uint32_t(m8[r]);
                 const uint32_t Y = y0 + 8 * yt + (lane >> 1);
                 if (lane < 16 && Z < n.nz && Y < n.ny && ((lane & 1u) == 0 || x0 + 32 < n.nx))
-                    a.rep.bitmap[(((uint64_t(Z) * n.ny + Y) * n.nx + x0) >> 5) + (lane & 1u)] = wv;
+                    a.rep.bitmap[((row0 - yx + (lane >> 1) * n.nx) >> 5) + (lane & 1u)] = wv;
             }
             uint32_t total = 0;
 #pragma unroll
@@ -594,19 +599,22 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(&seg_fill, total);
                 base = __builtin_amdgcn_readfirstlane(base);
-                const uint64_t seg0 = uint64_t(wg) * a.rep.cap;
+                // this step's entries start at a wave-uniform slot of the segment
+                const uint64_t first = uint64_t(wg) * a.rep.cap + base;
+                const auto wi = window(a.rep.index + first);
+                const auto wp = window(static_cast<T*>(a.rep.payload) + first);
+                uint32_t before = 0;  // entries of the earlier rows
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                     const unsigned long long m = m8[r];
-                    if ((m >> lane) & 1ull) {
+                    if ((pbits >> r) & 1u) {
                         // entries of lower lanes: mbcnt counts the mask bits below this lane
-                        const uint32_t mine =
-                            __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-                        const uint32_t Y = y0 + 8 * yt + uint32_t(r);
-                        a.rep.index[seg0 + base + mine] = uint32_t((uint64_t(Z) * n.ny + Y) * n.nx + X);
-                        static_cast<T*>(a.rep.payload)[seg0 + base + mine] = err8[r];
+                        const uint32_t k = before + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                                              __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                        __builtin_amdgcn_raw_buffer_store_b32(row0 + uint32_t(r) * n.nx, wi, int(4 * k), 0, 0);
+                        buf_store(err8[r], wp, uint32_t(sizeof(T)) * k);
                     }
-                    base += uint32_t(__popcll(m));
+                    before += uint32_t(__popcll(m));
                 }
             }
             if (__ballot(!ok) && lane == 0) atomicOr(a.status, uint32_t(kRepairBound));
@@ -621,6 +629,8 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
 // ------------------------------------------------------------------ launchers
 template <class T>
 void launch_fwd_level(const FwdLevel<T>& a, hipStream_t s) {
+    // the kernel addresses an input plane with 32-bit byte offsets
+    if (!fused_plane_fits(a.spy, a.n.ny, sizeof(T))) throw std::logic_error("fused forward level: plane exceeds 4 GiB");
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
     hipLaunchKernelGGL((fwd_level_kernel<T>), grid, dim3(FT), 0, s, a);
 }
