@@ -103,7 +103,9 @@ int main() {
                 if (back[j] != m2[j]) ++bad;
         }
 
-        const uint32_t v = rnd(), mask = rnd() & rnd();
+        const uint32_t v = rnd();
+        const uint32_t mask = iter % 4 == 0 ? rnd() & rnd() : iter % 4 == 1 ? rnd() | rnd() : iter % 4 == 2 ? rnd()
+                              : (iter & 8 ? 0xFFFFFFFFu : (iter & 16 ? 0u : 1u << (iter % 32)));
         uint32_t c = 0, k = 0;
         for (int i = 0; i < 32; ++i)
             if ((mask >> i) & 1u) c |= ((v >> i) & 1u) << k++;

@@ -40,6 +40,22 @@ def same_bits(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
 
 
+def archive_diff(got, want, n):
+    """Where two archives of an n-sample field first differ (header, WZP
+    directory / metas / payload, or side sections), for assertion messages."""
+    if got == want:
+        return "identical"
+    m = min(len(got), len(want))
+    d = next((i for i in range(m) if got[i] != want[i]), m)
+    chunks = (n + 8191) // 8192
+    groups = (chunks + 15) // 16
+    wzp = int.from_bytes(want[16:24], "little")
+    region = ("header" if d < 32 else "directory" if d < 32 + 4 * groups else
+              "metas" if d < 32 + 4 * groups + 3 * chunks else "payload" if d < 32 + wzp else "sections")
+    return (f"lengths {len(got)} / {len(want)}, first difference at byte {d} ({region}), "
+            f"{sum(1 for i in range(m) if got[i] != want[i])} differing bytes")
+
+
 # ------------------------------------------------------------------ transform
 TRANSFORM_CASES = [
     ((1, 1, 1), 0, 0, True), ((4, 3, 2), 1, 1, True), ((5, 9, 7), 1, 0, False), ((13, 11, 9), 1, 1, True),
@@ -152,7 +168,8 @@ def test_codec_matches_oracle(wz, oracle, shape, dtype):
     for bound in (1e-3, 1e-5):
         ref = oracle.compress(v, bound)
         a = waltz_amd.compress(dev(v), abs_error=bound)
-        assert host(a).tobytes() == ref
+        got = host(a).tobytes()
+        assert got == ref, archive_diff(got, ref, v.size)
         d = host(waltz_amd.decompress(a, shape=shape, dtype=TDT[dtype]))
         assert same_bits(d, oracle.decompress(ref, shape, dtype))
         assert np.max(np.abs(d.astype(np.float64) - v.astype(np.float64))) <= bound
@@ -174,7 +191,8 @@ def test_codec_paired_bricks(wz, oracle, dtype):
     v = v.astype(dtype)
     ref = oracle.compress(v, 1e-3)
     a = waltz_amd.compress(dev(v), abs_error=1e-3)
-    assert host(a).tobytes() == ref
+    got = host(a).tobytes()
+    assert got == ref, archive_diff(got, ref, v.size)
     d = host(waltz_amd.decompress(a, shape=shape, dtype=TDT[dtype]))
     assert same_bits(d, oracle.decompress(ref, shape, dtype))
     assert np.max(np.abs(d.astype(np.float64) - v.astype(np.float64))) <= 1e-3

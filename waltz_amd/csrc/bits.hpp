@@ -102,27 +102,51 @@ WZB_HD void bit2_inverse_pair(const uint32_t (&planes)[16], uint32_t (&m)[32]) {
     }
 }
 
-// Gather the bits of v selected by mask into the low end, in order (x86 pext).
+// Gather the bits of v selected by mask into the low end, in order (x86
+// pext), branch-free in five rounds: round i moves every selected bit right
+// by 2^i when the count of unselected bits below it has bit i set (the
+// "compress" method of Hacker's Delight, section 7-4).
 WZB_HD uint32_t compress_bits(uint32_t v, uint32_t mask) {
-    uint32_t out = 0, k = 0;
-    while (mask) {
-        const uint32_t low = mask & (0u - mask);
-        if (v & low) out |= 1u << k;
-        ++k;
-        mask ^= low;
+    uint32_t x = v & mask, m = mask, mk = ~mask << 1;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        uint32_t mp = mk ^ (mk << 1);  // parity of the zeros below: prefix XOR
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        const uint32_t mv = mp & m;  // selected bits that move by 2^i this round
+        m = (m ^ mv) | (mv >> (1 << i));
+        const uint32_t t = x & mv;
+        x = (x ^ t) | (t >> (1 << i));
+        mk &= ~mp;
     }
-    return out;
+    return x;
 }
-// Scatter the low bits of v to the positions selected by mask (x86 pdep).
+// Scatter the low bits of v to the positions selected by mask (x86 pdep):
+// the compress rounds run forward to record which bits move, then the moves
+// are undone in reverse order (Hacker's Delight, section 7-5).
 WZB_HD uint32_t expand_bits(uint32_t v, uint32_t mask) {
-    uint32_t out = 0, k = 0;
-    while (mask) {
-        const uint32_t low = mask & (0u - mask);
-        if ((v >> k) & 1u) out |= low;
-        ++k;
-        mask ^= low;
+    uint32_t m = mask, mk = ~mask << 1, moved[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        uint32_t mp = mk ^ (mk << 1);
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        const uint32_t mv = mp & m;
+        moved[i] = mv;
+        m = (m ^ mv) | (mv >> (1 << i));
+        mk &= ~mp;
     }
-    return out;
+    uint32_t x = v;
+#pragma unroll
+    for (int i = 4; i >= 0; --i) {
+        const uint32_t mv = moved[i], t = x << (1 << i);
+        x = (x & ~mv) | (t & mv);
+    }
+    return x & mask;
 }
 
 // Per-byte nonzero flags of a 32-bit word packed into 4 bits (byte i -> bit i).

@@ -364,14 +364,18 @@ int waltz_amd_wzp_encode(const void* mag, const void* sign, uint32_t n, void* ou
         hipStream_t s = as_stream(stream);
         const uint64_t bytes = ws.chunks * 16ull + 64;
         char* scratch = nullptr;
+        uint8_t* staging = nullptr;
         wz::hip_check(hipMalloc(reinterpret_cast<void**>(&scratch), bytes + 512), "wzp scratch");
+        if (hipMalloc(reinterpret_cast<void**>(&staging), wz::wzp_staging_bytes(ws.chunks)) != hipSuccess) {
+            hipFree(scratch);
+            throw std::runtime_error("wzp staging allocation failed");
+        }
         wz::zero_words(scratch, (bytes + 512) / 4, s);
         unsigned long long* desc = reinterpret_cast<unsigned long long*>(scratch);
-        uint32_t* excl = reinterpret_cast<uint32_t*>(scratch + ws.chunks * 8ull);
-        uint32_t* size = excl + ws.chunks;
+        uint32_t* size = reinterpret_cast<uint32_t*>(scratch + ws.chunks * 8ull) + ws.chunks;
         uint32_t* tail = size + ws.chunks;  // ticket, status, pad, total(u64)
         unsigned long long* total = reinterpret_cast<unsigned long long*>(scratch + ((ws.chunks * 16ull + 16 + 7) & ~7ull));
-        wz::WzpScratch w{desc, excl, size, tail};
+        wz::WzpScratch w{desc, staging, size, tail};
         wz::wzp_encode(static_cast<const uint16_t*>(mag), static_cast<const uint32_t*>(sign), n,
                        static_cast<uint8_t*>(out), capacity, w, total, tail + 1, s);
         uint32_t st = 0;
@@ -380,6 +384,7 @@ int waltz_amd_wzp_encode(const void* mag, const void* sign, uint32_t n, void* ou
         wz::hip_check(hipMemcpyAsync(&tot, total, 8, hipMemcpyDeviceToHost, s), "total");
         wz::hip_check(hipStreamSynchronize(s), "wzp encode");
         hipFree(scratch);
+        hipFree(staging);
         if (st) throw std::runtime_error("WZP encode failed (capacity or lookback)");
         *written = n ? size_t(tot) : 0;
     });

@@ -187,12 +187,15 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
             uint32_t *sign, *qbm, *rbm, *qidx, *qval, *ridx, *rcount, *seg, *zeroed;
             T* rerr;
             unsigned long long* desc;
-            uint32_t *excl, *size, *counters;
+            uint32_t *size, *counters;
         } p;
         p.coeff = c.take<T>(n);
         p.t0 = c.take<T>(n);
         p.t1 = c.take<T>(n);
-        p.rec = c.take<T>(n);
+        // the reconstruction of an unfused finest level, and before it the
+        // encoder's staging slots (both are dead by the time the other is used)
+        p.rec = reinterpret_cast<T*>(
+            c.take<uint8_t>(std::max<uint64_t>(uint64_t(n) * sizeof(T), wzp_staging_bytes(ws.chunks))));
         p.mag = c.take<uint16_t>(n);
         p.ridx = c.take<uint32_t>(rslots);
         p.rerr = c.take<T>(rslots);
@@ -201,7 +204,6 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.qidx = c.take<uint32_t>(nq);
         p.qval = c.take<uint32_t>(nq);
         p.seg = c.take<uint32_t>(finish_scratch_words(n));
-        p.excl = c.take<uint32_t>(ws.chunks);
         p.size = c.take<uint32_t>(ws.chunks);
         // zero-initialised region: sign, q bitmap, lookback descriptors, counters
         const size_t z0 = c.used;
@@ -224,7 +226,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
 
     transform_forward<T>(input, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, status, s);
     SideList qo{p.qbm, p.qidx, p.qval, p.counters + 1, nq};
-    WzpScratch wsc{p.desc, p.excl, p.size, p.counters};
+    WzpScratch wsc{p.desc, reinterpret_cast<uint8_t*>(p.rec), p.size, p.counters};
     if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz)) {
         StageClock c("quantize_encode", s);
         quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,

@@ -138,6 +138,7 @@ constexpr uint32_t kChunk = 8192;        // magnitudes per chunk
 constexpr uint32_t kChunkBytes = 16384;  // raw magnitude bytes per chunk
 constexpr uint32_t kGroup = 16;          // chunks per group
 constexpr uint32_t kSignSlot = 1024;     // sign code 0 = full 1024-byte slot
+constexpr uint32_t kChunkSlot = kChunkBytes + kSignSlot;  // largest chunk payload (a multiple of 16)
 constexpr uint32_t kRawCode = 0x3fff;
 
 struct WzpShape {

@@ -90,11 +90,13 @@ void dequantize_bricks(const uint16_t* mag, const uint32_t* sign, const BrickGeo
 
 // wzp.hip
 struct WzpScratch {
-    unsigned long long* desc;  // chunks lookback descriptors (zeroed)
-    uint32_t* excl;            // chunks exclusive payload offsets
+    unsigned long long* desc;  // groups lookback descriptors (zeroed)
+    uint8_t* staging;          // wzp_staging_bytes(chunks): one payload slot per chunk
     uint32_t* size;            // chunks payload sizes
     uint32_t* ticket;          // 1 counter (zeroed)
 };
+// Staging of the encoder: kChunkSlot bytes per chunk plus read-ahead padding.
+inline uint64_t wzp_staging_bytes(uint32_t chunks) { return uint64_t(chunks) * kChunkSlot + 64; }
 // Encodes mag/sign into out (WZP stream at out[0..]); payload capacity bound
 // `capacity` (bytes available from out). Total bytes -> *total (device u64).
 void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity,

@@ -5,14 +5,16 @@
 // Chunk payload = magnitude bytes (RZE over BIT_2 planes, or raw LE u16 when
 // RZE+2 >= raw size) then the signs of the nonzero magnitudes, LSB first.
 //
-// Encoder: one 256-thread workgroup (4 wave64s) per chunk, taken in ticket
-// order. Each thread owns 32 magnitudes = one sign word = 64 transformed
-// bytes = 8 bytes of the zero bitmap B0 = one byte of B1. All counts are
-// packed two per 32-bit word into block-wide scans (wave shuffles + LDS).
-// The payload offset comes from a decoupled lookback over chunk descriptors
-// (aggregate / inclusive flags, relaxed agent-scope atomics), so every chunk
-// writes its bytes straight to its final place in logical order; a tiny
-// second kernel fills the per-group directory from the chunk prefixes.
+// Encoder: one 256-thread workgroup (4 wave64s) per chunk. Each thread owns
+// 32 magnitudes = one sign word = 64 transformed bytes = 8 bytes of the zero
+// bitmap B0 = one byte of B1. All counts are packed two per 32-bit word into
+// block-wide scans (wave shuffles + LDS). Each chunk writes its meta, its size
+// and its payload into a 16-byte aligned staging slot and ends; no workgroup
+// waits for another. A second kernel, one workgroup per 16-chunk group taken
+// in ticket order, finds the group's payload offset by a decoupled lookback
+// over group descriptors (aggregate / inclusive flags, relaxed agent-scope
+// atomics), writes the group directory entry and moves the group's payloads
+// from their slots to their final place in logical order.
 //
 // Decoder: one workgroup per chunk; the chunk offset is the group offset plus
 // the sizes of the earlier chunks of its group (from their metas). RZE is
@@ -137,16 +139,13 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     __shared__ uint32_t signbuf[kThreads + 1];
     __shared__ uint32_t sh[8];
     __shared__ uint32_t n3s[4], b3s[4];
-    __shared__ uint32_t chunk_sh;
-    __shared__ unsigned long long excl_sh;
 
     const uint32_t t = threadIdx.x;
-    if (t == 0) chunk_sh = atomicAdd(w.ticket, 1u);
     signbuf[t] = 0;
     if (t < 16) csel[t] = compaction_selector(t);
     if (t == 0) signbuf[kThreads] = 0;
     __syncthreads();
-    const uint32_t c = chunk_sh;
+    const uint32_t c = blockIdx.x;
     const uint32_t E = chunk_elements(n, c);
     const uint32_t raw_bytes = 2 * E;
     const uint32_t first = t * 32;
@@ -248,7 +247,7 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     const uint32_t chunk_bytes = NNZ ? stored + sign_bytes : 0u;
     // the planes are in registers (tw) since the B0 barrier: clear the buffer
     // for the OR-assembled payload (a raw chunk re-reads its planes instead;
-    // the barrier after the lookback orders both before the payload writes)
+    // the barrier after step 5 orders both before the payload writes)
     if (!raw)
         for (uint32_t i = t; i < (kChunkBytes + kSignSlot) / 16 + 1; i += kThreads)
             reinterpret_cast<uint4*>(ob32)[i] = make_uint4(0, 0, 0, 0);
@@ -263,74 +262,19 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
         }
     }
 
-    // 5. decoupled lookback for this chunk's payload offset: wave 0 reads 64
-    // predecessor descriptors per round trip, stops at the nearest inclusive
-    // prefix (ballot) and sums the aggregates up to it (wave reduction).
-    // Descriptors are the only data passed between chunks, so relaxed
-    // agent-scope atomics suffice: a release here would write back this XCD's
-    // whole L2 (full of freshly written payload) on every publish.
-    if (t < 64) {
-        const uint32_t lane = t;
-        if (lane == 0)
-            __hip_atomic_store(&w.desc[c], (c == 0 ? kFlagInclusive : kFlagAggregate) | chunk_bytes, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        unsigned long long acc = 0;
-        if (c > 0) {
-            int64_t j = int64_t(c) - 1;
-            uint32_t spins = 0;
-            for (;;) {
-                const int64_t idx = j - int64_t(lane);
-                const unsigned long long d =
-                    idx >= 0 ? __hip_atomic_load(&w.desc[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                             : kFlagInclusive;  // nothing before chunk 0
-                const unsigned long long flag = d & ~kValueMask;
-                const unsigned long long incl = __ballot(flag == kFlagInclusive);
-                const unsigned long long waiting = __ballot(flag == 0);
-                const uint32_t stop = incl ? uint32_t(__ffsll((long long)incl) - 1) : 63u;
-                const unsigned long long need = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
-                if (waiting & need) {
-                    if (++spins > kSpinLimit) {
-                        if (lane == 0) atomicOr(status, uint32_t(kLookbackTimeout));
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                unsigned long long v = lane <= stop ? (d & kValueMask) : 0ull;
-#pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-                acc += v;
-                if (incl) break;
-                j -= 64;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&w.desc[c], kFlagInclusive | (acc + chunk_bytes), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (lane == 0) {
-            w.excl[c] = uint32_t(acc);
-            w.size[c] = chunk_bytes;
-            if (c == chunks - 1) *total = uint64_t(groups) * 4 + uint64_t(chunks) * 3 + acc + chunk_bytes;
-            excl_sh = acc;
-        }
-    }
-    __syncthreads();
-
-    // 6. payload at its final position
-    const uint64_t header = uint64_t(groups) * 4 + uint64_t(chunks) * 3;
-    const uint64_t start = header + excl_sh;
-    if (start + chunk_bytes > capacity) {
-        if (t == 0) atomicOr(status, uint32_t(kWzpCapacity));
-        return;
-    }
+    // 5. size and meta of the chunk (its payload offset is found by the group pass)
     if (t == 0) {
-        const uint32_t sign_code = sign_bytes == kSignSlot ? 0u : sign_bytes;
-        const uint32_t meta = NNZ ? ((raw ? kRawCode : stored) | (sign_code << 14)) : 0u;
-        uint8_t* mp = out + uint64_t(groups) * 4 + uint64_t(c) * 3;
-        mp[0] = uint8_t(meta);
-        mp[1] = uint8_t(meta >> 8);
-        mp[2] = uint8_t(meta >> 16);
+        w.size[c] = chunk_bytes;
+        if (uint64_t(groups) * 4 + uint64_t(c) * 3 + 3 <= capacity) {
+            const uint32_t sign_code = sign_bytes == kSignSlot ? 0u : sign_bytes;
+            const uint32_t meta = NNZ ? ((raw ? kRawCode : stored) | (sign_code << 14)) : 0u;
+            uint8_t* mp = out + uint64_t(groups) * 4 + uint64_t(c) * 3;
+            mp[0] = uint8_t(meta);
+            mp[1] = uint8_t(meta >> 8);
+            mp[2] = uint8_t(meta >> 16);
+        }
     }
+    __syncthreads();  // the payload buffer is cleared (above) before any thread ORs bytes into it
     if (!NNZ) return;
     // 7. assemble the chunk payload in LDS
     uint8_t* ob = reinterpret_cast<uint8_t*>(ob32);
@@ -407,34 +351,175 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
     for (uint32_t i = t; i < sign_bytes; i += kThreads) ob[stored + i] = uint8_t(signbuf[i >> 2] >> (8 * (i & 3u)));
     __syncthreads();
 
-    // 8. copy out: aligned 32-bit stores, byte stores only in the two edge words
-    const uintptr_t g0 = reinterpret_cast<uintptr_t>(out + start), g1 = g0 + chunk_bytes;
-    for (uintptr_t a = (g0 & ~uintptr_t(3)) + 4u * t; a < g1; a += 4u * kThreads) {
-        if (a >= g0 && a + 4 <= g1) {
-            const uint32_t o = uint32_t(a - g0);
-            const uint32_t lo = ob32[o >> 2], hi = ob32[(o >> 2) + 1];
-            *reinterpret_cast<uint32_t*>(a) = uint32_t(((uint64_t(hi) << 32) | lo) >> (8 * (o & 3u)));
-        } else {
-            for (uintptr_t b = a < g0 ? g0 : a; b < a + 4 && b < g1; ++b) *reinterpret_cast<uint8_t*>(b) = ob[b - g0];
-        }
-    }
+    // 8. the payload into the chunk's staging slot (16-byte aligned)
+    uint4* const slot = reinterpret_cast<uint4*>(w.staging + uint64_t(c) * kChunkSlot);
+    for (uint32_t q = t; q < (chunk_bytes + 15) / 16; q += kThreads) slot[q] = reinterpret_cast<const uint4*>(ob32)[q];
 }
 
-// Group directory: offset of the group's first payload byte, or 0 when the
-// whole group is empty (cpu/src/wzp_cpu.cpp:789-830).
-__global__ void wzp_directory_kernel(uint8_t* __restrict__ out, uint32_t chunks, uint32_t groups, WzpScratch w,
-                                     uint64_t capacity) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= groups || uint64_t(g) * 4 + 4 > capacity) return;
-    const uint32_t first = g * kGroup, last = min(chunks, first + kGroup);
-    uint32_t len = 0;
-    for (uint32_t c = first; c < last; ++c) len += w.size[c];
-    const uint32_t off = len ? w.excl[first] : 0u;
-    uint8_t* p = out + uint64_t(g) * 4;
-    p[0] = uint8_t(off);
-    p[1] = uint8_t(off >> 8);
-    p[2] = uint8_t(off >> 16);
-    p[3] = uint8_t(off >> 24);
+// Group pass (cpu/src/wzp_cpu.cpp:789-830 for the directory): workgroup =
+// one 16-chunk group, in ticket order. Wave 0 scans the group's chunk sizes
+// and publishes the group length; the whole workgroup looks back over the
+// earlier groups' descriptors (1024 per round trip, nearest inclusive prefix
+// by ballots, block reduction of the aggregates up to it). Descriptors are the only data passed
+// between workgroups, so relaxed agent-scope atomics suffice (the slots were
+// written by the previous kernel). Then the group's directory entry (offset
+// of its first payload byte, 0 for an empty group) and its payloads, moved
+// from the slots with 16-byte stores over funnel-shifted 16-byte loads.
+constexpr uint32_t kPackThreads = 1024, kPackUnroll = 4;
+__global__ void __launch_bounds__(kPackThreads) wzp_pack_kernel(uint8_t* __restrict__ out, uint64_t capacity, uint32_t chunks,
+                                                       uint32_t groups, WzpScratch w,
+                                                       unsigned long long* __restrict__ total,
+                                                       uint32_t* __restrict__ status) {
+    __shared__ uint32_t off_sh[kGroup + 1];  // chunk offsets inside the group; [kGroup] = group length
+    __shared__ uint32_t qpre[kGroup + 1];    // 16-byte destination groups before each chunk
+    __shared__ uint32_t g_sh, ok_sh, wait_sh;
+    __shared__ uint32_t near_w[kPackThreads / 64];        // per wave: nearest inclusive lane (or 64)
+    __shared__ unsigned long long part_w[kPackThreads / 64];
+    __shared__ unsigned long long excl_sh;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    if (t == 0) g_sh = atomicAdd(w.ticket, 1u);
+    __syncthreads();
+    const uint32_t g = g_sh, first = g * kGroup, count = min(kGroup, chunks - first);
+    const uint64_t header = uint64_t(groups) * 4 + uint64_t(chunks) * 3;
+    // the group's chunk offsets and length (wave 0), published at once
+    if (t < 64) {
+        const uint32_t size = lane < count ? w.size[first + lane] : 0u;
+        const uint32_t incl = wave_inclusive(size);
+        if (lane <= kGroup) off_sh[lane] = incl - size;  // lane kGroup: the group length
+        if (lane == 63)
+            __hip_atomic_store(&w.desc[g], (g == 0 ? kFlagInclusive : kFlagAggregate) | incl, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t glen = off_sh[kGroup];
+    // decoupled lookback with the whole workgroup: thread t reads the
+    // descriptor of group j - t, so one round trip covers kPackThreads groups;
+    // stop at the nearest inclusive prefix and sum the aggregates up to it
+    unsigned long long acc = 0;
+    if (g > 0) {
+        int64_t j = int64_t(g) - 1;
+        uint32_t spins = 0;
+        for (;;) {
+            const int64_t idx = j - int64_t(t);
+            const unsigned long long d =
+                idx >= 0 ? __hip_atomic_load(&w.desc[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : kFlagInclusive;  // nothing before group 0
+            const unsigned long long flag = d & ~kValueMask;
+            const unsigned long long im = __ballot(flag == kFlagInclusive);
+            if (lane == 0) near_w[wv] = im ? uint32_t(__ffsll((long long)im) - 1) : 64u;
+            if (t == 0) wait_sh = 0;
+            __syncthreads();
+            uint32_t stop = kPackThreads;  // block position of the nearest inclusive descriptor
+            for (uint32_t k = 0; k < kPackThreads / 64; ++k)
+                if (near_w[k] < 64) {
+                    stop = 64 * k + near_w[k];
+                    break;
+                }
+            if (t <= stop && flag == 0) wait_sh = 1;  // a descriptor we need is not published yet
+            unsigned long long v = t <= stop ? (d & kValueMask) : 0ull;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) part_w[wv] = v;
+            __syncthreads();
+            const bool wait = wait_sh != 0;
+            unsigned long long sum = 0;
+            for (uint32_t k = 0; k < kPackThreads / 64; ++k) sum += part_w[k];
+            __syncthreads();  // near_w / part_w / wait_sh are rewritten by the next round
+            if (wait) {
+                if (++spins > kSpinLimit) {
+                    if (t == 0) atomicOr(status, uint32_t(kLookbackTimeout));
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            acc += sum;
+            if (stop < kPackThreads) break;
+            j -= kPackThreads;
+        }
+        if (t == 0)
+            __hip_atomic_store(&w.desc[g], kFlagInclusive | (acc + glen), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (t == 0) {
+        excl_sh = acc;
+        const bool fits = header + acc + glen <= capacity;
+        ok_sh = fits;
+        if (!fits) atomicOr(status, uint32_t(kWzpCapacity));
+        if (g == groups - 1) *total = header + acc + glen;
+        if (uint64_t(g) * 4 + 4 <= capacity) {
+            const uint32_t off = glen ? uint32_t(acc) : 0u;
+            uint8_t* p = out + uint64_t(g) * 4;
+            p[0] = uint8_t(off);
+            p[1] = uint8_t(off >> 8);
+            p[2] = uint8_t(off >> 16);
+            p[3] = uint8_t(off >> 24);
+        }
+    }
+    if (t < 64) {  // destination 16-byte groups per chunk, then their prefix
+        const uint32_t size = lane < kGroup ? off_sh[lane + 1] - off_sh[lane] : 0u;
+        const uint64_t dst0 = reinterpret_cast<uintptr_t>(out) + header + acc + (lane < kGroup ? off_sh[lane] : 0u);
+        const uint32_t lead = uint32_t(dst0 & 15u);
+        const uint32_t nq = size ? (lead + size + 15) / 16 : 0u;
+        const uint32_t qi = wave_inclusive(nq);
+        if (lane <= kGroup) qpre[lane] = qi - nq;
+    }
+    __syncthreads();
+    if (!ok_sh) return;
+    // every destination group of every chunk of the group, kPackUnroll per
+    // thread per round with all their loads issued before any store
+    const uint32_t all = qpre[kGroup];
+    struct Unit {
+        uint8_t* dst;        // 16-byte aligned destination group
+        const uint8_t* src;  // the chunk's slot
+        int32_t o;           // payload byte at dst
+        uint32_t size;
+    };
+    auto unit = [&](uint32_t u) {
+        uint32_t i = 0;
+#pragma unroll
+        for (uint32_t k = 1; k < kGroup; ++k) i += u >= qpre[k] ? 1u : 0u;
+        uint8_t* const d = out + header + excl_sh + off_sh[i];
+        const uint32_t lead = uint32_t(reinterpret_cast<uintptr_t>(d) & 15u);
+        const uint32_t q = u - qpre[i];
+        return Unit{d - lead + 16 * q, w.staging + uint64_t(first + i) * kChunkSlot, int32_t(16 * q) - int32_t(lead),
+                    off_sh[i + 1] - off_sh[i]};
+    };
+    for (uint32_t u0 = t; u0 < all; u0 += kPackThreads * kPackUnroll) {
+        uint4 lo[kPackUnroll], hi[kPackUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kPackUnroll; ++r) {
+            const uint32_t u = u0 + r * kPackThreads;
+            if (u >= all) break;
+            const Unit x = unit(u);
+            if (x.o >= 0 && uint32_t(x.o) + 16 <= x.size) {  // the 16 bytes lie in two aligned 16-byte loads
+                lo[r] = *reinterpret_cast<const uint4*>(x.src + (uint32_t(x.o) & ~15u));
+                hi[r] = *reinterpret_cast<const uint4*>(x.src + (uint32_t(x.o) & ~15u) + 16);
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kPackUnroll; ++r) {
+            const uint32_t u = u0 + r * kPackThreads;
+            if (u >= all) break;
+            const Unit x = unit(u);
+            if (x.o >= 0 && uint32_t(x.o) + 16 <= x.size) {
+                // starting at word k of lo, byte sb
+                const uint32_t k = (uint32_t(x.o) >> 2) & 3u, sb = uint32_t(x.o) & 3u;
+                const uint32_t wv[8] = {lo[r].x, lo[r].y, lo[r].z, lo[r].w, hi[r].x, hi[r].y, hi[r].z, hi[r].w};
+                uint32_t v[5];
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    v[j] = k == 0 ? wv[j] : k == 1 ? wv[j + 1] : k == 2 ? wv[j + 2] : wv[j + 3];
+                *reinterpret_cast<uint4*>(x.dst) =
+                    make_uint4(__builtin_amdgcn_alignbyte(v[1], v[0], sb), __builtin_amdgcn_alignbyte(v[2], v[1], sb),
+                               __builtin_amdgcn_alignbyte(v[3], v[2], sb), __builtin_amdgcn_alignbyte(v[4], v[3], sb));
+            } else {  // an edge group: the chunk's bytes only
+                for (int j = 0; j < 16; ++j) {
+                    const int32_t p = x.o + j;
+                    if (p >= 0 && uint32_t(p) < x.size) x.dst[j] = x.src[p];
+                }
+            }
+        }
+    }
 }
 
 void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity, WzpScratch w,
@@ -443,8 +528,8 @@ void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* 
     if (ws.chunks == 0) return;
     hipLaunchKernelGGL(wzp_encode_kernel<MagSource>, dim3(ws.chunks), dim3(kThreads), 0, s, MagSource{mag, sign}, n,
                        ws.chunks, ws.groups, out, capacity, w, total, status);
-    hipLaunchKernelGGL(wzp_directory_kernel, dim3((ws.groups + 255) / 256), dim3(256), 0, s, out, ws.chunks,
-                       ws.groups, w, capacity);
+    hipLaunchKernelGGL(wzp_pack_kernel, dim3(ws.groups), dim3(kPackThreads), 0, s, out, capacity, ws.chunks,
+                       ws.groups, w, total, status);
 }
 
 template <class T>
@@ -456,8 +541,8 @@ void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_
     const BrickSource<T> src{coeff, d.nx, d.ny, T(quantum), T(1.0 / quantum), qo};
     hipLaunchKernelGGL(wzp_encode_kernel<BrickSource<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, src, n, ws.chunks,
                        ws.groups, out, capacity, w, total, status);
-    hipLaunchKernelGGL(wzp_directory_kernel, dim3((ws.groups + 255) / 256), dim3(256), 0, s, out, ws.chunks,
-                       ws.groups, w, capacity);
+    hipLaunchKernelGGL(wzp_pack_kernel, dim3(ws.groups), dim3(kPackThreads), 0, s, out, capacity, ws.chunks,
+                       ws.groups, w, total, status);
 }
 
 template <class Sink>
