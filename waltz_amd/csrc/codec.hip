@@ -227,10 +227,13 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     transform_forward<T>(input, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, status, s);
     SideList qo{p.qbm, p.qidx, p.qval, p.counters + 1, nq};
     WzpScratch wsc{p.desc, reinterpret_cast<uint8_t*>(p.rec), p.size, p.counters};
+    // with the fused repair emitter on brick pairs the finest level's high
+    // bands stay unquantized in coeff; that kernel requantizes them on load
+    const bool requant = fsegs && pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
     if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz)) {
         StageClock c("quantize_encode", s);
         quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,
-                                 status, s);
+                                 status, s, requant);
     } else {
         const BrickGeometry bg = brick_geometry(g.d, g.bz);
         const uint16_t* rank = w.ranks(g.d, g.bz, s);
@@ -243,7 +246,8 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     }
     // finest inverse level with native repair detection fused in when that
     // level runs as a fused kernel; otherwise reconstruct, then a repair pass
-    Level0Emit<T> emit{kEmitRepair, status, input, abs_bound, SegList{p.rbm, p.ridx, p.rerr, p.rcount, fsegs, fcap}};
+    Level0Emit<T> emit{kEmitRepair, status, input, abs_bound, SegList{p.rbm, p.ridx, p.rerr, p.rcount, fsegs, fcap},
+                       requant ? quantum : 0.0};
     SegList rp = emit.rep;
     if (!transform_inverse<T>(p.coeff, p.rec, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s, fsegs ? &emit : nullptr)) {
         rp = SegList{p.rbm, p.ridx, p.rerr, p.rcount, rtiles, kRepairTile};

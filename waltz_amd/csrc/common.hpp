@@ -122,6 +122,21 @@ WZ_D bool repair_decision(T rec, T orig, T native_bound, double bound, T* err, b
     return patch;
 }
 
+// Quantize one coefficient and dequantize it again (cpu/src/quantize_avx2.cpp:45-62,
+// cpu/src/codec.cpp:299-302): round to nearest even of v / quantum, magnitude
+// saturated at INT32_MAX, zero stays +0. Callers that get here have screened
+// the domain (finite, |v / quantum| < 2^63).
+// The integer round trip is the identity on the rounded value except for zero
+// (always +0) and magnitudes of 2^31 and more (T(INT32_MAX) with its sign), so
+// it is done with two selects.
+template <class T>
+WZ_D T requantize(T v, T q, T inv) {
+    const T rounded = __builtin_rint(v * inv);
+    const T sat = T(INT32_MAX);  // INT32_MAX in T: 2^31 for float, exact for double
+    const T m = __builtin_fabs(rounded) >= T(2147483648.0) ? __builtin_copysign(sat, rounded) : rounded;
+    return rounded == T(0) ? T(0) : m * q;
+}
+
 // Segmented side list: each producer workgroup appends to its own segment of
 // `cap` entries and records the count; a complete row-major flag bitmap lets
 // the archive assembly find every entry's index-order position.

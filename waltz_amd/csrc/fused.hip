@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 // and stores go through buffer windows at wave-uniform plane or row bases.
 // For the repair emitter each Y thread's 8 original samples are loaded as
 // the step begins.
-template <class T, int Emit>
+template <class T, int Emit, bool Requant = false>
 __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
     __shared__ T Zp[2 * FH * PA];                // two reconstructed z planes (split XY)
@@ -402,6 +402,19 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             ev = at(((low_lanes >> s) & 1u) ? q.le : q.he, eoff[s]);
         ov = buf_load<T>(window(q.ho), ooff[s]);
     };
+    // Values of slot s as the Z synthesis takes them (at use, so the loads
+    // stay in flight): high-band values of the repair emitter on unquantized
+    // bands are quantized and dequantized; LLL-box values never are.
+    auto band = [&](T v) {
+        if constexpr (Requant) return requantize<T>(v, a.q, a.inv);
+        else return v;
+    };
+    auto even_band = [&](int s, T v) {
+        if constexpr (!Requant) return v;
+        else if ((lo_wave >> s) & 1u) return v;
+        else if ((hi_wave >> s) & 1u) return band(v);
+        else return ((low_lanes >> s) & 1u) ? v : band(v);
+    };
     // Y item of this thread (t < 512): plane yp, column yx, y tile yt; a wave
     // is one (plane, tile) and its lanes are 64 consecutive x
     const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -453,8 +466,8 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         const int phase = ((g % H) + H) % H;
 #pragma unroll
         for (int s = 0; s < FSLOTS; ++s) {
-            const T oraw = vo[s];
-            const T es = ve[s] * C::ik, os = oraw * (-C::k);
+            const T oraw = band(vo[s]);
+            const T es = even_band(s, ve[s]) * C::ik, os = oraw * (-C::k);
             const T eA = fmadd(-C::d, OS[s] + os, es);   // eA[g]
             const T oB = fmadd(-C::g, EA[s] + eA, OS[s]); // oB[g-1]
             const T eC = fmadd(-C::b, OB[s] + oB, EA[s]); // eC[g-1]
@@ -641,7 +654,9 @@ void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
         !fused_plane_fits(a.opy, a.n.ny, sizeof(T)))
         throw std::logic_error("fused inverse level: plane exceeds 4 GiB");
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
-    if (a.emit == kEmitRepair)
+    if (a.emit == kEmitRepair && a.q > T(0))
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true>), grid, dim3(FT), 0, s, a);
+    else if (a.emit == kEmitRepair)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(FT), 0, s, a);
     else if (a.emit == kEmitStoreScreen)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(FT), 0, s, a);
@@ -654,9 +669,13 @@ uint32_t repair_z_segment(Dims n, uint32_t tuned) {
     // workgroups of the repair emitter one CU holds at once (registers, LDS)
     static const uint32_t per_cu = [] {
         int b = 0;
+        int b2 = 0;  // both repair variants: the smaller residency
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, inv_level_kernel<T, kEmitRepair>, FT, 0) != hipSuccess ||
-            b < 1)
-            b = 1;
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, inv_level_kernel<T, kEmitRepair, true>, FT, 0) !=
+                hipSuccess)
+            b = b2 = 1;
+        b = b < b2 ? b : b2;
+        if (b < 1) b = 1;
         return uint32_t(b);
     }();
     int dev = 0, cus = 0;

@@ -45,6 +45,7 @@ struct InvLevel {
     const T* orig;      // kEmitRepair: the original field
     double bound;
     float bound_lo;     // float repair emitter: the largest float <= bound
+    T q, inv;           // repair emitter, q > 0: high-band loads are quantized and dequantized
     SegList rep;        // kEmitRepair: one segment per workgroup, cap = 64*32*sz
 };
 

@@ -32,6 +32,8 @@ struct Level0Emit {
     const T* orig;
     double bound;
     SegList rep;       // segments/cap per level0_repair_segments()
+    double quantum;    // kEmitRepair: > 0 when the finest level's high bands hold the
+                       // unquantized coefficients (quantized and dequantized on load)
 };
 // Returns true when `emit` was carried out inside the fused finest level;
 // false means the caller must run the equivalent separate pass on `output`.
@@ -109,7 +111,8 @@ void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, ui
 // and dequantization fused into the decoder (writes the coefficient volume).
 template <class T>
 void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
-                           WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s);
+                           WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
+                           bool coarse_box_only = false);
 template <class T>
 void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double quantum, T* coeff, uint32_t* status,
                              hipStream_t s);

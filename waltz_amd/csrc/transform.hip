@@ -341,6 +341,10 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
             a.status = emit->status;
             a.orig = emit->orig;
             a.bound = emit->bound;
+            if (emit->mode == kEmitRepair && emit->quantum > 0) {
+                a.q = T(emit->quantum);
+                a.inv = T(1.0 / emit->quantum);
+            }
             a.bound_lo = float(emit->bound);
             if (double(a.bound_lo) > emit->bound) a.bound_lo = std::nextafter(a.bound_lo, 0.0f);
             a.rep = emit->rep;

@@ -75,6 +75,10 @@ struct BrickSource {
     uint32_t nx, ny;
     T q, inv;
     SideList qo;
+    // dequantized coefficients are written back inside [0,hx) x [0,hy) x
+    // [0,hz) only: the whole volume, or the coarser levels' box when the
+    // finest inverse level requantizes its high bands on load
+    uint32_t hx, hy, hz;
     WZ_D uint32_t load(uint32_t c, uint32_t t, uint32_t, uint32_t (&m)[32], uint32_t* status) const {
         uint32_t X, Y, Z;
         pair_subblock(nx, ny, c, t, X, Y, Z);
@@ -115,7 +119,7 @@ struct BrickSource {
                     }
                 }
             }
-            store4(coeff + at, d);
+            if (X < hx && Y + dy < hy && Z + dz < hz) store4(coeff + at, d);
         }
         if (bad) atomicOr(status, uint32_t(kQuantDomain));
         return sw;
@@ -359,11 +363,12 @@ __global__ void __launch_bounds__(256) wzp_encode_kernel(Source src, uint32_t n,
 // Group pass (cpu/src/wzp_cpu.cpp:789-830 for the directory): workgroup =
 // one 16-chunk group, in ticket order. Wave 0 scans the group's chunk sizes
 // and publishes the group length; the whole workgroup looks back over the
-// earlier groups' descriptors (1024 per round trip, nearest inclusive prefix
-// by ballots, block reduction of the aggregates up to it). Descriptors are the only data passed
-// between workgroups, so relaxed agent-scope atomics suffice (the slots were
-// written by the previous kernel). Then the group's directory entry (offset
-// of its first payload byte, 0 for an empty group) and its payloads, moved
+// earlier groups' descriptors (one per thread per round trip, nearest
+// inclusive prefix by ballots, block reduction of the aggregates up to it).
+// Descriptors are the only data passed between workgroups, so relaxed
+// agent-scope atomics suffice (the slots were written by the previous
+// kernel). Then the group's directory entry (offset of its first payload
+// byte, 0 for an empty group) and its payloads, moved
 // from the slots with 16-byte stores over funnel-shifted 16-byte loads.
 constexpr uint32_t kPackThreads = 1024, kPackUnroll = 4;
 __global__ void __launch_bounds__(kPackThreads) wzp_pack_kernel(uint8_t* __restrict__ out, uint64_t capacity, uint32_t chunks,
@@ -534,11 +539,20 @@ void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* 
 
 template <class T>
 void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
-                           WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s) {
+                           WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
+                           bool coarse_box_only) {
     if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
     const uint32_t n = d.nx * d.ny * d.nz;
     const WzpShape ws = wzp_shape(n);
-    const BrickSource<T> src{coeff, d.nx, d.ny, T(quantum), T(1.0 / quantum), qo};
+    const BrickSource<T> src{coeff,
+                             d.nx,
+                             d.ny,
+                             T(quantum),
+                             T(1.0 / quantum),
+                             qo,
+                             coarse_box_only ? half_len(d.nx) : d.nx,
+                             coarse_box_only ? half_len(d.ny) : d.ny,
+                             coarse_box_only ? half_len(d.nz) : d.nz};
     hipLaunchKernelGGL(wzp_encode_kernel<BrickSource<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, src, n, ws.chunks,
                        ws.groups, out, capacity, w, total, status);
     hipLaunchKernelGGL(wzp_pack_kernel, dim3(ws.groups), dim3(kPackThreads), 0, s, out, capacity, ws.chunks,
@@ -571,9 +585,9 @@ void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double q
 }
 
 template void quantize_encode_pairs<float>(float*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
-                                           unsigned long long*, uint32_t*, hipStream_t);
+                                           unsigned long long*, uint32_t*, hipStream_t, bool);
 template void quantize_encode_pairs<double>(double*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
-                                            unsigned long long*, uint32_t*, hipStream_t);
+                                            unsigned long long*, uint32_t*, hipStream_t, bool);
 template void decode_dequantize_pairs<float>(const uint8_t*, uint64_t, Dims, double, float*, uint32_t*,
                                              hipStream_t);
 template void decode_dequantize_pairs<double>(const uint8_t*, uint64_t, Dims, double, double*, uint32_t*,
