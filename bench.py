@@ -182,7 +182,7 @@ def measured_traffic(stage):
     try:
         for line in TRAFFIC_FILE.read_text().splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) >= 6 and cells[0] == f"`{stage}`":
+            if len(cells) >= 5 and cells[0] == f"`{stage}`":
                 return int(cells[-1])
     except (OSError, ValueError):
         pass
@@ -193,7 +193,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (each MI355X "
-              "compresses its own fields independently; see DESIGN.md for sharding)", file=sys.stderr)
+              "compresses its own fields independently)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
