@@ -298,6 +298,76 @@ uint64_t archive_size_from_header(const uint8_t* h, Dims d, unsigned width) {
     return archive_layout(wzp, np, pb, nq).end;
 }
 
+// One decode of a validated archive into `output`; returns the device status.
+// canonical: both side lists are taken as strictly increasing (the archives
+// compress writes), and a list found out of order comes back as kUnordered;
+// otherwise both lists are first checked for duplicate indices.
+template <class T>
+uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry& g, T* output, bool canonical,
+                     DeviceWorkspace& w, hipStream_t s) {
+    const uint32_t n = g.n;
+    const uint32_t bz = hdr[3];
+    uint64_t wzp = 0;
+    for (int k = 7; k >= 0; --k) wzp = (wzp << 8) | hdr[16 + k];
+    auto rd32 = [&](int o) {
+        return uint32_t(hdr[o]) | (uint32_t(hdr[o + 1]) << 8) | (uint32_t(hdr[o + 2]) << 16) |
+               (uint32_t(hdr[o + 3]) << 24);
+    };
+    const uint32_t np = rd32(24), nq = rd32(28);
+    double quantum;
+    const uint64_t qb = uint64_t(rd32(8)) | (uint64_t(rd32(12)) << 32);
+    std::memcpy(&quantum, &qb, 8);
+    const Layout L = archive_layout(wzp, np, uint64_t(np) * sizeof(T), nq);
+    const uint64_t words = (uint64_t(n) + 31) / 32;
+    Carve probe{nullptr};
+    probe.take<T>(n);
+    probe.take<T>(n);
+    probe.take<T>(n);
+    probe.take<uint16_t>(n);
+    probe.take<uint32_t>(words);
+    Carve c{static_cast<uint8_t*>(w.reserve(probe.used))};
+    T* coeff = c.take<T>(n);
+    T* t0 = c.take<T>(n);
+    T* t1 = c.take<T>(n);
+    uint16_t* mag = c.take<uint16_t>(n);
+    uint32_t* sign = c.take<uint32_t>(words);
+    uint32_t* status = &w.result->status;
+    zero_words(w.result, sizeof(EncodeResult) / 4, s);
+    if (!canonical) {  // duplicate indices fail (cpu/src/codec.cpp:260-275); `sign` is free until decode
+        check_unique(archive + L.qi, nq, n, sign, status, s);
+        check_unique(archive + L.pi, np, n, sign, status, s);
+    }
+    Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}, 0.0};
+    if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz)) {
+        StageClock ck("decode_dequantize", s);
+        decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s);
+    } else {
+        {
+            StageClock ck("wzp_decode", s);
+            wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
+        }
+        const BrickGeometry bg = brick_geometry(g.d, bz);
+        const uint16_t* rank = w.ranks(g.d, bz, s);
+        StageClock ck("dequantize", s);
+        dequantize_bricks<T>(mag, sign, bg, quantum, rank, coeff, s);
+    }
+    {
+        StageClock ck("qoutliers", s);
+        apply_qoutliers<T>(coeff, archive + L.qi, archive + L.qv, nq, n, quantum, status, s);
+    }
+    const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, g.dyadic, s, &screen);
+    {
+        StageClock ck("patches", s);
+        apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);  // screens patched samples
+        if (!screened) check_finite<T>(output, n, status, s);
+    }
+    hip_check(hipMemcpyAsync(w.host_result, w.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s),
+              "status copy");
+    hip_check(hipStreamSynchronize(s), "decompress");
+    hip_check(hipGetLastError(), "decompress launch");
+    return w.host_result->status;
+}
+
 template <class T>
 void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s) {
     if (reinterpret_cast<uintptr_t>(archive) % 8 || reinterpret_cast<uintptr_t>(output) % sizeof(T))
@@ -314,68 +384,18 @@ void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* outpu
     const uint64_t end = archive_size_from_header(hdr, d, sizeof(T));
     if (archive_bytes != ~uint64_t(0) && end != archive_bytes)
         throw std::runtime_error("truncated archive or trailing data");
-    const uint32_t n = g.n;
-    const uint32_t bz = hdr[3];
     const uint64_t wzp = uint64_t(hdr[16]) | (uint64_t(hdr[17]) << 8) | (uint64_t(hdr[18]) << 16) |
                          (uint64_t(hdr[19]) << 24) | (uint64_t(hdr[20]) << 32) | (uint64_t(hdr[21]) << 40) |
                          (uint64_t(hdr[22]) << 48) | (uint64_t(hdr[23]) << 56);
-    auto rd32 = [&](int o) {
-        return uint32_t(hdr[o]) | (uint32_t(hdr[o + 1]) << 8) | (uint32_t(hdr[o + 2]) << 16) |
-               (uint32_t(hdr[o + 3]) << 24);
-    };
-    const uint32_t np = rd32(24), nq = rd32(28);
-    double quantum;
-    const uint64_t qb = uint64_t(rd32(8)) | (uint64_t(rd32(12)) << 32);
-    std::memcpy(&quantum, &qb, 8);
-    const Layout L = archive_layout(wzp, np, uint64_t(np) * sizeof(T), nq);
-    const WzpShape ws = wzp_shape(n);
-    if (wzp < ws.header || wzp > ws.header + uint64_t(n) * 2 + (uint64_t(n) + 7) / 8)
+    const WzpShape ws = wzp_shape(g.n);
+    if (wzp < ws.header || wzp > ws.header + uint64_t(g.n) * 2 + (uint64_t(g.n) + 7) / 8)
         throw std::runtime_error("WZP archive size is outside its valid bounds");
-    const uint64_t words = (uint64_t(n) + 31) / 32;
-    Carve probe{nullptr};
-    probe.take<T>(n);
-    probe.take<T>(n);
-    probe.take<T>(n);
-    probe.take<uint16_t>(n);
-    probe.take<uint32_t>(words);
-    Carve c{static_cast<uint8_t*>(w.reserve(probe.used))};
-    T* coeff = c.take<T>(n);
-    T* t0 = c.take<T>(n);
-    T* t1 = c.take<T>(n);
-    uint16_t* mag = c.take<uint16_t>(n);
-    uint32_t* sign = c.take<uint32_t>(words);
-    uint32_t* status = &w.result->status;
-    zero_words(w.result, sizeof(EncodeResult) / 4, s);
-    if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz)) {
-        StageClock c("decode_dequantize", s);
-        decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s);
-    } else {
-        {
-            StageClock c("wzp_decode", s);
-            wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
-        }
-        const BrickGeometry bg = brick_geometry(g.d, bz);
-        const uint16_t* rank = w.ranks(g.d, bz, s);
-        StageClock c("dequantize", s);
-        dequantize_bricks<T>(mag, sign, bg, quantum, rank, coeff, s);
-    }
-    {
-        StageClock c("qoutliers", s);
-        apply_qoutliers<T>(coeff, archive + L.qi, archive + L.qv, nq, n, quantum, status, s);
-    }
-    Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}};
-    const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, g.dyadic, s, &screen);
-    {
-        StageClock c("patches", s);
-        apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);  // screens patched samples
-        if (!screened) check_finite<T>(output, n, status, s);
-    }
-    hip_check(hipMemcpyAsync(w.host_result, w.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s),
-              "status copy");
-    hip_check(hipStreamSynchronize(s), "decompress");
-    hip_check(hipGetLastError(), "decompress launch");
+    uint32_t st = decode_pass<T>(archive, hdr, g, output, true, w, s);
+    // a side list out of order (not written by compress, but valid when its
+    // indices are distinct): decode again with the duplicate checks
+    if (st & kUnordered) st = decode_pass<T>(archive, hdr, g, output, false, w, s) & ~uint32_t(kUnordered);
     profile_collect();
-    if (w.host_result->status) raise_status(w.host_result->status);
+    if (st) raise_status(st);
 }
 
 template uint64_t compress<float>(const float*, Dims, double, uint8_t*, uint64_t, hipStream_t);

@@ -195,6 +195,7 @@ enum Status : uint32_t {
     kCorrupt = 1u << 5,
     kNonFiniteOutput = 1u << 6,
     kSideCapacity = 1u << 7,
+    kUnordered = 1u << 8,  // a side list is not strictly increasing: decode again on the general path
 };
 
 }  // namespace wz

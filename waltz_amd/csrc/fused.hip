@@ -23,6 +23,8 @@
 #include "fused.hpp"
 #include "launch.hpp"
 
+#include <type_traits>
+
 namespace wz {
 namespace {
 
@@ -80,6 +82,13 @@ WZ_D void fwd_lift(T (&e)[P], T (&o)[P]) {
 WZ_D __amdgpu_buffer_rsrc_t window(const void* base) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
 }
+// The same with a byte range: loads beyond it read 0, stores beyond it are
+// dropped (how the kernels mask rows and lanes without branching, which keeps
+// every step's memory operations and so the memory-counter waits the same).
+WZ_D __amdgpu_buffer_rsrc_t window_n(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes), 0x00020000);
+}
+constexpr uint32_t kOutside = 0xFFFFFFF0u;  // a byte offset beyond every window_n range
 template <class T>
 WZ_D T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
 template <>
@@ -396,10 +405,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         return PairSrc{a.hi + uint64_t(ze) * a.hpz, a.hi + uint64_t(zo) * a.hpz, a.lo + uint64_t(ze) * a.lpz};
     };
     auto load_slot = [&](const PairSrc& q, int s, T& ev, T& ov) {
-        if (((lo_wave | hi_wave) >> s) & 1u)
-            ev = buf_load<T>(window(((lo_wave >> s) & 1u) ? q.le : q.he), eoff[s]);
-        else
-            ev = at(((low_lanes >> s) & 1u) ? q.le : q.he, eoff[s]);
+        ev = at(((low_lanes >> s) & 1u) ? q.le : q.he, eoff[s]);  // one load on every path
         ov = buf_load<T>(window(q.ho), ooff[s]);
     };
     // Values of slot s as the Z synthesis takes them (at use, so the loads
@@ -425,13 +431,17 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     // row Y of plane Z of a full-size field (wave-uniform) and this lane's byte offset in it
     auto row_of = [&](auto* base, uint32_t Z, uint32_t Y) { return base + uint64_t(Z) * a.opz + uint64_t(Y) * a.opy; };
     const uint32_t xoff = X * uint32_t(sizeof(T));
+    // status of all steps, reported once at the end
+    bool all_finite = true, bound_ok = true;
     // original samples of the Y item for step g (repair emitter)
     auto fetch_orig = [&](int32_t g, T (&ref)[8]) {
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const uint32_t Y = y0 + 8 * yt + j;
-            ref[j] = yown && xin && Z < n.nz && Y < n.ny ? buf_load<T>(window(row_of(a.orig, Z, Y)), xoff) : T(0);
+            const bool in = yown && Z < n.nz && Y < n.ny;  // wave-uniform; lanes x >= nx read 0
+            ref[j] = buf_load<T>(window_n(row_of(a.orig, in ? Z : 0u, in ? Y : 0u), in ? n.nx * uint32_t(sizeof(T)) : 0u),
+                                 xoff);
         }
     };
     // streaming state per slot: OS = os[g-1], EA = eA[g-1], OB = oB[g-2],
@@ -441,7 +451,6 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     for (int s = 0; s < FSLOTS; ++s) OS[s] = EA[s] = OB[s] = EC[s] = SF[s] = T(0);
 
     const int32_t g0 = int32_t(zs / 2) - 2, g1 = int32_t((zs + a.sz) / 2) + 1;
-    auto emits = [&](int32_t g) { return g - 2 >= int32_t(zs / 2); };
     // float repair emitter: a step's originals are loaded as the step begins,
     // ahead of the next pair's coefficients, so they arrive during the Z and X
     // passes and the in-order memory counter never makes a column wait for
@@ -457,8 +466,13 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
 #pragma unroll
         for (int s = 0; s < FSLOTS; ++s) load_slot(q, s, ve[s], vo[s]);
     }
-    for (int32_t g = g0; g <= g1; ++g) {
-        const bool emit = emits(g);
+    // One step per z pair. The warm-up steps (Z only), the emitting steps of
+    // the Y waves and those of the X-only wave run as separate loops, so
+    // each loop issues the same memory operations every iteration and the
+    // memory-counter waits stay exact (a merged loop makes a wave wait for
+    // its own previous stores before it starts the Z pass).
+    auto step = [&](int32_t g, auto emit_c, auto y_c) {
+        const bool emit = emit_c, yw = y_c;  // compile-time for float, run-time for double
         if constexpr (early) {
             if (emit) fetch_orig(g, ref);
         }
@@ -501,7 +515,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             // (the last step reloads its own pair, unused)
             load_slot(nq, s, ve[s], vo[s]);
         }
-        if (!emit) continue;
+        if (!emit) return;
         __syncthreads();
         // X (unfolded, so any tile width gives the same bits; 16 samples per
         // item keeps the 320 items in one round): plane p, row rr, tile xt
@@ -521,7 +535,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 });
         }
         __syncthreads();
-        if (!yown) continue;
+        if (!yw) return;
         // Y (folded, tile 8) of this thread's column -> 8 samples
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
         const bool xz = xin && Z < n.nz;
@@ -534,17 +548,16 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             bool fin = true;
             auto put = [&](uint32_t j, T v) {
                 const uint32_t Y = y0 + 8 * yt + j;
-                if (xz && Y < n.ny) {
-                    buf_store(v, window(row_of(a.out, Z, Y)), xoff);
-                    fin = fin && __builtin_isfinite(v);
-                }
+                const bool in = Z < n.nz && Y < n.ny;  // wave-uniform; lanes x >= nx are dropped
+                buf_store(v, window_n(row_of(a.out, in ? Z : 0u, in ? Y : 0u), in ? n.nx * uint32_t(sizeof(T)) : 0u),
+                          xoff);
+                fin = fin && (!(xin && in) || __builtin_isfinite(v));
             };
             inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
                 put(2 * (j - 2), e);
                 put(2 * (j - 2) + 1, o);
             });
-            if (Emit == kEmitStoreScreen && __ballot(!fin) && (t & 63u) == 0)
-                atomicOr(a.status, uint32_t(kNonFiniteOutput));
+            all_finite = all_finite && fin;
         } else {
             if constexpr (!early) fetch_orig(g, ref);
             // decide each sample as the lifting finishes it; then the step's
@@ -602,36 +615,63 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 for (int r = 0; r < 8; ++r)
                     if ((lane >> 1) == uint32_t(r)) wv = (lane & 1u) ? uint32_t(m8[r] >> 32) : uint32_t(m8[r]);
                 const uint32_t Y = y0 + 8 * yt + (lane >> 1);
-                if (lane < 16 && Z < n.nz && Y < n.ny && ((lane & 1u) == 0 || x0 + 32 < n.nx))
-                    a.rep.bitmap[((row0 - yx + (lane >> 1) * n.nx) >> 5) + (lane & 1u)] = wv;
+                const bool put = lane < 16 && Z < n.nz && Y < n.ny && ((lane & 1u) == 0 || x0 + 32 < n.nx);
+                const uint32_t words = uint32_t((uint64_t(n.nx) * n.ny * n.nz + 31) / 32);
+                __builtin_amdgcn_raw_buffer_store_b32(
+                    wv, window_n(a.rep.bitmap, 4 * words),
+                    int(put ? 4 * (((row0 - yx + (lane >> 1) * n.nx) >> 5) + (lane & 1u)) : kOutside), 0, 0);
             }
             uint32_t total = 0;
 #pragma unroll
             for (int r = 0; r < 8; ++r) total += uint32_t(__popcll(m8[r]));
-            if (total) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&seg_fill, total);
-                base = __builtin_amdgcn_readfirstlane(base);
-                // this step's entries start at a wave-uniform slot of the segment
-                const uint64_t first = uint64_t(wg) * a.rep.cap + base;
-                const auto wi = window(a.rep.index + first);
-                const auto wp = window(static_cast<T*>(a.rep.payload) + first);
-                uint32_t before = 0;  // entries of the earlier rows
+            uint32_t base = 0;
+            if (total && lane == 0) base = atomicAdd(&seg_fill, total);
+            base = __builtin_amdgcn_readfirstlane(base);
+            // this step's entries start at a wave-uniform slot of the segment;
+            // lanes without an entry store beyond the windows (dropped)
+            const uint64_t first = uint64_t(wg) * a.rep.cap + base;
+            const uint32_t room = base < a.rep.cap ? a.rep.cap - base : 0u;
+            const auto wi = window_n(a.rep.index + first, 4 * room);
+            const auto wp = window_n(static_cast<T*>(a.rep.payload) + first, uint32_t(sizeof(T)) * room);
+            uint32_t before = 0;  // entries of the earlier rows
 #pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const unsigned long long m = m8[r];
-                    if ((pbits >> r) & 1u) {
-                        // entries of lower lanes: mbcnt counts the mask bits below this lane
-                        const uint32_t k = before + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
-                                                                              __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-                        __builtin_amdgcn_raw_buffer_store_b32(row0 + uint32_t(r) * n.nx, wi, int(4 * k), 0, 0);
-                        buf_store(err8[r], wp, uint32_t(sizeof(T)) * k);
-                    }
-                    before += uint32_t(__popcll(m));
-                }
+            for (int r = 0; r < 8; ++r) {
+                const unsigned long long m = m8[r];
+                // entries of lower lanes: mbcnt counts the mask bits below this lane
+                const uint32_t k = before + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                const bool mine = (pbits >> r) & 1u;
+                __builtin_amdgcn_raw_buffer_store_b32(row0 + uint32_t(r) * n.nx, wi, int(mine ? 4 * k : kOutside), 0, 0);
+                buf_store(err8[r], wp, mine ? uint32_t(sizeof(T)) * k : kOutside);
+                before += uint32_t(__popcll(m));
             }
-            if (__ballot(!ok) && lane == 0) atomicOr(a.status, uint32_t(kRepairBound));
+            bound_ok = bound_ok && ok;
         }
+    };
+    const int32_t gE = int32_t(zs / 2) + 2;  // first step that emits
+    int32_t g = g0;
+    if constexpr (sizeof(T) == 4) {
+        if (yown) {
+            for (; g < gE; ++g) step(g, std::false_type{}, std::true_type{});
+            // as many dropped stores as an emitting step issues, so the
+            // emitting loop is entered with the memory-counter state it keeps
+            // (the counter tracks stores too) and its waits stay exact
+            constexpr int kStores = Emit == kEmitRepair ? 17 : 8;
+            const auto none = window_n(a.out ? static_cast<const void*>(a.out) : a.hi, 0u);
+#pragma unroll
+            for (int k = 0; k < kStores; ++k) __builtin_amdgcn_raw_buffer_store_b32(0u, none, int(kOutside), 0, 0);
+            for (; g <= g1; ++g) step(g, std::true_type{}, std::true_type{});
+        } else {
+            for (; g < gE; ++g) step(g, std::false_type{}, std::false_type{});
+            for (; g <= g1; ++g) step(g, std::true_type{}, std::false_type{});
+        }
+    } else {  // double: one loop (three copies would not fit the registers)
+        for (; g <= g1; ++g) step(g, g >= gE, yown);
+    }
+    {
+        const bool lead = (t & 63u) == 0;
+        if (Emit == kEmitStoreScreen && __ballot(!all_finite) && lead) atomicOr(a.status, uint32_t(kNonFiniteOutput));
+        if (Emit == kEmitRepair && __ballot(!bound_ok) && lead) atomicOr(a.status, uint32_t(kRepairBound));
     }
     if constexpr (Emit == kEmitRepair) {
         __syncthreads();

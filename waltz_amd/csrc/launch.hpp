@@ -50,6 +50,7 @@ void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsig
 void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width, uint32_t* segments,
                             uint32_t* cap);
 
+
 // Bit-neutral launch choices of the dyadic schedule for one geometry, type
 // and direction: leading levels run as fused kernels, and their z segments
 // (positive multiples of 16).
@@ -141,6 +142,9 @@ void apply_qoutliers(T* coeff, const uint8_t* idx, const uint8_t* val, uint32_t 
 template <class T>
 void apply_repairs(T* out, const uint8_t* idx, const uint8_t* err, uint32_t count, uint32_t n, uint32_t* status,
                    hipStream_t s);
+// Flags kCorrupt when an index of the list repeats (or is >= n); `bitmap`:
+// ceil(n/32) words, cleared here.
+void check_unique(const uint8_t* idx, uint32_t count, uint32_t n, uint32_t* bitmap, uint32_t* status, hipStream_t s);
 // Fills `words` 32-bit words with zero (a kernel, so calls stay graph-capturable).
 void zero_words(void* p, uint64_t words, hipStream_t s);
 

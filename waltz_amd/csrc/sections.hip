@@ -351,6 +351,7 @@ __global__ void qpatch_kernel(T* __restrict__ c, const uint32_t* __restrict__ id
         atomicOr(status, uint32_t(kCorrupt));
         return;
     }
+    if (j && idx[j - 1] >= i) atomicOr(status, uint32_t(kUnordered));  // duplicates are checked apart
     const T x = T(v) * q;  // cpu/src/codec.cpp:299-302
     c[i] = c[i] < T(0) ? -x : x;
 }
@@ -366,9 +367,24 @@ __global__ void rpatch_kernel(T* __restrict__ out, const uint32_t* __restrict__ 
         atomicOr(status, uint32_t(kCorrupt));
         return;
     }
+    if (j && idx[j - 1] >= i) atomicOr(status, uint32_t(kUnordered));
     const T v = out[i] - e;  // cpu/src/codec.cpp:330
     out[i] = v;
     if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
+}
+
+// Duplicate detection of an unordered side list (cpu/src/codec.cpp:260-275).
+__global__ void __launch_bounds__(256) unique_kernel(const uint32_t* __restrict__ idx, uint32_t count, uint32_t n,
+                                                     uint32_t* __restrict__ bitmap, uint32_t* __restrict__ status) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= count) return;
+    const uint32_t i = idx[j];
+    if (i >= n) {
+        atomicOr(status, uint32_t(kCorrupt));
+        return;
+    }
+    const uint32_t bit = 1u << (i & 31u);
+    if (atomicOr(&bitmap[i >> 5], bit) & bit) atomicOr(status, uint32_t(kCorrupt));
 }
 
 template <class T>
@@ -393,6 +409,13 @@ void apply_repairs(T* out, const uint8_t* idx, const uint8_t* err, uint32_t coun
     if (!count) return;
     hipLaunchKernelGGL((rpatch_kernel<T>), dim3((count + 255) / 256), dim3(256), 0, s, out,
                        reinterpret_cast<const uint32_t*>(idx), reinterpret_cast<const T*>(err), count, n, status);
+}
+
+void check_unique(const uint8_t* idx, uint32_t count, uint32_t n, uint32_t* bitmap, uint32_t* status, hipStream_t s) {
+    if (!count) return;
+    zero_words(bitmap, (uint64_t(n) + 31) / 32, s);
+    hipLaunchKernelGGL(unique_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                       reinterpret_cast<const uint32_t*>(idx), count, n, bitmap, status);
 }
 
 template <class T>
