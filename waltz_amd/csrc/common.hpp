@@ -91,6 +91,15 @@ template <class T> struct Lift {
     static constexpr T ik = T(1) / T(1.1496043988602418);
 };
 
+// Per-type |x|, round-to-nearest-even and copysign (the plain builtins take
+// and return double, which would widen float work to f64 instructions).
+WZ_HD float abs_t(float x) { return __builtin_fabsf(x); }
+WZ_HD double abs_t(double x) { return __builtin_fabs(x); }
+WZ_HD float rint_t(float x) { return __builtin_rintf(x); }
+WZ_HD double rint_t(double x) { return __builtin_rint(x); }
+WZ_HD float copysign_t(float m, float s) { return __builtin_copysignf(m, s); }
+WZ_HD double copysign_t(double m, double s) { return __builtin_copysign(m, s); }
+
 WZ_HD float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 WZ_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -113,7 +122,7 @@ WZ_D bool repair_decision(T rec, T orig, T native_bound, double bound, T* err, b
     const T e = value - orig;
     bool patch = false;
     if (!__builtin_isfinite(value) || !__builtin_isfinite(e)) *ok = false;
-    if (__builtin_fabs(e) > native_bound) {
+    if (abs_t(e) > native_bound) {
         patch = true;
         value = value - e;
     }
@@ -131,9 +140,9 @@ WZ_D bool repair_decision(T rec, T orig, T native_bound, double bound, T* err, b
 // it is done with two selects.
 template <class T>
 WZ_D T requantize(T v, T q, T inv) {
-    const T rounded = __builtin_rint(v * inv);
+    const T rounded = rint_t(v * inv);
     const T sat = T(INT32_MAX);  // INT32_MAX in T: 2^31 for float, exact for double
-    const T m = __builtin_fabs(rounded) >= T(2147483648.0) ? __builtin_copysign(sat, rounded) : rounded;
+    const T m = abs_t(rounded) >= T(2147483648.0) ? copysign_t(sat, rounded) : rounded;
     return rounded == T(0) ? T(0) : m * q;
 }
 

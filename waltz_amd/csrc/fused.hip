@@ -583,9 +583,9 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                     // every operand was finite. Other lanes (ties at the
                     // bound, non-finite samples) take the full test.
                     const T e = v - orig;
-                    const bool p = __builtin_fabs(e) > nb;
+                    const bool p = abs_t(e) > nb;
                     const T d = p ? (v - e) - orig : e;
-                    const bool fast = __builtin_fabs(d) < a.bound_lo;
+                    const bool fast = abs_t(d) < T(a.bound_lo);
                     if (__ballot(inb && !fast)) {
                         if (inb) patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
                     } else {

@@ -82,10 +82,10 @@ __global__ void __launch_bounds__(256) quantize_kernel(T* __restrict__ coeff, Br
         const uint32_t r = rank[k.shape * 8192 + i];
         const T scaled = coeff[at] * inv;
         // cpu/src/quantize_avx2.cpp:45-62: finite, |scaled| < 2^63, llrint, INT32 saturation.
-        if (!__builtin_isfinite(scaled) || !(__builtin_fabs(scaled) < T(9223372036854775808.0))) bad = true;
-        const T rounded = __builtin_rint(scaled);
+        if (!__builtin_isfinite(scaled) || !(abs_t(scaled) < T(9223372036854775808.0))) bad = true;
+        const T rounded = rint_t(scaled);
         const bool negative = rounded < T(0);
-        const T am = __builtin_fabs(rounded);
+        const T am = abs_t(rounded);
         const int32_t m = am >= T(2147483648.0) ? INT32_MAX : int32_t(am);
         coeff[at] = m ? T(negative ? -m : m) * q : T(0);
         mag[k.base + r] = uint16_t(m > 65535 ? 65535 : m);

@@ -100,10 +100,10 @@ struct BrickSource {
             for (uint32_t dx = 0; dx < 4; ++dx) {
                 // cpu/src/quantize_avx2.cpp:45-62: finite, |scaled| < 2^63, llrint, INT32 saturation
                 const T scaled = v[r][dx] * inv;
-                if (!__builtin_isfinite(scaled) || !(__builtin_fabs(scaled) < T(9223372036854775808.0))) bad = true;
-                const T rounded = __builtin_rint(scaled);
+                if (!__builtin_isfinite(scaled) || !(abs_t(scaled) < T(9223372036854775808.0))) bad = true;
+                const T rounded = rint_t(scaled);
                 const bool negative = rounded < T(0);
-                const T am = __builtin_fabs(rounded);
+                const T am = abs_t(rounded);
                 const int32_t mv = am >= T(2147483648.0) ? INT32_MAX : int32_t(am);
                 d[dx] = mv ? T(negative ? -mv : mv) * q : T(0);
                 const uint32_t i = subblock_rank(dx, dy, dz);
