@@ -33,12 +33,12 @@ enum InvEmit : int {
 template <class T>
 struct InvLevel {
     const T* hi;        // coefficient volume (high bands of this level)
-    uint64_t hpy, hpz;
+    uint32_t hpy, hpz;  // pitches in elements (planes below 4 GiB)
     const T* lo;        // reconstructed LLL box of the coarser level
-    uint64_t lpy, lpz;
+    uint32_t lpy, lpz;
     Dims n;             // level box extents
     T* out;             // reconstructed level box
-    uint64_t opy, opz;  // (also the pitches of `orig`)
+    uint32_t opy, opz;  // (also the pitches of `orig`)
     uint32_t sz;        // z segment length (multiple of 16)
     int emit;           // InvEmit
     uint32_t* status;   // screen / strict-bound failures
