@@ -95,31 +95,41 @@ struct BrickSource {
         for (uint32_t r = 0; r < 8; ++r) {
             const uint32_t dy = r & 3u, dz = r >> 2;
             const uint64_t at = (uint64_t(Z + dz) * ny + (Y + dy)) * nx + X;
-            T d[4];
+            uint32_t mv[4], over = 0;
 #pragma unroll
             for (uint32_t dx = 0; dx < 4; ++dx) {
-                // cpu/src/quantize_avx2.cpp:45-62: finite, |scaled| < 2^63, llrint, INT32 saturation
+                // cpu/src/quantize_avx2.cpp:45-62: finite and |scaled| < 2^63
+                // (one compare: false for NaN and infinities), llrint, INT32
+                // saturation
                 const T scaled = v[r][dx] * inv;
-                if (!__builtin_isfinite(scaled) || !(abs_t(scaled) < T(9223372036854775808.0))) bad = true;
+                bad = bad || !(abs_t(scaled) < T(9223372036854775808.0));
                 const T rounded = rint_t(scaled);
-                const bool negative = rounded < T(0);
                 const T am = abs_t(rounded);
-                const int32_t mv = am >= T(2147483648.0) ? INT32_MAX : int32_t(am);
-                d[dx] = mv ? T(negative ? -mv : mv) * q : T(0);
+                mv[dx] = am >= T(2147483648.0) ? uint32_t(INT32_MAX) : uint32_t(int32_t(am));
                 const uint32_t i = subblock_rank(dx, dy, dz);
-                m[i] = uint32_t(mv > 65535 ? 65535 : mv);
-                sw |= uint32_t(negative && mv) << i;
-                if (mv > 65535) {
+                m[i] = mv[dx] > 65535u ? 65535u : mv[dx];
+                sw |= uint32_t(rounded < T(0)) << i;
+                over |= uint32_t(mv[dx] > 65535u) << dx;
+            }
+            if (over) {  // q-outliers (rare): one branch per row
+#pragma unroll
+                for (uint32_t dx = 0; dx < 4; ++dx) {
+                    if (!((over >> dx) & 1u)) continue;
                     const uint64_t g = at + dx;
                     atomicOr(&qo.bitmap[g >> 5], 1u << (g & 31u));
                     const uint32_t slot = atomicAdd(qo.count, 1u);
                     if (slot < qo.capacity) {
                         qo.index[slot] = uint32_t(g);
-                        static_cast<uint32_t*>(qo.payload)[slot] = uint32_t(mv);
+                        static_cast<uint32_t*>(qo.payload)[slot] = mv[dx];
                     }
                 }
             }
-            if (X < hx && Y + dy < hy && Z + dz < hz) store4(coeff + at, d);
+            if (X < hx && Y + dy < hy && Z + dz < hz) {  // dequantized write-back inside the box
+                T d[4];
+#pragma unroll
+                for (uint32_t dx = 0; dx < 4; ++dx) d[dx] = requantize<T>(v[r][dx], q, inv);
+                store4(coeff + at, d);
+            }
         }
         if (bad) atomicOr(status, uint32_t(kQuantDomain));
         return sw;
