@@ -105,13 +105,13 @@ WZ_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c)
 
 // Exact |value - original| <= bound with a TwoDiff tie check
 // (cpu/src/repair_avx2.cpp:48-56).
+// Branch-free: the tie check's TwoDiff is computed either way.
 WZ_D bool within_bound_exact(double value, double original, double bound) {
     const double high = value - original, magnitude = __builtin_fabs(high);
-    if (!__builtin_isfinite(high) || magnitude > bound) return false;
-    if (magnitude < bound) return true;
     const double bv = value - high, av = high + bv;
     const double br = bv - original, ar = value - av, low = ar + br;
-    return !((high > 0 && low > 0) || (high < 0 && low < 0));
+    const bool tie_ok = !((high > 0 && low > 0) || (high < 0 && low < 0));
+    return __builtin_isfinite(high) && !(magnitude > bound) && (magnitude < bound || tie_ok);
 }
 
 // One native repair decision (cpu/src/repair_avx2.cpp:58-74): returns whether

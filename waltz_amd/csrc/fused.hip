@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 // and stores go through buffer windows at wave-uniform plane or row bases.
 // For the repair emitter each Y thread's 8 original samples are loaded as
 // the step begins.
-template <class T, int Emit, bool Requant = false>
+template <class T, int Emit, bool Requant = false, bool RowWords = true>
 __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
     __shared__ T Zp[2 * FH * PA];                // two reconstructed z planes (split XY)
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     const uint32_t wg = tl.id;
     const Dims n = a.n;
     const uint32_t hx = half_len(n.nx), hy = half_len(n.ny), hz = half_len(n.nz);
-    const bool rowwords = (n.nx & 31u) == 0;  // bitmap words never straddle rows
+    // RowWords (repair emitter): nx % 32 == 0, so bitmap words never straddle rows
     if (t == 0) seg_fill = 0;
 
     // Footprint positions (row r, column c of the 72 x 40 window) in slot
@@ -430,19 +430,22 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     const bool xin = X < n.nx;
     // row Y of plane Z of a full-size field (wave-uniform) and this lane's byte offset in it
     auto row_of = [&](auto* base, uint32_t Z, uint32_t Y) { return base + uint64_t(Z) * a.opz + uint64_t(Y) * a.opy; };
-    const uint32_t xoff = X * uint32_t(sizeof(T));
     // status of all steps, reported once at the end
     bool all_finite = true, bound_ok = true;
     // original samples of the Y item for step g (repair emitter)
+    // one window over plane Z (wave-uniform; none past the field) and per-lane
+    // offsets, beyond the window for samples outside the field
+    const uint32_t plane_bytes = n.ny * a.opy * uint32_t(sizeof(T));
+    auto plane_window = [&](const T* base, uint32_t Z) {
+        const bool in = Z < n.nz;
+        return window_n(base + uint64_t(in ? Z : 0u) * a.opz, in ? plane_bytes : 0u);
+    };
+    auto lane_off = [&](uint32_t Y) { return xin && Y < n.ny ? (Y * a.opy + X) * uint32_t(sizeof(T)) : kOutside; };
     auto fetch_orig = [&](int32_t g, T (&ref)[8]) {
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
+        const auto w = plane_window(a.orig, yown ? Z : n.nz);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t Y = y0 + 8 * yt + j;
-            const bool in = yown && Z < n.nz && Y < n.ny;  // wave-uniform; lanes x >= nx read 0
-            ref[j] = buf_load<T>(window_n(row_of(a.orig, in ? Z : 0u, in ? Y : 0u), in ? n.nx * uint32_t(sizeof(T)) : 0u),
-                                 xoff);
-        }
+        for (int j = 0; j < 8; ++j) ref[j] = buf_load<T>(w, lane_off(y0 + 8 * yt + j));
     };
     // streaming state per slot: OS = os[g-1], EA = eA[g-1], OB = oB[g-2],
     // EC = eC[g-2]; SF rotates through o_raw[B-2], eA'[B-1], oB'[B-1], eC'[B]
@@ -546,12 +549,11 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         };
         if constexpr (Emit != kEmitRepair) {
             bool fin = true;
+            const auto wout = plane_window(a.out, Z);
             auto put = [&](uint32_t j, T v) {
                 const uint32_t Y = y0 + 8 * yt + j;
-                const bool in = Z < n.nz && Y < n.ny;  // wave-uniform; lanes x >= nx are dropped
-                buf_store(v, window_n(row_of(a.out, in ? Z : 0u, in ? Y : 0u), in ? n.nx * uint32_t(sizeof(T)) : 0u),
-                          xoff);
-                fin = fin && (!(xin && in) || __builtin_isfinite(v));
+                buf_store(v, wout, lane_off(Y));
+                fin = fin && (!(xz && Y < n.ny) || __builtin_isfinite(v));
             };
             inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
                 put(2 * (j - 2), e);
@@ -582,15 +584,22 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                     // (round to nearest is monotonic), which also implies
                     // every operand was finite. Other lanes (ties at the
                     // bound, non-finite samples) take the full test.
+                    // The full test gives the fast lanes the same result, so a
+                    // wave with any slow lane runs it on all lanes (one
+                    // wave-uniform branch, no per-lane one).
                     const T e = v - orig;
                     const bool p = abs_t(e) > nb;
                     const T d = p ? (v - e) - orig : e;
                     const bool fast = abs_t(d) < T(a.bound_lo);
+                    patch = inb && p;
+                    err = inb ? e : T(0);
                     if (__ballot(inb && !fast)) {
-                        if (inb) patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
-                    } else {
-                        patch = inb && p;
-                        err = inb ? e : T(0);
+                        bool okj = true;
+                        T ej = T(0);
+                        const bool pj = repair_decision<T>(v, orig, nb, a.bound, &ej, &okj);
+                        patch = inb && pj;
+                        err = inb ? ej : T(0);
+                        ok = ok && (!inb || okj);
                     }
                 } else if (inb) {
                     patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
@@ -598,10 +607,6 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 m8[j] = __ballot(patch);
                 pbits |= uint32_t(patch) << j;
                 err8[j] = err;
-                if (!rowwords && patch) {
-                    const uint64_t at = (uint64_t(Z) * n.ny + Y) * n.nx + X;
-                    atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
-                }
             };
             inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
                 decide(2 * (j - 2), e, ref[2 * (j - 2)]);
@@ -609,7 +614,14 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             });
             // sample index of this lane in row y0 + 8 yt of plane Z (indices fit 32 bits)
             const uint32_t row0 = uint32_t((uint64_t(Z) * n.ny + y0 + 8 * yt) * n.nx) + X;
-            if (rowwords) {  // 64 consecutive x at a 32-aligned row start: each row owns two words
+            if constexpr (!RowWords) {  // bitmap words straddle rows: one atomic per patched sample
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if ((pbits >> r) & 1u) {
+                        const uint32_t at = row0 + uint32_t(r) * n.nx;
+                        atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
+                    }
+            } else {  // 64 consecutive x at a 32-aligned row start: each row owns two words
                 uint32_t wv = 0;  // lane 2r + h holds word h of row r
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
@@ -694,10 +706,15 @@ void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
         !fused_plane_fits(a.opy, a.n.ny, sizeof(T)))
         throw std::logic_error("fused inverse level: plane exceeds 4 GiB");
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
-    if (a.emit == kEmitRepair && a.q > T(0))
+    const bool words = (a.n.nx & 31u) == 0;
+    if (a.emit == kEmitRepair && a.q > T(0) && words)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true>), grid, dim3(FT), 0, s, a);
-    else if (a.emit == kEmitRepair)
+    else if (a.emit == kEmitRepair && a.q > T(0))
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true, false>), grid, dim3(FT), 0, s, a);
+    else if (a.emit == kEmitRepair && words)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(FT), 0, s, a);
+    else if (a.emit == kEmitRepair)
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, false, false>), grid, dim3(FT), 0, s, a);
     else if (a.emit == kEmitStoreScreen)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(FT), 0, s, a);
     else

@@ -19,9 +19,10 @@ struct FwdLevel {
 };
 
 // Whether `rows` rows of pitch `pitch` elements of `width` bytes stay below
-// 4 GiB (the inverse level kernel's 32-bit byte offsets within a plane).
+// 4 GiB less 1 MiB (the level kernels' 32-bit byte offsets within a plane,
+// with room above the plane for the offset that masks a lane off).
 inline bool fused_plane_fits(uint64_t pitch, uint64_t rows, uint64_t width) {
-    return pitch * rows * width < (uint64_t(1) << 32);
+    return pitch * rows * width <= (uint64_t(1) << 32) - (uint64_t(1) << 20);
 }
 
 enum InvEmit : int {
