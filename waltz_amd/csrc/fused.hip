@@ -441,6 +441,10 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         return window_n(base + uint64_t(in ? Z : 0u) * a.opz, in ? plane_bytes : 0u);
     };
     auto lane_off = [&](uint32_t Y) { return xin && Y < n.ny ? (Y * a.opy + X) * uint32_t(sizeof(T)) : kOutside; };
+    // repair emitter: this workgroup's record segment (cap entries)
+    const auto seg_index = window_n(a.rep.index + uint64_t(wg) * a.rep.cap, 4 * a.rep.cap);
+    const auto seg_payload =
+        window_n(static_cast<T*>(a.rep.payload) + uint64_t(wg) * a.rep.cap, uint32_t(sizeof(T)) * a.rep.cap);
     auto fetch_orig = [&](int32_t g, T (&ref)[8]) {
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
         const auto w = plane_window(a.orig, yown ? Z : n.nz);
@@ -641,11 +645,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             base = __builtin_amdgcn_readfirstlane(base);
             // this step's entries start at a wave-uniform slot of the segment;
             // lanes without an entry store beyond the windows (dropped)
-            const uint64_t first = uint64_t(wg) * a.rep.cap + base;
-            const uint32_t room = base < a.rep.cap ? a.rep.cap - base : 0u;
-            const auto wi = window_n(a.rep.index + first, 4 * room);
-            const auto wp = window_n(static_cast<T*>(a.rep.payload) + first, uint32_t(sizeof(T)) * room);
-            uint32_t before = 0;  // entries of the earlier rows
+            uint32_t before = base;  // entries of the earlier steps and rows
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 const unsigned long long m = m8[r];
@@ -653,8 +653,8 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
                 const uint32_t k = before + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
                                                                       __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
                 const bool mine = (pbits >> r) & 1u;
-                __builtin_amdgcn_raw_buffer_store_b32(row0 + uint32_t(r) * n.nx, wi, int(mine ? 4 * k : kOutside), 0, 0);
-                buf_store(err8[r], wp, mine ? uint32_t(sizeof(T)) * k : kOutside);
+                __builtin_amdgcn_raw_buffer_store_b32(row0 + uint32_t(r) * n.nx, seg_index, int(mine ? 4 * k : kOutside), 0, 0);
+                buf_store(err8[r], seg_payload, mine ? uint32_t(sizeof(T)) * k : kOutside);
                 before += uint32_t(__popcll(m));
             }
             bound_ok = bound_ok && ok;
