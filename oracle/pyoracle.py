@@ -26,9 +26,9 @@ def _load(path):
 def build():
     """Compile liboracle.so (and the reference lib when its sources exist)."""
     import subprocess
-    subprocess.run(["make", "-s", "-C", os.fspath(HERE)], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", os.fspath(HERE)], check=True)  # -B: always recompile
     if Path("/root/reference/cpu/src/codec.cpp").exists():
-        subprocess.run(["make", "-s", "-C", os.fspath(HERE), "ref"], check=True)
+        subprocess.run(["make", "-s", "-B", "-C", os.fspath(HERE), "ref"], check=True)
 
 
 class OracleError(RuntimeError):
