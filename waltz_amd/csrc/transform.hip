@@ -21,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 
 #include "common.hpp"
 #include "fused.hpp"
@@ -35,12 +36,45 @@ WZ_D bool finite_value(T v) {
     return __builtin_isfinite(v);
 }
 
+// One tile (Tile outputs) of one line: the line's n samples at src[base +
+// k * stride], results to dst[dbase + k * dstride] (the band split of the
+// forward, the interleave of the inverse). Generic pointers: global or LDS.
+template <class T, bool Inverse, int Axis, int Tile>
+WZ_D void lift_tile(const T* src, uint64_t base, uint64_t stride, T* dst, uint64_t dbase, uint64_t dstride,
+                    uint32_t n, uint32_t t, uint32_t* status) {
+    constexpr int P = Tile / 2 + 4;
+    const int32_t start = int32_t(t) * Tile;
+    T e[P], o[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int32_t p = start - 4 + 2 * j;
+        const uint32_t ia = Inverse ? split_pos(p, n) : reflect_pos(p, n);
+        const uint32_t ib = Inverse ? split_pos(p + 1, n) : reflect_pos(p + 1, n);
+        e[j] = src[base + uint64_t(ia) * stride];
+        o[j] = src[base + uint64_t(ib) * stride];
+    }
+    if (status) {  // input screen on the first forward pass (cpu/src/codec.cpp:183)
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < P; ++j) ok = ok && finite_value(e[j]) && finite_value(o[j]);
+        if (!ok) atomicOr(status, uint32_t(kNonFiniteInput));
+    }
+    if constexpr (Inverse) lift_inverse<T, P, Axis != 0>(e, o);
+    else lift_forward<T, P>(e, o);
+    const uint32_t h = half_len(n);
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) {
+        const uint32_t p = uint32_t(start) + 2u * uint32_t(j - 2);
+        if (p < n) dst[dbase + uint64_t(Inverse ? p : p >> 1) * dstride] = e[j];
+        if (p + 1 < n) dst[dbase + uint64_t(Inverse ? p + 1 : h + (p >> 1)) * dstride] = o[j];
+    }
+}
+
 // Axis: 0 = X, 1 = Y, 2 = Z.
 template <class T, bool Inverse, int Axis, int Tile>
 __global__ void __launch_bounds__(256) axis_pass(const T* __restrict__ src, T* __restrict__ dst, Dims full,
                                                  Dims act, uint32_t ntiles, uint32_t total,
                                                  uint32_t* __restrict__ status) {
-    constexpr int P = Tile / 2 + 4;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= total) return;
     uint32_t x = 0, y = 0, z = 0, t;
@@ -64,31 +98,38 @@ __global__ void __launch_bounds__(256) axis_pass(const T* __restrict__ src, T* _
     const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
     const uint64_t stride = Axis == 0 ? 1u : Axis == 1 ? uint64_t(full.nx) : plane;
     const uint64_t base = uint64_t(z) * plane + uint64_t(y) * full.nx + x;
-    const int32_t start = int32_t(t) * Tile;
-    T e[P], o[P];
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        const int32_t p = start - 4 + 2 * j;
-        const uint32_t ia = Inverse ? split_pos(p, n) : reflect_pos(p, n);
-        const uint32_t ib = Inverse ? split_pos(p + 1, n) : reflect_pos(p + 1, n);
-        e[j] = src[base + uint64_t(ia) * stride];
-        o[j] = src[base + uint64_t(ib) * stride];
-    }
-    if (status) {  // input screen on the first forward pass (cpu/src/codec.cpp:183)
-        bool ok = true;
-#pragma unroll
-        for (int j = 0; j < P; ++j) ok = ok && finite_value(e[j]) && finite_value(o[j]);
-        if (!ok) atomicOr(status, uint32_t(kNonFiniteInput));
-    }
-    if constexpr (Inverse) lift_inverse<T, P, Axis != 0>(e, o);
-    else lift_forward<T, P>(e, o);
-    const uint32_t h = half_len(n);
-#pragma unroll
-    for (int j = 2; j < P - 2; ++j) {
-        const uint32_t p = uint32_t(start) + 2u * uint32_t(j - 2);
-        if (p < n) dst[base + uint64_t(Inverse ? p : p >> 1) * stride] = e[j];
-        if (p + 1 < n) dst[base + uint64_t(Inverse ? p + 1 : h + (p >> 1)) * stride] = o[j];
-    }
+    lift_tile<T, Inverse, Axis, Tile>(src, base, stride, dst, base, stride, n, t, status);
+}
+
+// Two consecutive in-plane passes (X then Y, or Y then X) of a small box in
+// one launch: one workgroup per z plane stages the plane in LDS, runs the
+// first pass LDS to LDS and the second LDS to dst. The same tile lifting as
+// axis_pass, so the same bits.
+constexpr uint32_t kPlaneMax = 32 * 32;  // samples of a staged plane
+template <class T, bool Inverse, int A0, int A1>
+__global__ void __launch_bounds__(256) plane_pair_pass(const T* __restrict__ src, T* __restrict__ dst, Dims full,
+                                                       Dims act) {
+    __shared__ T P0[kPlaneMax], P1[kPlaneMax];
+    const uint32_t z = blockIdx.x, t = threadIdx.x;
+    const uint64_t plane = uint64_t(full.nx) * full.ny, zb = uint64_t(z) * plane;
+    const uint32_t cells = act.nx * act.ny;
+    for (uint32_t i = t; i < cells; i += 256u) P0[i] = src[zb + uint64_t(i / act.nx) * full.nx + i % act.nx];
+    __syncthreads();
+    auto pass = [&](auto axis_c, const T* s, uint64_t sbase, uint64_t spitch, T* d, uint64_t dbase, uint64_t dpitch) {
+        constexpr int A = decltype(axis_c)::value;
+        const uint32_t n = A == 0 ? act.nx : act.ny, lines = A == 0 ? act.ny : act.nx;
+        const uint32_t ntiles = (n + 7) / 8;
+        for (uint32_t i = t; i < lines * ntiles; i += 256u) {
+            const uint32_t tile = i % ntiles, line = i / ntiles;
+            // X: line y, stride 1; Y: line x, stride = the row pitch
+            const uint64_t so = A == 0 ? uint64_t(line) * spitch : line, ss = A == 0 ? 1u : spitch;
+            const uint64_t dof = A == 0 ? uint64_t(line) * dpitch : line, ds = A == 0 ? 1u : dpitch;
+            lift_tile<T, Inverse, A, 8>(s, sbase + so, ss, d, dbase + dof, ds, n, tile, nullptr);
+        }
+    };
+    pass(std::integral_constant<int, A0>{}, P0, 0, act.nx, P1, 0, act.nx);
+    __syncthreads();
+    pass(std::integral_constant<int, A1>{}, P1, 0, act.nx, dst, zb, full.nx);
 }
 
 template <class T>
@@ -129,6 +170,19 @@ void axis(const T* src, T* dst, Dims full, Dims act, int axis, uint32_t* status,
     else launch_axis<T, Inverse, 2, 8>(src, dst, full, act, status, s);
 }
 
+// X and Y passes of a box whose planes fit the LDS stage, in one launch.
+template <class T, bool Inverse>
+bool plane_pair(const T* src, T* dst, Dims full, Dims act, int a0, int a1, hipStream_t s) {
+    // planes of up to 32 x 32: larger planes keep a workgroup busy for longer
+    // than two launches of the separate passes take (measured at 64 x 64)
+    if (uint64_t(act.nx) * act.ny > 1024 || act.nz == 0 || a0 + a1 != 1) return false;
+    if (a0 == 0)
+        hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 0, 1>), dim3(act.nz), dim3(256), 0, s, src, dst, full, act);
+    else
+        hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 1, 0>), dim3(act.nz), dim3(256), 0, s, src, dst, full, act);
+    return true;
+}
+
 template <class T>
 void copy_box(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
     const uint64_t total = uint64_t(act.nx) * act.ny * act.nz;
@@ -160,9 +214,18 @@ struct Passes {
         const T* src = first ? first_src : home;
         T* final_dst = last_level && last_dst ? last_dst : home;
         for (int k = 0; k < count; ++k) {
-            T* dst = k == count - 1 ? final_dst : tmp[k & 1];
-            if (dst == src) dst = tmp[0];
-            axis<T, Inverse>(src, dst, full, act, axes[k], first && k == 0 ? status : nullptr, s);
+            const bool screen = first && k == 0 && status;
+            // two in-plane passes of a small box: one launch
+            const bool pair = !screen && k + 1 < count && axes[k] != 2 && axes[k + 1] != 2;
+            const int last = pair ? k + 1 : k;
+            T* const other = src == tmp[0] ? tmp[1] : tmp[0];  // never the source
+            T* dst = last == count - 1 ? final_dst : other;
+            if (dst == src) dst = other;  // in place at the end: through scratch, copied back below
+            if (pair && plane_pair<T, Inverse>(src, dst, full, act, axes[k], axes[k + 1], s)) {
+                ++k;
+            } else {
+                axis<T, Inverse>(src, dst, full, act, axes[k], screen ? status : nullptr, s);
+            }
             src = dst;
         }
         if (src != final_dst) copy_box<T>(src, final_dst, full, act, s);
