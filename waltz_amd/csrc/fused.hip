@@ -99,6 +99,15 @@ template <>
 WZ_D double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
 }
+// Streaming (non-temporal) stores of a level's output: they would otherwise
+// evict the input lines that neighbouring tiles' halos read again from L2.
+WZ_D void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(off), 0, 2);
+}
+WZ_D void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, int(off), 0, 2);
+}
 WZ_D void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(off), 0, 0);
 }
@@ -146,12 +155,15 @@ WZ_D void inv_lift_stream(Ld&& ld, Out&& out) {
 }
 
 // Four consecutive staged values to a 16-byte aligned destination.
+// (non-temporal, see buf_store_nt)
 WZ_D void store_vec4(float* dst, const float* v) {
-    *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(v);
+    using F4 = float __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(*reinterpret_cast<const F4*>(v), reinterpret_cast<F4*>(dst));
 }
 WZ_D void store_vec4(double* dst, const double* v) {
-    reinterpret_cast<double2*>(dst)[0] = reinterpret_cast<const double2*>(v)[0];
-    reinterpret_cast<double2*>(dst)[1] = reinterpret_cast<const double2*>(v)[1];
+    using D2 = double __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(reinterpret_cast<const D2*>(v)[0], reinterpret_cast<D2*>(dst));
+    __builtin_nontemporal_store(reinterpret_cast<const D2*>(v)[1], reinterpret_cast<D2*>(dst) + 1);
 }
 
 }  // namespace
@@ -325,7 +337,7 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
             } else {
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    if (2 * (ix + k) + xb < n.nx) dst[k] = v[k];
+                    if (2 * (ix + k) + xb < n.nx) __builtin_nontemporal_store(v[k], dst + k);
             }
         }
     }
@@ -556,7 +568,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
             const auto wout = plane_window(a.out, Z);
             auto put = [&](uint32_t j, T v) {
                 const uint32_t Y = y0 + 8 * yt + j;
-                buf_store(v, wout, lane_off(Y));
+                buf_store_nt(v, wout, lane_off(Y));
                 fin = fin && (!(xz && Y < n.ny) || __builtin_isfinite(v));
             };
             inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
