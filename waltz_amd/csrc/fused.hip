@@ -34,6 +34,12 @@ constexpr int PA = FW + 1;                 // LDS pitch (odd: spreads banks)
 constexpr int FT = 576;                    // threads (9 wave64s) per workgroup
 constexpr int FSLOTS = FW * FH / FT;       // 5 footprint samples per thread and plane
 static_assert(FW * FH == FSLOTS * FT, "footprint must tile the workgroup");
+// The inverse level runs 8 waves (two per SIMD): six footprint slots per
+// thread, the last slot of threads past the footprint repeating their fifth
+// (the same column computed twice and written to the same place).
+constexpr int IT = 512;
+constexpr int ISLOTS = (FW * FH + IT - 1) / IT;
+static_assert(ISLOTS == 6 && FW * FH > (ISLOTS - 1) * IT, "inverse footprint slots");
 
 // Workgroup -> tile, XCD-aware. The dispatcher deals consecutive workgroups
 // round-robin to the 8 XCDs, each with its own L2; XCD k is given the k-th
@@ -343,8 +349,9 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
     }
 }
 
-// Inverse level, 576 threads per 64 x 32 x sz tile; one step per z pair:
-// every thread streams 5 footprint columns through the Z synthesis (with the
+// Inverse level, 512 threads (8 wave64s, two per SIMD) per 64 x 32 x sz tile;
+// one step per z pair: every thread streams 6 footprint columns (the last one
+// a repeat for the threads past the 2880 columns) through the Z synthesis (with the
 // tile-end fold side chain), refilling each column's registers with the next
 // pair as soon as they are consumed, and stages the two reconstructed planes
 // in Zp; X then runs on 320 (plane, row, 16-sample) items and Y on 512
@@ -354,7 +361,7 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 // For the repair emitter each Y thread's 8 original samples are loaded as
 // the step begins.
 template <class T, int Emit, bool Requant = false, bool RowWords = true>
-__global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
+__global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
     __shared__ T Zp[2 * FH * PA];                // two reconstructed z planes (split XY)
     __shared__ T Xp[2 * FH * (TX + 1)];          // after X: rows still split in Y
@@ -376,12 +383,12 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     // Waves then read from one base pointer (wave-uniform) with 32-bit byte
     // offsets; only the one wave that straddles the two groups selects per lane.
     constexpr uint32_t kLow = (FW / 2) * (FH / 2);
-    uint32_t eoff[FSLOTS], ooff[FSLOTS];   // even-plane and odd-plane byte offsets
-    uint32_t zpk[(FSLOTS + 1) / 2] = {};     // Zp offsets, two 16-bit halves per word
+    uint32_t eoff[ISLOTS], ooff[ISLOTS];   // even-plane and odd-plane byte offsets
+    uint32_t zpk[(ISLOTS + 1) / 2] = {};     // Zp offsets, two 16-bit halves per word
     uint32_t low_lanes = 0, lo_wave = 0, hi_wave = 0;   // per slot: lane / whole-wave classes
 #pragma unroll
-    for (int s = 0; s < FSLOTS; ++s) {
-        const uint32_t idx = t + FT * s;
+    for (int s = 0; s < ISLOTS; ++s) {
+        const uint32_t idx = t + IT * s < uint32_t(FW * FH) ? t + IT * s : t + IT * (s - 1);
         uint32_t r, c;
         if (idx < kLow) {
             r = 2 * (idx / (FW / 2));
@@ -465,9 +472,9 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     };
     // streaming state per slot: OS = os[g-1], EA = eA[g-1], OB = oB[g-2],
     // EC = eC[g-2]; SF rotates through o_raw[B-2], eA'[B-1], oB'[B-1], eC'[B]
-    T OS[FSLOTS], EA[FSLOTS], OB[FSLOTS], EC[FSLOTS], SF[FSLOTS];
+    T OS[ISLOTS], EA[ISLOTS], OB[ISLOTS], EC[ISLOTS], SF[ISLOTS];
 #pragma unroll
-    for (int s = 0; s < FSLOTS; ++s) OS[s] = EA[s] = OB[s] = EC[s] = SF[s] = T(0);
+    for (int s = 0; s < ISLOTS; ++s) OS[s] = EA[s] = OB[s] = EC[s] = SF[s] = T(0);
 
     const int32_t g0 = int32_t(zs / 2) - 2, g1 = int32_t((zs + a.sz) / 2) + 1;
     // float repair emitter: a step's originals are loaded as the step begins,
@@ -479,15 +486,15 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
     // ve/vo hold pair g's coefficients; each slot's registers are refilled
     // with pair g+1 as soon as the slot has consumed them (in flight during
     // the rest of the step)
-    T ve[FSLOTS], vo[FSLOTS];
+    T ve[ISLOTS], vo[ISLOTS];
     {
         const PairSrc q = pair_src(g0);
 #pragma unroll
-        for (int s = 0; s < FSLOTS; ++s) load_slot(q, s, ve[s], vo[s]);
+        for (int s = 0; s < ISLOTS; ++s) load_slot(q, s, ve[s], vo[s]);
     }
-    // One step per z pair. The warm-up steps (Z only), the emitting steps of
-    // the Y waves and those of the X-only wave run as separate loops, so
-    // each loop issues the same memory operations every iteration and the
+    // One step per z pair. The warm-up steps (Z only) and the emitting steps
+    // run as separate loops (and a wave without Y items would run its own),
+    // so each loop issues the same memory operations every iteration and the
     // memory-counter waits stay exact (a merged loop makes a wave wait for
     // its own previous stores before it starts the Z pass).
     auto step = [&](int32_t g, auto emit_c, auto y_c) {
@@ -498,7 +505,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         const PairSrc nq = pair_src(g < g1 ? g + 1 : g);
         const int phase = ((g % H) + H) % H;
 #pragma unroll
-        for (int s = 0; s < FSLOTS; ++s) {
+        for (int s = 0; s < ISLOTS; ++s) {
             const T oraw = band(vo[s]);
             const T es = even_band(s, ve[s]) * C::ik, os = oraw * (-C::k);
             const T eA = fmadd(-C::d, OS[s] + os, es);   // eA[g]
@@ -538,7 +545,7 @@ __global__ void __launch_bounds__(FT) inv_level_kernel(InvLevel<T> a) {
         __syncthreads();
         // X (unfolded, so any tile width gives the same bits; 16 samples per
         // item keeps the 320 items in one round): plane p, row rr, tile xt
-        for (uint32_t w = t; w < 2 * FH * (TX / 16); w += FT) {
+        for (uint32_t w = t; w < 2 * FH * (TX / 16); w += IT) {
             const uint32_t p = w / (FH * (TX / 16)), rem = w % (FH * (TX / 16));
             const uint32_t rr = rem % FH, xt = rem / FH;
             const T* src = Zp + p * FH * PA + rr * PA + 16 * xt;
@@ -720,17 +727,17 @@ void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
     const dim3 grid(xcd_blocks(tile_grid(a.n, a.sz)));
     const bool words = (a.n.nx & 31u) == 0;
     if (a.emit == kEmitRepair && a.q > T(0) && words)
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true>), grid, dim3(FT), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true>), grid, dim3(IT), 0, s, a);
     else if (a.emit == kEmitRepair && a.q > T(0))
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true, false>), grid, dim3(FT), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, true, false>), grid, dim3(IT), 0, s, a);
     else if (a.emit == kEmitRepair && words)
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(FT), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(IT), 0, s, a);
     else if (a.emit == kEmitRepair)
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, false, false>), grid, dim3(FT), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, false, false>), grid, dim3(IT), 0, s, a);
     else if (a.emit == kEmitStoreScreen)
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(FT), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(IT), 0, s, a);
     else
-        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore>), grid, dim3(FT), 0, s, a);
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore>), grid, dim3(IT), 0, s, a);
 }
 
 template <class T>
@@ -739,8 +746,8 @@ uint32_t repair_z_segment(Dims n, uint32_t tuned) {
     static const uint32_t per_cu = [] {
         int b = 0;
         int b2 = 0;  // both repair variants: the smaller residency
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, inv_level_kernel<T, kEmitRepair>, FT, 0) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, inv_level_kernel<T, kEmitRepair, true>, FT, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, inv_level_kernel<T, kEmitRepair>, IT, 0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, inv_level_kernel<T, kEmitRepair, true>, IT, 0) !=
                 hipSuccess)
             b = b2 = 1;
         b = b < b2 ? b : b2;
