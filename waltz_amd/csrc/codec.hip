@@ -180,6 +180,8 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     std::lock_guard<std::mutex> guard(w.lock);
     w.init();
     // layout of the per-call scratch
+    // brick-pair fields quantize inside the encoder: no sign words to clear
+    const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
     auto plan = [&](Carve& c) {
         struct P {
             T *coeff, *t0, *t1, *rec;
@@ -207,7 +209,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.size = c.take<uint32_t>(ws.chunks);
         // zero-initialised region: sign, q bitmap, lookback descriptors, counters
         const size_t z0 = c.used;
-        p.sign = c.take<uint32_t>(words);
+        p.sign = c.take<uint32_t>(pairs ? 0 : words);
         p.qbm = c.take<uint32_t>(words);
         p.desc = c.take<unsigned long long>(ws.chunks);
         p.counters = c.take<uint32_t>(4);  // ticket, q count, repair total
