@@ -105,6 +105,15 @@ template <>
 WZ_D double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
 }
+// Streaming (non-temporal) load of data read exactly once (the repair
+// emitter's originals), so it does not evict the coefficient lines that
+// neighbouring tiles' halos read again.
+WZ_D float buf_load_nt(float, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(off), 0, 2));
+}
+WZ_D double buf_load_nt(double, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 2));
+}
 // Streaming (non-temporal) stores of a level's output: they would otherwise
 // evict the input lines that neighbouring tiles' halos read again from L2.
 WZ_D void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
@@ -468,7 +477,7 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
         const uint32_t Z = 2 * uint32_t(g - 2) + yp;
         const auto w = plane_window(a.orig, yown ? Z : n.nz);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) ref[j] = buf_load<T>(w, lane_off(y0 + 8 * yt + j));
+        for (int j = 0; j < 8; ++j) ref[j] = buf_load_nt(T(0), w, lane_off(y0 + 8 * yt + j));
     };
     // streaming state per slot: OS = os[g-1], EA = eA[g-1], OB = oB[g-2],
     // EC = eC[g-2]; SF rotates through o_raw[B-2], eA'[B-1], oB'[B-1], eC'[B]
