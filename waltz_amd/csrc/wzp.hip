@@ -499,16 +499,19 @@ __global__ void __launch_bounds__(kPackThreads) wzp_pack_kernel(uint8_t* __restr
         return Unit{d - lead + 16 * q, w.staging + uint64_t(first + i) * kChunkSlot, int32_t(16 * q) - int32_t(lead),
                     off_sh[i + 1] - off_sh[i]};
     };
+    // the archive is not read again here: non-temporal stores (staging keeps
+    // ordinary loads, which neighbouring destination groups share)
+    using U4 = uint32_t __attribute__((ext_vector_type(4)));
     for (uint32_t u0 = t; u0 < all; u0 += kPackThreads * kPackUnroll) {
-        uint4 lo[kPackUnroll], hi[kPackUnroll];
+        U4 lo[kPackUnroll], hi[kPackUnroll];
 #pragma unroll
         for (uint32_t r = 0; r < kPackUnroll; ++r) {
             const uint32_t u = u0 + r * kPackThreads;
             if (u >= all) break;
             const Unit x = unit(u);
             if (x.o >= 0 && uint32_t(x.o) + 16 <= x.size) {  // the 16 bytes lie in two aligned 16-byte loads
-                lo[r] = *reinterpret_cast<const uint4*>(x.src + (uint32_t(x.o) & ~15u));
-                hi[r] = *reinterpret_cast<const uint4*>(x.src + (uint32_t(x.o) & ~15u) + 16);
+                lo[r] = *reinterpret_cast<const U4*>(x.src + (uint32_t(x.o) & ~15u));
+                hi[r] = *reinterpret_cast<const U4*>(x.src + (uint32_t(x.o) & ~15u) + 16);
             }
         }
 #pragma unroll
@@ -524,9 +527,9 @@ __global__ void __launch_bounds__(kPackThreads) wzp_pack_kernel(uint8_t* __restr
 #pragma unroll
                 for (int j = 0; j < 5; ++j)
                     v[j] = k == 0 ? wv[j] : k == 1 ? wv[j + 1] : k == 2 ? wv[j + 2] : wv[j + 3];
-                *reinterpret_cast<uint4*>(x.dst) =
-                    make_uint4(__builtin_amdgcn_alignbyte(v[1], v[0], sb), __builtin_amdgcn_alignbyte(v[2], v[1], sb),
-                               __builtin_amdgcn_alignbyte(v[3], v[2], sb), __builtin_amdgcn_alignbyte(v[4], v[3], sb));
+                const U4 o = {__builtin_amdgcn_alignbyte(v[1], v[0], sb), __builtin_amdgcn_alignbyte(v[2], v[1], sb),
+                              __builtin_amdgcn_alignbyte(v[3], v[2], sb), __builtin_amdgcn_alignbyte(v[4], v[3], sb)};
+                __builtin_nontemporal_store(o, reinterpret_cast<U4*>(x.dst));
             } else {  // an edge group: the chunk's bytes only
                 for (int j = 0; j < 16; ++j) {
                     const int32_t p = x.o + j;
