@@ -7,7 +7,7 @@ NumPy-facing ctypes bindings for:
     sources by ``oracle/Makefile`` (present only when it was built).
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg use
-this module, and only as the checker. The product never imports it.
+this module, and only to check results against. The product never imports it.
 """
 import ctypes as C
 import os

@@ -3,7 +3,7 @@
  *
  * A plain-C restatement of the reference Waltz CPU scalar codec (the normative
  * byte/bit profile of the archive format). Only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg may load it, and there only as the checker;
+ * and bench.py's cpu_baseline leg may load it, and there only to check against;
  * the product library never links, loads or calls it.
  *
  * Every function cites the reference file:line it restates (paths relative to
