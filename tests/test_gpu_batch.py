@@ -1,9 +1,9 @@
-"""Batched decode parity: every tile of a batch equals the single-field
-decoder's output for the same archive bit for bit (which the codec tests pin
-to the oracle), through the mixed-shape batch ABI (eager and replayed from a
-captured graph) and the reference's frame ABI waltz_gpu_decode_batch (clipping
-at the frame edge, holes filled), following test_gpu_loader.py:104-125 of the
-reference (batch == per-chunk decode placed into the frame)."""
+"""Batched decode parity: every tile of a batch equals the oracle's decode of
+the same archive bit for bit, through the mixed-shape batch ABI (eager and
+replayed from a captured graph) and the reference's frame ABI
+waltz_gpu_decode_batch (clipping at the frame edge, holes filled), following
+python/tests/test_gpu_loader.py:104-125 of the reference (batch == per-chunk
+CPU decode placed into the frame)."""
 import ctypes as C
 
 import numpy as np
@@ -38,7 +38,7 @@ def _archives(waltz_amd, shapes, dtype, bound):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
-def test_mixed_batch_matches_single_decoder(wz, dtype):
+def test_mixed_batch_matches_oracle(wz, oracle, dtype):
     waltz_amd, batch = wz
     tdt = torch.float32 if dtype == np.float32 else torch.float64
     _, archives = _archives(waltz_amd, SHAPES, dtype, 1e-3)
@@ -46,8 +46,8 @@ def test_mixed_batch_matches_single_decoder(wz, dtype):
     dec = batch.BatchDecoder(SHAPES, tdt)
     flat = dec.decode(packed, spans)
     for s, a, tile in zip(SHAPES, archives, dec.views(flat)):
-        ref = waltz_amd.decompress(a, shape=s, dtype=tdt)
-        assert torch.equal(tile, ref), s
+        want = oracle.decompress(a.cpu().numpy().tobytes(), s, dtype)
+        assert np.array_equal(tile.cpu().numpy().view(np.uint8), want.view(np.uint8)), s
     again = dec.decode(packed, spans)
     assert torch.equal(again, flat)
 
@@ -78,7 +78,7 @@ def test_graph_replay_matches_eager(wz):
         _ = eager.sum()  # eager work between replays
 
 
-def test_corrupt_archive_in_batch_is_reported(wz):
+def test_corrupt_archive_in_batch_is_reported(wz, oracle):
     waltz_amd, batch = wz
     shapes = SHAPES[:3]
     _, archives = _archives(waltz_amd, shapes, np.float32, 1e-3)
@@ -90,11 +90,12 @@ def test_corrupt_archive_in_batch_is_reported(wz):
         dec.decode(packed, spans)
     packed, spans = batch.pack(archives)
     flat = dec.decode(packed, spans)  # usable afterwards
-    assert torch.equal(dec.views(flat)[0], waltz_amd.decompress(archives[0], shape=shapes[0], dtype=torch.float32))
+    want = oracle.decompress(archives[0].cpu().numpy().tobytes(), shapes[0], np.float32)
+    assert np.array_equal(dec.views(flat)[0].cpu().numpy().view(np.uint32), want.view(np.uint32))
 
 
 @pytest.mark.parametrize("width", [4, 8])
-def test_reference_frame_abi(wz, width):
+def test_reference_frame_abi(wz, oracle, width):
     """waltz_gpu_decode_batch: 16x16x8 tiles in 40x30x20 frames (clipped at the
     frame edge), two samples, two sides, a few holes filled with -7."""
     waltz_amd, batch = wz
@@ -147,7 +148,7 @@ def test_reference_frame_abi(wz, width):
         r //= gz
         side, sample = r % sides, r // sides
         tile = (np.full((tz, ty, tx), -7.0, dtype) if a is None else
-                waltz_amd.decompress(a, shape=(tz, ty, tx), dtype=tdt).cpu().numpy())
+                oracle.decompress(a.cpu().numpy().tobytes(), (tz, ty, tx), dtype))
         ez, ey, ex = min(tz, fz - oz), min(ty, fy - oy), min(tx, fx - ox)
         want[side, sample, oz:oz + ez, oy:oy + ey, ox:ox + ex] = tile[:ez, :ey, :ex]
     assert np.array_equal(out.cpu().numpy().view(np.uint8), want.view(np.uint8))

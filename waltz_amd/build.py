@@ -1,4 +1,5 @@
-"""Build libwaltz.so (all HIP kernels + host pipelines + C/C++ ABI) for gfx950.
+"""Build libwaltz.so (all HIP kernels + host pipelines + C/C++ ABI) for gfx950,
+and the `waltz` command-line tool (bin/waltz) over its C++ API.
 
     python -m waltz_amd.build          # or waltz_amd.build.build()
 
@@ -58,7 +59,29 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI_SRC = PKG / "cli" / "waltz.cpp"
+CLI = PKG / "bin" / "waltz"
+
+
+def build_cli(verbose=False):
+    """The `waltz` command-line tool (host C++ over the C++ API, linked to libwaltz.so)."""
+    CLI.parent.mkdir(exist_ok=True)
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    tmp = CLI.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", f"-I{rocm / 'include'}",
+           str(CLI_SRC), "-o", str(tmp), f"-L{PKG}", "-lwaltz", f"-L{rocm / 'lib'}", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{rocm / 'lib'}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError("CLI build failed\n" + r.stderr[-4000:])
+    os.replace(tmp, CLI)
+    if verbose:
+        print("built", CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -138,19 +138,34 @@ void extrema(const T* d, uint64_t n, double& mn, double& mx, hipStream_t s) {
 // ------------------------------------------------------------------ C++ API
 namespace WALTZ {
 
+// Shape and buffer checks of src/waltz.cu:335-349: nonzero dimensions whose
+// product fits the WZP element limit and equals config.num, non-null buffers,
+// and an input that does not overlap the archive's capacity.
+template <typename T>
+wz::Dims checked_shape(const Config& c, const void* input, const void* archive) {
+    if (!c.dimx || !c.dimy || !c.dimz || c.dimx > 0xFFFFFFFFull / 2 || c.dimy > (0xFFFFFFFFull / 2) / c.dimx ||
+        c.dimz > (0xFFFFFFFFull / 2) / (c.dimx * c.dimy) || c.num != c.dimx * c.dimy * c.dimz || !input || !archive)
+        throw std::invalid_argument("Waltz: invalid buffers or shape for WZP");
+    return dims_of(c);
+}
+
 template <typename T, int NDIMS>
 size_t d_WALTZ_compress(T* d_oridata, char* d_cmpData, const Config& config, const char*, void* stream) {
     static_assert(NDIMS == 3, "3-D only");
-    const wz::Dims d = dims_of(config);
+    const wz::Dims d = checked_shape<T>(config, d_oridata, d_cmpData);
     const uint64_t n = uint64_t(d.nx) * d.ny * d.nz;
+    const size_t capacity = compressed_buffer_capacity<T>(n);
+    const uintptr_t src = reinterpret_cast<uintptr_t>(d_oridata), dst = reinterpret_cast<uintptr_t>(d_cmpData);
+    if (src <= dst ? dst - src < n * sizeof(T) : src - dst < capacity)
+        throw std::invalid_argument("Waltz: input and output buffers overlap");
     return size_t(wz::compress<T>(d_oridata, d, config.absErrorBound, reinterpret_cast<uint8_t*>(d_cmpData),
-                                  compressed_buffer_capacity<T>(n), as_stream(stream)));
+                                  capacity, as_stream(stream)));
 }
 
 template <typename T, int NDIMS>
 size_t d_WALTZ_decompress(char* d_cmpData, T* d_decData, const Config& config, const char*, void* stream) {
     static_assert(NDIMS == 3, "3-D only");
-    const wz::Dims d = dims_of(config);
+    const wz::Dims d = checked_shape<T>(config, d_decData, d_cmpData);
     wz::decompress<T>(reinterpret_cast<const uint8_t*>(d_cmpData), ~uint64_t(0), d, d_decData, as_stream(stream));
     return size_t(uint64_t(d.nx) * d.ny * d.nz * sizeof(T));
 }
@@ -448,14 +463,31 @@ int waltz_hip_decode(waltz_hip_session* s, const waltz_hip_config* c, const void
     return rc;
 }
 
+// src/waltz.cu:836-865: argument rule (width 4 or 8, nonzero dimensions, a
+// volume of at most 2^32 - 1 bytes, out_bytes exactly the volume's bytes, an
+// archive of at least the header) and a canonical archive of exactly
+// archive_bytes. Calls are serialised; the stream is synchronised before
+// returning, also on failure.
 const char* waltz_cuda_decode_chunk(uint32_t nx, uint32_t ny, uint32_t nz, int width, const void* archive,
                                     size_t archive_bytes, void* output, size_t output_bytes, void* stream) {
-    if (uint64_t(nx) * ny * nz * uint64_t(width) > output_bytes) {
-        g_error = "output capacity too small";
-        return g_error.c_str();
-    }
-    const int rc = waltz_amd_decompress(width, archive, archive_bytes, nx, ny, nz, output, stream);
-    return rc ? g_error.c_str() : nullptr;
+    static std::mutex chunk_lock;
+    const int rc = guarded([&] {
+        std::lock_guard<std::mutex> hold(chunk_lock);
+        if ((width != 4 && width != 8) || !nx || !ny || !nz || !archive || !output || archive_bytes < 32 ||
+            uint64_t(nx) * ny > 0xFFFFFFFFull / unsigned(width) / nz ||
+            output_bytes != uint64_t(nx) * ny * nz * unsigned(width))
+            throw std::invalid_argument("invalid chunk dimensions or buffer sizes");
+        const wz::Dims d = dims_of(nx, ny, nz);
+        if (width == 4)
+            wz::decompress<float>(static_cast<const uint8_t*>(archive), archive_bytes, d, static_cast<float*>(output),
+                                  as_stream(stream), true);
+        else
+            wz::decompress<double>(static_cast<const uint8_t*>(archive), archive_bytes, d,
+                                   static_cast<double*>(output), as_stream(stream), true);
+    });
+    if (!rc) return nullptr;
+    (void)hipStreamSynchronize(as_stream(stream));
+    return g_error.c_str();
 }
 
 }  // extern "C"

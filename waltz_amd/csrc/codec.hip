@@ -145,6 +145,13 @@ Geometry geometry(Dims d, unsigned width) {
 
 uint32_t qcap(uint32_t n) { return n <= (1u << 20) ? n : (1u << 20); }
 
+// Caller buffers must not overlap (cpu/src/codec.cpp:82-86).
+void check_disjoint(const void* a, uint64_t as, const void* b, uint64_t bs) {
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    if (as > UINTPTR_MAX - x || bs > UINTPTR_MAX - y) throw std::invalid_argument("pointer range overflow");
+    if (x < y + bs && y < x + as) throw std::invalid_argument("input and output must not overlap");
+}
+
 }  // namespace
 
 uint64_t max_compressed_bytes(Dims d, unsigned width) {
@@ -166,6 +173,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     if (!(q > T(0)) || !std::isfinite(q) || !std::isfinite(inv))
         throw std::invalid_argument("quantum not representable in native type");
     if (capacity < 32) throw std::runtime_error("archive capacity too small");
+    check_disjoint(input, uint64_t(g.n) * sizeof(T), archive, capacity);
     const uint32_t n = g.n;
     const uint64_t words = (uint64_t(n) + 31) / 32;
     const WzpShape ws = wzp_shape(n);
@@ -300,15 +308,48 @@ uint64_t archive_size_from_header(const uint8_t* h, Dims d, unsigned width) {
     return archive_layout(wzp, np, pb, nq).end;
 }
 
+// Scratch of one decode: coefficient volume, two lifting scratch volumes,
+// magnitudes and sign words, each 256-byte aligned.
+uint64_t decode_scratch_bytes(uint32_t n, unsigned width) {
+    Carve probe{nullptr};
+    probe.take<uint8_t>(uint64_t(n) * width);
+    probe.take<uint8_t>(uint64_t(n) * width);
+    probe.take<uint8_t>(uint64_t(n) * width);
+    probe.take<uint16_t>(n);
+    probe.take<uint32_t>((uint64_t(n) + 31) / 32);
+    return probe.used;
+}
+
+namespace {
+
+// Where one decode's memory comes from: the device scratch (decode_scratch_bytes),
+// the rank tables for geometry d and band bz, the device result record and a
+// host copy of it (at least 48 bytes, also used for the 32-byte header).
+struct DecodeSource {
+    uint8_t* scratch;
+    EncodeResult* result;
+    EncodeResult* host;
+    DeviceWorkspace* cache;  // cached rank tables, or nullptr: rebuilt into `rank`
+    uint16_t* rank;
+    const uint16_t* ranks(Dims d, uint32_t bz, hipStream_t s) const {
+        if (cache) return cache->ranks(d, bz, s);
+        build_rank_tables(d, bz, rank, rank + 8 * 8192, s);
+        return rank;
+    }
+};
+
 // One decode of a validated archive into `output`; returns the device status.
 // canonical: both side lists are taken as strictly increasing (the archives
 // compress writes), and a list found out of order comes back as kUnordered;
 // otherwise both lists are first checked for duplicate indices.
 template <class T>
 uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry& g, T* output, bool canonical,
-                     DeviceWorkspace& w, hipStream_t s) {
+                     const DecodeSource& m, hipStream_t s) {
     const uint32_t n = g.n;
     const uint32_t bz = hdr[3];
+    // the archive's own schedule: a plane-schedule archive with equal level
+    // counts is valid (cpu/src/codec.cpp:279-283)
+    const bool dyadic = (hdr[0] & 1u) != 0;
     uint64_t wzp = 0;
     for (int k = 7; k >= 0; --k) wzp = (wzp << 8) | hdr[16 + k];
     auto rd32 = [&](int o) {
@@ -321,20 +362,14 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
     std::memcpy(&quantum, &qb, 8);
     const Layout L = archive_layout(wzp, np, uint64_t(np) * sizeof(T), nq);
     const uint64_t words = (uint64_t(n) + 31) / 32;
-    Carve probe{nullptr};
-    probe.take<T>(n);
-    probe.take<T>(n);
-    probe.take<T>(n);
-    probe.take<uint16_t>(n);
-    probe.take<uint32_t>(words);
-    Carve c{static_cast<uint8_t*>(w.reserve(probe.used))};
+    Carve c{m.scratch};
     T* coeff = c.take<T>(n);
     T* t0 = c.take<T>(n);
     T* t1 = c.take<T>(n);
     uint16_t* mag = c.take<uint16_t>(n);
     uint32_t* sign = c.take<uint32_t>(words);
-    uint32_t* status = &w.result->status;
-    zero_words(w.result, sizeof(EncodeResult) / 4, s);
+    uint32_t* status = &m.result->status;
+    zero_words(m.result, sizeof(EncodeResult) / 4, s);
     if (!canonical) {  // duplicate indices fail (cpu/src/codec.cpp:260-275); `sign` is free until decode
         check_unique(archive + L.qi, nq, n, sign, status, s);
         check_unique(archive + L.pi, np, n, sign, status, s);
@@ -349,7 +384,7 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
             wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
         }
         const BrickGeometry bg = brick_geometry(g.d, bz);
-        const uint16_t* rank = w.ranks(g.d, bz, s);
+        const uint16_t* rank = m.ranks(g.d, bz, s);
         StageClock ck("dequantize", s);
         dequantize_bricks<T>(mag, sign, bg, quantum, rank, coeff, s);
     }
@@ -357,52 +392,90 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
         StageClock ck("qoutliers", s);
         apply_qoutliers<T>(coeff, archive + L.qi, archive + L.qv, nq, n, quantum, status, s);
     }
-    const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, g.dyadic, s, &screen);
+    const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, dyadic, s, &screen);
     {
         StageClock ck("patches", s);
         apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);  // screens patched samples
         if (!screened) check_finite<T>(output, n, status, s);
     }
-    hip_check(hipMemcpyAsync(w.host_result, w.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s),
-              "status copy");
+    hip_check(hipMemcpyAsync(m.host, m.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s), "status copy");
     hip_check(hipStreamSynchronize(s), "decompress");
     hip_check(hipGetLastError(), "decompress launch");
-    return w.host_result->status;
+    return m.host->status;
 }
 
+// Header read and checks, then the decode (with the duplicate-checking pass
+// when a side list is out of order). The scratch of `m` must hold
+// decode_scratch_bytes(n, sizeof(T)).
 template <class T>
-void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s) {
+void decompress_with(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s,
+                     bool canonical_chunk, const DecodeSource& m) {
+    const Geometry g = geometry(d, sizeof(T));
+    uint8_t* h = reinterpret_cast<uint8_t*>(m.host);  // 48 host bytes, reused for the header
+    hip_check(hipMemcpyAsync(h, archive, 32, hipMemcpyDeviceToHost, s), "header copy");
+    hip_check(hipStreamSynchronize(s), "header copy");
+    uint8_t hdr[32];
+    std::memcpy(hdr, h, 32);
+    const uint64_t wzp = uint64_t(hdr[16]) | (uint64_t(hdr[17]) << 8) | (uint64_t(hdr[18]) << 16) |
+                         (uint64_t(hdr[19]) << 24) | (uint64_t(hdr[20]) << 32) | (uint64_t(hdr[21]) << 40) |
+                         (uint64_t(hdr[22]) << 48) | (uint64_t(hdr[23]) << 56);
+    const WzpShape ws = wzp_shape(g.n);
+    if (canonical_chunk) {  // the chunk decoder takes canonical archives only (src/waltz.cu:631-646)
+        const uint8_t flags = uint8_t((g.dyadic ? 1u : 0u) | (sizeof(T) == 8 ? 4u : 0u) | (3u << 3));
+        if (hdr[0] != flags || hdr[1] != g.lx || hdr[2] != g.lz || hdr[3] != g.bz || wzp < ws.header ||
+            wzp > archive_bytes - 32)
+            throw std::runtime_error("invalid canonical Waltz chunk shape, header or length");
+    }
+    const uint64_t end = archive_size_from_header(hdr, d, sizeof(T));
+    if (archive_bytes != ~uint64_t(0) && end != archive_bytes)
+        throw std::runtime_error(canonical_chunk ? "invalid canonical Waltz chunk shape, header or length"
+                                                 : "truncated archive or trailing data");
+    check_disjoint(archive, end, output, uint64_t(g.n) * sizeof(T));
+    if (wzp < ws.header || wzp > ws.header + uint64_t(g.n) * 2 + (uint64_t(g.n) + 7) / 8)
+        throw std::runtime_error("WZP archive size is outside its valid bounds");
+    uint32_t st = decode_pass<T>(archive, hdr, g, output, true, m, s);
+    // a side list out of order (not written by compress, but valid when its
+    // indices are distinct): decode again with the duplicate checks
+    if (st & kUnordered) st = decode_pass<T>(archive, hdr, g, output, false, m, s) & ~uint32_t(kUnordered);
+    profile_collect();
+    if (st) raise_status(st);
+}
+
+}  // namespace
+
+template <class T>
+void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s,
+                bool canonical_chunk) {
     if (reinterpret_cast<uintptr_t>(archive) % 8 || reinterpret_cast<uintptr_t>(output) % sizeof(T))
         throw std::invalid_argument("archive must be 8-byte aligned and output naturally aligned");
     const Geometry g = geometry(d, sizeof(T));
     DeviceWorkspace& w = workspace();
     std::lock_guard<std::mutex> guard(w.lock);
     w.init();
-    uint8_t* h = reinterpret_cast<uint8_t*>(w.host_result);  // 48 pinned bytes, reused for the header
-    hip_check(hipMemcpyAsync(h, archive, 32, hipMemcpyDeviceToHost, s), "header copy");
-    hip_check(hipStreamSynchronize(s), "header copy");
-    uint8_t hdr[32];
-    std::memcpy(hdr, h, 32);
-    const uint64_t end = archive_size_from_header(hdr, d, sizeof(T));
-    if (archive_bytes != ~uint64_t(0) && end != archive_bytes)
-        throw std::runtime_error("truncated archive or trailing data");
-    const uint64_t wzp = uint64_t(hdr[16]) | (uint64_t(hdr[17]) << 8) | (uint64_t(hdr[18]) << 16) |
-                         (uint64_t(hdr[19]) << 24) | (uint64_t(hdr[20]) << 32) | (uint64_t(hdr[21]) << 40) |
-                         (uint64_t(hdr[22]) << 48) | (uint64_t(hdr[23]) << 56);
-    const WzpShape ws = wzp_shape(g.n);
-    if (wzp < ws.header || wzp > ws.header + uint64_t(g.n) * 2 + (uint64_t(g.n) + 7) / 8)
-        throw std::runtime_error("WZP archive size is outside its valid bounds");
-    uint32_t st = decode_pass<T>(archive, hdr, g, output, true, w, s);
-    // a side list out of order (not written by compress, but valid when its
-    // indices are distinct): decode again with the duplicate checks
-    if (st & kUnordered) st = decode_pass<T>(archive, hdr, g, output, false, w, s) & ~uint32_t(kUnordered);
-    profile_collect();
-    if (st) raise_status(st);
+    const DecodeSource m{static_cast<uint8_t*>(w.reserve(decode_scratch_bytes(g.n, sizeof(T)))), w.result,
+                         w.host_result, &w, nullptr};
+    decompress_with<T>(archive, archive_bytes, d, output, s, canonical_chunk, m);
+}
+
+template <class T>
+void decompress_into(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s,
+                     bool canonical_chunk, const DecodeMemory& mem) {
+    if (reinterpret_cast<uintptr_t>(archive) % 8 || reinterpret_cast<uintptr_t>(output) % sizeof(T))
+        throw std::invalid_argument("archive must be 8-byte aligned and output naturally aligned");
+    const Geometry g = geometry(d, sizeof(T));
+    if (decode_scratch_bytes(g.n, sizeof(T)) > mem.scratch_bytes)
+        throw std::invalid_argument("field exceeds the decoder's fixed scratch");
+    EncodeResult host{};
+    const DecodeSource m{mem.scratch, mem.result, &host, nullptr, mem.rank};
+    decompress_with<T>(archive, archive_bytes, d, output, s, canonical_chunk, m);
 }
 
 template uint64_t compress<float>(const float*, Dims, double, uint8_t*, uint64_t, hipStream_t);
 template uint64_t compress<double>(const double*, Dims, double, uint8_t*, uint64_t, hipStream_t);
-template void decompress<float>(const uint8_t*, uint64_t, Dims, float*, hipStream_t);
-template void decompress<double>(const uint8_t*, uint64_t, Dims, double*, hipStream_t);
+template void decompress<float>(const uint8_t*, uint64_t, Dims, float*, hipStream_t, bool);
+template void decompress<double>(const uint8_t*, uint64_t, Dims, double*, hipStream_t, bool);
+template void decompress_into<float>(const uint8_t*, uint64_t, Dims, float*, hipStream_t, bool, const DecodeMemory&);
+template void decompress_into<double>(const uint8_t*, uint64_t, Dims, double*, hipStream_t, bool,
+                                      const DecodeMemory&);
 
 }  // namespace wz

@@ -104,6 +104,10 @@ inline uint64_t wzp_staging_bytes(uint32_t chunks) { return uint64_t(chunks) * k
 // `capacity` (bytes available from out). Total bytes -> *total (device u64).
 void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity,
                 WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s);
+// Checks the group directory of a WZP stream of `bytes` (>= its header)
+// bytes: spans inside the payload and tiling it exactly, else kCorrupt.
+// wzp_decode and decode_dequantize_pairs run it first.
+void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s);
 // Decodes a WZP stream of `bytes` bytes into packed mag/sign arrays.
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
                 hipStream_t s);
@@ -152,8 +156,27 @@ void zero_words(void* p, uint64_t words, hipStream_t s);
 // Throw std::invalid_argument / std::runtime_error with the reference's messages.
 template <class T>
 uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, uint64_t capacity, hipStream_t s);
+// archive_bytes = ~0: size taken from the header. canonical_chunk: the header
+// must be exactly the one compress writes for this geometry and type, and the
+// archive exactly archive_bytes long (the chunk decoder's rule).
 template <class T>
-void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s);
+void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s,
+                bool canonical_chunk = false);
+// Decode with memory the caller supplies (no device allocation inside):
+// `scratch` holds scratch_bytes >= decode_scratch_bytes(n, width) bytes,
+// 256-byte aligned; `rank` 2 * 8 * 8192 u16 (rebuilt by every call that
+// needs them); `result` one device EncodeResult. Same checks and results as
+// decompress.
+struct DecodeMemory {
+    uint8_t* scratch;
+    uint64_t scratch_bytes;
+    uint16_t* rank;
+    EncodeResult* result;
+};
+uint64_t decode_scratch_bytes(uint32_t n, unsigned width);
+template <class T>
+void decompress_into(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s,
+                     bool canonical_chunk, const DecodeMemory& mem);
 // Archive bytes implied by a 32-byte header (validated), or throws.
 uint64_t archive_size_from_header(const uint8_t* header, Dims d, unsigned width);
 uint64_t max_compressed_bytes(Dims d, unsigned width);

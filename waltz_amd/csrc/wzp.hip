@@ -579,10 +579,29 @@ __global__ void __launch_bounds__(256) wzp_decode_kernel(const uint8_t* __restri
     decode_chunk(in, bytes, n, chunks, groups, blockIdx.x, sink, status);
 }
 
+__global__ void __launch_bounds__(kDirThreads) wzp_directory_kernel(const uint8_t* __restrict__ in, uint64_t bytes,
+                                                                   uint32_t n, uint32_t chunks, uint32_t groups,
+                                                                   uint32_t* __restrict__ status) {
+    __shared__ uint32_t len[kMaxGroups];
+    __shared__ uint32_t sh[kDirThreads / 64];
+    check_directory(in, bytes, n, chunks, groups, len, sh, status);
+}
+
+// The stream's directory check runs beside its chunk decoders (both only
+// read the stream and report into `status`). bytes >= the stream header.
+void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s) {
+    const WzpShape ws = wzp_shape(n);
+    if (ws.chunks == 0) return;
+    if (bytes < ws.header) throw std::runtime_error("WZP archive size is outside its valid bounds");
+    hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, s, in, bytes, n, ws.chunks, ws.groups,
+                       status);
+}
+
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
                 hipStream_t s) {
     const WzpShape ws = wzp_shape(n);
     if (ws.chunks == 0) return;
+    wzp_check_directory(in, bytes, n, status, s);
     hipLaunchKernelGGL(wzp_decode_kernel<MagSink>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n, ws.chunks,
                        ws.groups, MagSink{mag, sign}, status);
 }
@@ -593,6 +612,7 @@ void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double q
     if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
     const uint32_t n = d.nx * d.ny * d.nz;
     const WzpShape ws = wzp_shape(n);
+    wzp_check_directory(in, bytes, n, status, s);
     hipLaunchKernelGGL(wzp_decode_kernel<BrickSink<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n,
                        ws.chunks, ws.groups, BrickSink<T>{coeff, d.nx, d.ny, T(quantum)}, status);
 }

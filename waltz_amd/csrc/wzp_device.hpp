@@ -372,4 +372,90 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
     sink.store(c, t, E, m, expand_bits(bits, nzmask));
 }
 
+// Groups of the largest stream (n < 2^31 samples, 8192 per chunk, 16 chunks per group).
+constexpr uint32_t kMaxGroups = 16384;
+constexpr uint32_t kDirThreads = 1024;
+
+// Exclusive scan over a workgroup of kDirThreads threads; `sh` holds 16 words.
+WZ_D uint32_t dir_exclusive(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const uint32_t inc = wave_inclusive(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 63u) sh[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kDirThreads / 64; ++i) {
+        const uint32_t s = sh[i];
+        before += i < w ? s : 0u;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return before + inc - v;
+}
+
+// Directory of one WZP stream of `bytes` >= header bytes, checked by one
+// workgroup of kDirThreads threads (cpu/src/wzp_cpu.cpp:713-761): each group's
+// span [offset, offset + length) lies inside the payload, an empty group has
+// offset 0, and the non-empty spans tile the payload exactly (no overlap, gap,
+// trailing or missing bytes). Lengths follow from the chunk metas. The encoder
+// writes the spans in group order (offset = sum of the earlier lengths), which
+// one scan confirms; any other order is checked by ranks: the starts must be
+// distinct and each span must start at the summed length of the spans that
+// start before it. `len` is kMaxGroups words of LDS.
+WZ_D void check_directory(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t n, uint32_t chunks,
+                          uint32_t groups, uint32_t* len, uint32_t* sh, uint32_t* __restrict__ status) {
+    const uint32_t t = threadIdx.x;
+    const uint64_t header = uint64_t(groups) * 4 + uint64_t(chunks) * 3;
+    const uint64_t payload = bytes - header;
+    const uint8_t* meta = in + uint64_t(groups) * 4;
+    if (payload > 0xFFFFFFFFull || groups > kMaxGroups) {  // uniform: no workgroup barrier follows
+        if (t == 0) atomicOr(status, uint32_t(kCorrupt));
+        return;
+    }
+    int bad = 0, chain = 1;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < groups; base += kDirThreads) {
+        const uint32_t g = base + t;
+        uint32_t L = 0, O = 0;
+        if (g < groups) {
+            for (uint32_t j = 0; j < kGroup; ++j) {
+                const uint32_t c = g * kGroup + j;
+                if (c >= chunks) break;
+                const uint32_t mc = load24(meta + uint64_t(c) * 3);
+                const uint32_t code = mc & kRawCode, sc = mc >> 14;
+                if (code) L += (code == kRawCode ? 2 * chunk_elements(n, c) : code) + (sc ? sc : kSignSlot);
+            }
+            O = load32(in + uint64_t(g) * 4);
+            len[g] = L;
+            if (O > payload || L > payload - O || (L == 0 && O != 0)) bad = 1;
+        }
+        uint32_t round_total;
+        const uint32_t ex = dir_exclusive(L, sh, round_total);
+        if (L && uint64_t(O) != carry + ex) chain = 0;
+        carry += round_total;
+    }
+    if (carry != payload) bad = 1;
+    bad = __syncthreads_or(bad);
+    chain = __syncthreads_and(chain);
+    if (!bad && !chain) {
+        for (uint32_t g = t; g < groups; g += kDirThreads) {
+            const uint32_t L = len[g];
+            if (!L) continue;
+            const uint32_t O = load32(in + uint64_t(g) * 4);
+            uint64_t before = 0;
+            for (uint32_t h = 0; h < groups; ++h) {
+                const uint32_t Lh = len[h];
+                if (!Lh || h == g) continue;
+                const uint32_t Oh = load32(in + uint64_t(h) * 4);
+                if (Oh == O) bad = 1;
+                before += Oh < O ? Lh : 0u;
+            }
+            if (before != O) bad = 1;
+        }
+        bad = __syncthreads_or(bad);
+    }
+    if (bad && t == 0) atomicOr(status, uint32_t(kCorrupt));
+}
+
 }  // namespace wz
