@@ -37,6 +37,18 @@ int waltz_hip_decode(waltz_hip_session*, const waltz_hip_config*, const void* ar
 const char* waltz_cuda_decode_chunk(uint32_t nx, uint32_t ny, uint32_t nz, int width, const void* archive,
                                     size_t archive_bytes, void* output, size_t output_bytes, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc): waltz_cuda_decode_chunk with the same arguments, rules and
+ * results, whose device scratch is one fixed array inside the library (about
+ * 26.4 MiB per device, held until the process ends) instead of a workspace
+ * grown with hipMalloc. Chunks of at most WALTZ_NOALLOC_MAX_SAMPLES samples;
+ * larger ones are rejected with a diagnostic. Calls share the array: they are
+ * serialised within a process, and each process has its own copy. */
+#define WALTZ_NOALLOC_MAX_SAMPLES 1048576u
+const char* waltz_cuda_decode_chunk_noalloc(uint32_t nx, uint32_t ny, uint32_t nz, int width, const void* archive,
+                                            size_t archive_bytes, void* output, size_t output_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
