@@ -38,8 +38,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs):
-    """Algorithmic HBM bytes of one pipeline stage (minimum dataflow)."""
+def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, requant):
+    """Algorithmic HBM bytes of one pipeline stage (minimum dataflow).
+
+    requant: the finest inverse level requantizes its high bands on load, so
+    the quantizing encoder writes dequantized coefficients back only inside
+    the coarser levels' box (n/8 samples)."""
     if name == "inverse_L0_repairs":
         # coefficients (7 bands + the coarser box) and the original field read,
         # flag bitmap and repair records written
@@ -51,7 +55,10 @@ def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs):
     table = {
         "quantize": 2 * w * n + 2 * n + n / 8,      # read coeff, write dequant, mags, signs
         "wzp_encode": 2 * n + n / 8 + wzp_bytes,    # read mags+signs, write stream
-        "quantize_encode": 2 * w * n + wzp_bytes,   # read coeff, write dequant + stream
+        # read coeff, write the dequantized coefficients (all, or the coarser
+        # levels' box) and the chunk payloads into their staging slots
+        "quantize_encode": w * n + (w * n / 8 if requant else w * n) + wzp_bytes,
+        "wzp_pack": 2 * wzp_bytes,                   # payloads from the staging slots into the archive
         "repairs": 2 * w * n,                        # read reconstruction + original
         "wzp_decode": wzp_bytes + 2 * n + n / 8,    # read stream, write mags+signs
         "dequantize": 2 * n + n / 8 + w * n,         # read mags+signs, write coefficients
@@ -60,19 +67,24 @@ def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs):
     return table.get(name)
 
 
+def batched_shapes():
+    import numpy as np
+    rng = np.random.default_rng(20261202 + 4)
+    kinds = [(32, 32, 32), (32, 64, 64), (48, 64, 96), (64, 128, 128)]  # (z, y, x)
+    return [kinds[k] for k in rng.integers(0, len(kinds), 256)]
+
+
 def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps):
     """256 float32 chunks of uneven shape (seeded mix of 32^3, 64x64x32,
     96x64x48, 128x128x64; each its own k^-5/3 field), ABS 1e-3: decoded one
     archive per call through the plain API, then in one batched pass (eager
     and replayed as a captured graph) straight into one contiguous tensor."""
-    import numpy as np
     from waltz_amd import batch
-    rng = np.random.default_rng(20261202 + 4)
-    kinds = [(32, 32, 32), (32, 64, 64), (48, 64, 96), (64, 128, 128)]  # (z, y, x)
-    shapes = [kinds[k] for k in rng.integers(0, len(kinds), 256)]
-    archives = []
+    shapes = batched_shapes()
+    archives, fields = [], []
     for j, s in enumerate(shapes):
         f = synthetic.grf(s, slope=-5.0 / 3.0, seed=20261202 + 4 + 1000 * (j + 1))
+        fields.append(f)
         archives.append(waltz_amd.compress(f, abs_error=1e-3).clone())
     packed, spans = batch.pack(archives)
     dec = batch.BatchDecoder(shapes, torch.float32)
@@ -113,11 +125,13 @@ def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps):
     nbytes = dec.numel * 4
     g_batch, g_graph, g_plain = nbytes / t_batch / 1e9, nbytes / t_graph / 1e9, nbytes / t_plain / 1e9
     return {"chunks": len(shapes), "samples": dec.numel, "archive_bytes": int(packed.numel()),
+            "bound": "ABS 1e-3",
             "per_chunk_calls_GBps": round(g_plain, 2), "batched_GBps": round(g_batch, 2),
             "graph_replay_GBps": round(g_graph, 2), "batched_over_per_chunk": round(g_batch / g_plain, 2),
             "graph_over_per_chunk": round(g_graph / g_plain, 2),
+            "single_field_decompress_GBps_same_bound": round(single_field_decompress_gbps, 2),
             "graph_over_single_field_decompress": round(g_graph / single_field_decompress_gbps, 3),
-            "bit_identical_to_per_chunk_decode": identical}
+            "bit_identical_to_per_chunk_decode": identical}, [f.cpu().numpy() for f in fields]
 
 
 def autograd_leg(torch, waltz_amd, synthetic):
@@ -172,7 +186,124 @@ def autograd_leg(torch, waltz_amd, synthetic):
             "forward_backward_over_plain_decode": round(t_dec / t_step, 3)}
 
 
+def cpu_batched_all_cores(pyoracle, fields):
+    """The batched workload's 256 chunks through the reference CPU codec
+    (AVX2 backend) on every host core: independent chunks over a thread pool,
+    one codec workspace per thread (a workspace is not shareable)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    local = threading.local()
+
+    def codec():
+        if not hasattr(local, "ref"):
+            local.ref = pyoracle.Reference()
+        return local.ref
+
+    def comp(f):
+        r = codec()
+        return r.compress(f, 1e-3, r.AVX2 if r.has_avx2 else r.SCALAR)
+
+    def decomp(args):
+        a, f = args
+        r = codec()
+        return r.decompress(a, f.shape, f.dtype, r.AVX2 if r.has_avx2 else r.SCALAR)
+
+    nbytes = sum(f.nbytes for f in fields)
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(comp, fields[:threads]))  # one workspace per thread, warm
+        t = time.perf_counter()
+        archives = list(ex.map(comp, fields))
+        tc = time.perf_counter() - t
+        t = time.perf_counter()
+        list(ex.map(decomp, zip(archives, fields)))
+        td = time.perf_counter() - t
+    return {"threads": threads, "compress_GBps": round(nbytes / tc / 1e9, 3),
+            "decompress_GBps": round(nbytes / td / 1e9, 3),
+            "sample": "the 256 chunks of the batched workload, ABS 1e-3, reference CPU codec, one pass"}
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "codec_512cube_f32_rel1e-3_traffic.md"
+KERNELS_FILE = ROOT / "profiles" / "codec_512cube_f32_rel1e-3_kernels.md"
+# the kernel each timed stage launches (as rocprofv3 names it)
+STAGE_KERNEL = {
+    "inverse_L0_repairs": "wz::inv_level_kernel<float, 2, true, true>",
+    "inverse_L0": "wz::inv_level_kernel<float, 1, false, true>",
+    "forward_L0": "wz::fwd_level_kernel<float>",
+    "decode_dequantize": "wz::wzp_decode_kernel<wz::BrickSink<float> >",
+    "quantize_encode": "wz::wzp_encode_kernel<wz::BrickSource<float> >",
+}
+
+
+def rocprof_duration_ms(stage):
+    """Median duration (ms) of a stage's kernel in the committed rocprofv3
+    kernel table (largest entry of that kernel: the finest level), or None."""
+    name = STAGE_KERNEL.get(stage)
+    best = None
+    try:
+        for line in KERNELS_FILE.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[1] == f"`{name}`":
+                us = float(cells[3])
+                best = us if best is None else max(best, us)
+    except (OSError, ValueError):
+        return None
+    return None if best is None else best / 1e3
+
+
+def flush_caches(torch, dev):
+    """Evict L2 and the Infinity Cache: write 1 GiB that the codec never reads."""
+    buf = torch.empty(256 << 20, dtype=torch.float32, device=dev)
+    buf.fill_(1.0)
+    torch.cuda.synchronize()
+    del buf
+
+
+def extra_config(torch, waltz_amd, field, rel, label, ref_cpu):
+    """Compress / decompress GB/s of one more workload at a REL bound, its %
+    of HBM peak on the algorithmic-bytes model and byte identity with the
+    reference CPU codec (when it is built)."""
+    import numpy as np
+    from waltz_amd import autograd, stages as stage_api
+    shape = tuple(field.shape)
+    n = field.numel()
+    w = field.element_size()
+    lo, hi = float(field.min()), float(field.max())
+    bound = rel * (hi - lo)
+    stage_api.autotune(shape, field.dtype)
+    archive = waltz_amd.compress(field, abs_error=bound)
+    out = waltz_amd.decompress(archive, shape=shape, dtype=field.dtype)
+    tc, td = [], []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        archive = waltz_amd.compress(field, abs_error=bound)
+        b = time.perf_counter()
+        waltz_amd.decompress(archive, shape=shape, dtype=field.dtype, out=out)
+        c = time.perf_counter()
+        tc.append(b - a)
+        td.append(c - b)
+    err = float((out.double() - field.double()).abs().max())
+    A = archive.numel()
+    lxy, lz, dyadic = autograd.default_levels(shape)
+    if dyadic:
+        coarse = sum(8.0 ** -l for l in range(1, lxy))
+    else:
+        coarse = 1 + sum(2.0 ** -l for l in range(1, lz)) + sum(4.0 ** -l for l in range(1, lxy))
+    alg_c = (4 * w + 4.25) * n + 4 * w * n * coarse + A
+    alg_d = 3 * w * n + 2 * w * n * coarse + A
+    t_c, t_d = statistics.median(tc), statistics.median(td)
+    res = {"workload": label, "shape_zyx": list(shape), "dtype": str(field.dtype).replace("torch.", ""),
+           "levels": [lxy, lz, "dyadic" if dyadic else "plane"], "abs_bound": bound,
+           "compress_GBps": round(n * w / t_c / 1e9, 2), "decompress_GBps": round(n * w / t_d / 1e9, 2),
+           "compress_pct_hbm_alg": round(100 * alg_c / t_c / HBM_PEAK, 2),
+           "decompress_pct_hbm_alg": round(100 * alg_d / t_d / HBM_PEAK, 2),
+           "archive_bytes": A, "max_abs_error_fp64": err, "bound_held": err <= bound}
+    if ref_cpu is not None:
+        host = field.cpu().numpy()
+        cpu = ref_cpu.compress(host, bound, ref_cpu.AVX2 if ref_cpu.has_avx2 else ref_cpu.SCALAR)
+        res["byte_identical_vs_reference_cpu"] = cpu == archive.cpu().numpy().tobytes()
+    return res
 
 
 def measured_traffic(stage):
@@ -287,8 +418,9 @@ def main():
     wzp_bytes = int.from_bytes(header[16:24], "little")
     np_repairs = int.from_bytes(header[24:28], "little")
     rows = []
+    requant = "inverse_L0_repairs" in stages and "quantize_encode" in stages
     for name, (ms, cnt) in stages.items():
-        ab = stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs)
+        ab = stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, requant)
         rows.append((ms / max(cnt, 1), name, ab))
     rows.sort(reverse=True)
     dom_ms, dom_name, dom_bytes = next(r for r in rows if r[2])
@@ -296,6 +428,10 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": measured_traffic(dom_name),
                 "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_ms, 4)}
+    rp = rocprof_duration_ms(dom_name)
+    if rp:  # the committed rocprofv3 kernel table's duration of the same kernel
+        roofline["rocprof_launch_ms"] = round(rp, 4)
+        roofline["frac_rocprof"] = round(dom_bytes / (rp * 1e-3) / HBM_PEAK, 4)
     stage_table = {name: {"ms": round(ms, 4), "alg_GBps": round(ab / (ms * 1e-3) / 1e9, 1) if ab else None}
                    for ms, name, ab in rows}
     log("stages:", json.dumps(stage_table))
@@ -314,6 +450,30 @@ def main():
     arch2_host = arch2.cpu().numpy().tobytes()
     max_err2 = float((out.double() - x.double()).abs().max())
     assert max_err2 <= abs2
+
+    # cold caches: L2 and the Infinity Cache evicted before every call
+    tcc, tcd = [], []
+    for _ in range(3):
+        flush_caches(torch, dev)
+        a = time.perf_counter()
+        archive = compress(bound)
+        b = time.perf_counter()
+        tcc.append(b - a)
+        flush_caches(torch, dev)
+        b = time.perf_counter()
+        decompress(archive)
+        tcd.append(time.perf_counter() - b)
+
+    # single-field decode at the batched workload's bound (ABS 1e-3)
+    arch3 = compress(1e-3)
+    t3 = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        decompress(arch3)
+        t3.append(time.perf_counter() - a)
+    d_gbps_abs1e3 = field_bytes / statistics.median(t3) / 1e9
+    del arch3
 
     # pre-quantization coefficients in the input's precision vs an FP64 transform
     from waltz_amd import autograd
@@ -349,16 +509,36 @@ def main():
         "abs_1e-4": {"compress_GBps": round(field_bytes / statistics.median(t2c) / 1e9, 2),
                      "decompress_GBps": round(field_bytes / statistics.median(t2d) / 1e9, 2),
                      "archive_bytes": len(arch2_host), "max_abs_error_fp64": max_err2},
+        "cold_cache": {"compress_GBps": round(field_bytes / statistics.median(tcc) / 1e9, 2),
+                       "decompress_GBps": round(field_bytes / statistics.median(tcd) / 1e9, 2),
+                       "note": "1 GiB written before every call (L2 and Infinity Cache evicted); the headline "
+                               "figures are warm medians"},
         "stages_ms": stage_table, "roofline": roofline, "tuning": tuning,
     }
 
+    batch_host = None
     if not args.no_extra:
         log("batched leg ...")
-        result["batched"] = batched_leg(torch, waltz_amd, synthetic, d_gbps)
+        result["batched"], batch_host = batched_leg(torch, waltz_amd, synthetic, d_gbps_abs1e3)
         log("batched:", json.dumps(result["batched"]))
         log("autograd leg ...")
         result["autograd"] = autograd_leg(torch, waltz_amd, synthetic)
         log("autograd:", json.dumps(result["autograd"]))
+        ref_cpu = None
+        if not args.no_cpu:
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import pyoracle
+            ref_cpu = pyoracle.reference_or_none()
+        log("more workloads ...")
+        result["more_workloads"] = [
+            extra_config(torch, waltz_amd, synthetic.fronts((96, 384, 384), seed=20261202 + 2, device=dev), 1e-6,
+                         "384x384x96 float64 tanh fronts + k^-3 noise (seed 20261204), REL 1e-6", ref_cpu),
+            extra_config(torch, waltz_amd, synthetic.grf((256, 256, 256), slope=-5.0 / 3.0, seed=20261202 + 5,
+                                                         device=dev), 1e-3,
+                         "256^3 float32 k^-5/3 Gaussian random field (seed 20261207), REL 1e-3", ref_cpu),
+        ]
+        log("more workloads:", json.dumps(result["more_workloads"]))
+        torch.cuda.empty_cache()
 
     # CPU leg: byte identity with the reference CPU codec + its timing on this host
     if not args.no_cpu:
@@ -366,6 +546,9 @@ def main():
         import pyoracle
         xh = x.cpu().numpy()
         ref = pyoracle.reference_or_none()
+        all_cores = None
+        if ref is not None and batch_host is not None:
+            all_cores = cpu_batched_all_cores(pyoracle, batch_host)
         try:
             os.sched_setaffinity(0, {min(os.sched_getaffinity(0))})
         except (AttributeError, OSError):
@@ -390,13 +573,26 @@ def main():
                 ref.decompress(cpu_arch, shape, np.float32, backend)
                 reps_d.append(time.perf_counter() - t)
             cc = field_bytes / statistics.median(reps_c) / 1e9
+            # the scalar backend (the canonical arithmetic, same bytes) on a 256^3 corner
+            sub = np.ascontiguousarray(xh[:256, :256, :256])
+            t = time.perf_counter()
+            sub_arch = ref.compress(sub, bound, ref.SCALAR)
+            ts_c = time.perf_counter() - t
+            t = time.perf_counter()
+            ref.decompress(sub_arch, sub.shape, np.float32, ref.SCALAR)
+            ts_d = time.perf_counter() - t
             result["cpu_baseline"] = {
                 "value": round(cc, 4), "unit": "GB/s", "cores": 1, "kind": "reference",
                 "sample": f"whole {S}^3 f32 field, REL 1e-3, reference CPU codec "
                           f"({'AVX2/FMA' if backend == ref.AVX2 else 'scalar'} backend, single-threaded, "
                           f"pinned to 1 core), median of 3 compress calls",
                 "decompress_GBps": round(field_bytes / statistics.median(reps_d) / 1e9, 4),
-                "gpu_over_cpu_compress": round(c_gbps / cc, 1)}
+                "gpu_over_cpu_compress": round(c_gbps / cc, 1),
+                "scalar_backend": {"compress_GBps": round(sub.nbytes / ts_c / 1e9, 4),
+                                   "decompress_GBps": round(sub.nbytes / ts_d / 1e9, 4), "cores": 1,
+                                   "sample": "256^3 corner of the same field, same bound, one call each"}}
+            if all_cores:
+                result["cpu_baseline"]["batched_all_cores"] = all_cores
         else:
             o = pyoracle.Oracle()
             sub = np.ascontiguousarray(xh[:128, :128, :128])

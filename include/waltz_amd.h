@@ -40,6 +40,17 @@ int waltz_amd_transform(int width, int inverse, const void* input, void* output,
 int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, void* output, uint32_t nx, uint32_t ny,
                                 uint32_t nz, unsigned levels_xy, unsigned levels_z, int dyadic, void* stream);
 
+/* Asynchronous forms of the two calls above (no reference counterpart), for
+ * autograd and stream capture: the caller passes the two scratch volumes
+ * tmp0/tmp1 (nx*ny*nz elements each, distinct from input and output) and the
+ * call only enqueues work on `stream`: no synchronisation, allocation or lock. */
+int waltz_amd_transform_async(int width, int inverse, const void* input, void* output, void* tmp0, void* tmp1,
+                              uint32_t nx, uint32_t ny, uint32_t nz, unsigned levels_xy, unsigned levels_z, int dyadic,
+                              void* stream);
+int waltz_amd_transform_adjoint_async(int width, int of_inverse, const void* input, void* output, void* tmp0,
+                                      void* tmp1, uint32_t nx, uint32_t ny, uint32_t nz, unsigned levels_xy,
+                                      unsigned levels_z, int dyadic, void* stream);
+
 /* Bit-neutral launch tuning of the dyadic schedule (no reference
  * counterpart): autotune times the fused level kernels of both directions at
  * each candidate z segment for this geometry and type on this device and keeps

@@ -135,3 +135,57 @@ def test_dense_transpose_small(wz, inverse):
     adj = torch.stack([stages.transform_adjoint(eye[i].view(shape), lxy, lz, dyadic, of_inverse=inverse).reshape(-1)
                        for i in range(n)], dim=1)
     assert float((adj - fwd.T).abs().max()) <= 1e-13 * float(fwd.abs().max())
+
+
+def test_async_and_sync_entry_points_agree(wz):
+    """waltz_amd_transform / _adjoint (synchronous, library scratch) and their
+    _async forms (caller scratch, no synchronisation) give the same bits."""
+    import ctypes as C
+    from waltz_amd._lib import lib
+    stages, _ = wz
+    shape, lxy, lz, dyadic = CASES[3]
+    nz, ny, nx = shape
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.randn(shape, dtype=torch.float32, device="cuda", generator=g)
+    for inverse in (0, 1):
+        out = torch.empty_like(x)
+        assert lib().waltz_amd_transform(4, inverse, x.data_ptr(), out.data_ptr(), nx, ny, nz, lxy, lz, int(dyadic),
+                                         None) == 0
+        assert torch.equal(out, stages.transform(x, lxy, lz, dyadic, inverse=bool(inverse)))
+        assert lib().waltz_amd_transform_adjoint(4, inverse, x.data_ptr(), out.data_ptr(), nx, ny, nz, lxy, lz,
+                                                 int(dyadic), None) == 0
+        assert torch.equal(out, stages.transform_adjoint(x, lxy, lz, dyadic, of_inverse=bool(inverse)))
+
+
+def test_autograd_step_replays_from_a_captured_graph(wz):
+    """IDWT forward and its backward through the adjoint (for a fixed output
+    gradient, so no reduction is captured) recorded in a graph: every replay,
+    with eager work between, equals the eager step bit for bit."""
+    _, autograd = wz
+    shape = (32, 48, 40)
+    g = torch.Generator(device="cuda").manual_seed(12)
+    c0 = torch.randn(shape, dtype=torch.float32, device="cuda", generator=g)
+    gout = torch.randn(shape, dtype=torch.float32, device="cuda", generator=g)
+    c = c0.clone().requires_grad_()
+    x = autograd.idwt(c)
+    x.backward(gout)
+    want_x, want_grad = x.detach().clone(), c.grad.clone()
+    cs = c0.clone().requires_grad_()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):  # warm-up on the capture stream
+        autograd.idwt(cs).backward(gout)
+    torch.cuda.current_stream().wait_stream(stream)
+    graph = torch.cuda.CUDAGraph()
+    cs.grad = None
+    with torch.cuda.graph(graph, stream=stream):
+        gx = autograd.idwt(cs)
+        gx.backward(gout)
+    for _ in range(3):
+        cs.grad.fill_(float("nan"))
+        gx.detach().fill_(float("nan"))
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(gx.detach(), want_x)
+        assert torch.equal(cs.grad, want_grad)
+        _ = autograd.idwt(c0) * 2.0  # eager work between the replays

@@ -156,3 +156,22 @@ def test_autograd_finite_differences_on_crop(wz, grf256x8):
         # the loss is quadratic in q: central differences are exact up to
         # rounding (about 1e-16 of the loss over 2 eps, far below 1e-13)
         assert abs(fd - float(grad[i])) <= 1e-6 * abs(float(grad[i])) + 1e-13, (i, fd, float(grad[i]))
+
+
+def test_wzp_lookback_beyond_one_round(wz, oracle):
+    """A field of more than 1024 WZP groups (1024 x 16 x 8192 = 2^27 samples):
+    the group pass's decoupled lookback needs a second 1024-descriptor round.
+    The archive still equals the oracle's and decodes to its bits."""
+    waltz_amd, synthetic = wz
+    shape = (513, 512, 512)
+    n = shape[0] * shape[1] * shape[2]
+    groups = -(-(-(-n // 8192)) // 16)
+    assert groups > 1024
+    x = synthetic.grf(shape, slope=-5.0 / 3.0, seed=20261209, offset=1.0)
+    host = x.cpu().numpy()
+    _say(f"{shape} f32: {groups} WZP groups")
+    archive = waltz_amd.compress(x, abs_error=2e-3)
+    want = oracle.compress(host, 2e-3)
+    assert archive.cpu().numpy().tobytes() == want
+    out = waltz_amd.decompress(archive, shape=shape, dtype=torch.float32)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), oracle.decompress(want, shape, np.float32).view(np.uint32))

@@ -54,6 +54,10 @@ _SIGS.update({
     "waltz_amd_batch_decode": (C.c_int, [_P, _P, _SZ, _P, _P, _P, _SZ, _P]),
     "waltz_amd_batch_check": (C.c_int, [_P, _P]),
     "waltz_amd_transform_adjoint": (C.c_int, [C.c_int, C.c_int, _P, _P, _U32, _U32, _U32, _U, _U, C.c_int, _P]),
+    "waltz_amd_transform_async": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _U32, _U32, _U32, _U, _U, C.c_int,
+                                            _P]),
+    "waltz_amd_transform_adjoint_async": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _U32, _U32, _U32, _U, _U,
+                                                    C.c_int, _P]),
     "waltz_amd_autotune": (C.c_int, [C.c_int, _U32, _U32, _U32, _P]),
     "waltz_amd_set_tuning": (C.c_int, [C.c_int, _U32, _U32, _U32, C.c_int, _U, C.POINTER(_U32)]),
     "waltz_amd_clear_tuning": (None, []),

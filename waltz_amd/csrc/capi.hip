@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 #include <mutex>
 
@@ -304,6 +305,57 @@ int waltz_amd_transform_adjoint(int width, int of_inverse, const void* input, vo
                                           reinterpret_cast<double*>(t1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
         wz::hip_check(hipGetLastError(), "adjoint launch");
         wz::hip_check(hipStreamSynchronize(s), "transform adjoint");
+    });
+}
+
+int waltz_amd_transform_async(int width, int inverse, const void* input, void* output, void* tmp0, void* tmp1,
+                              uint32_t nx, uint32_t ny, uint32_t nz, unsigned lxy, unsigned lz, int dyadic,
+                              void* stream) {
+    return guarded([&] {
+        if (lxy > 5 || lz > 5 || (dyadic && lxy != lz)) throw std::invalid_argument("invalid transform levels");
+        if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
+        if (!input || !output || !tmp0 || !tmp1) throw std::invalid_argument("null argument");
+        const wz::Dims d = dims_of(nx, ny, nz);
+        const uint64_t n = uint64_t(nx) * ny * nz;
+        hipStream_t s = as_stream(stream);
+        auto run = [&](auto* out) {
+            using T = std::remove_pointer_t<decltype(out)>;
+            T* t0 = static_cast<T*>(tmp0);
+            T* t1 = static_cast<T*>(tmp1);
+            if (inverse) {
+                if (input != output)
+                    wz::hip_check(hipMemcpyAsync(out, input, n * sizeof(T), hipMemcpyDeviceToDevice, s), "copy");
+                wz::transform_inverse<T>(out, out, t0, t1, d, lxy, lz, dyadic != 0, s);
+            } else {
+                wz::transform_forward<T>(static_cast<const T*>(input), out, t0, t1, d, lxy, lz, dyadic != 0, nullptr,
+                                         s);
+            }
+        };
+        if (width == 4) run(static_cast<float*>(output));
+        else run(static_cast<double*>(output));
+        wz::hip_check(hipGetLastError(), "transform launch");
+    });
+}
+
+int waltz_amd_transform_adjoint_async(int width, int of_inverse, const void* input, void* output, void* tmp0,
+                                      void* tmp1, uint32_t nx, uint32_t ny, uint32_t nz, unsigned lxy, unsigned lz,
+                                      int dyadic, void* stream) {
+    return guarded([&] {
+        if (lxy > 5 || lz > 5 || (dyadic && lxy != lz)) throw std::invalid_argument("invalid transform levels");
+        if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
+        if (!input || !output || !tmp0 || !tmp1) throw std::invalid_argument("null argument");
+        const wz::Dims d = dims_of(nx, ny, nz);
+        const uint64_t n = uint64_t(nx) * ny * nz;
+        hipStream_t s = as_stream(stream);
+        if (input != output)
+            wz::hip_check(hipMemcpyAsync(output, input, n * size_t(width), hipMemcpyDeviceToDevice, s), "copy");
+        if (width == 4)
+            wz::transform_adjoint<float>(static_cast<float*>(output), static_cast<float*>(tmp0),
+                                         static_cast<float*>(tmp1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
+        else
+            wz::transform_adjoint<double>(static_cast<double*>(output), static_cast<double*>(tmp0),
+                                          static_cast<double*>(tmp1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
+        wz::hip_check(hipGetLastError(), "adjoint launch");
     });
 }
 

@@ -56,6 +56,7 @@ struct DeviceWorkspace {
     uint32_t rank_bz = 0;
     EncodeResult* result = nullptr;      // device
     EncodeResult* host_result = nullptr;  // pinned
+    SideStream side{};                    // the WZP group pass beside the inverse levels (default priority)
 
     ~DeviceWorkspace() = default;  // process-lifetime cache
 
@@ -73,6 +74,9 @@ struct DeviceWorkspace {
             hip_check(hipMalloc(reinterpret_cast<void**>(&rank), 2 * 8 * 8192 * sizeof(uint16_t)), "rank tables");
             hip_check(hipMalloc(reinterpret_cast<void**>(&result), sizeof(EncodeResult)), "result");
             hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_result), sizeof(EncodeResult)), "pinned result");
+            hip_check(hipStreamCreateWithFlags(&side.stream, hipStreamNonBlocking), "side stream");
+            hip_check(hipEventCreateWithFlags(&side.ready, hipEventDisableTiming), "event");
+            hip_check(hipEventCreateWithFlags(&side.done, hipEventDisableTiming), "event");
         }
     }
     const uint16_t* ranks(Dims d, uint32_t bz, hipStream_t s) {
@@ -144,6 +148,15 @@ Geometry geometry(Dims d, unsigned width) {
 }
 
 uint32_t qcap(uint32_t n) { return n <= (1u << 20) ? n : (1u << 20); }
+
+// An early exit (exception) must not leave side-stream work of this call
+// running into the next one.
+struct SideJoin {
+    const SideStream* sd;
+    ~SideJoin() {
+        if (sd) (void)hipStreamSynchronize(sd->stream);
+    }
+};
 
 // Caller buffers must not overlap (cpu/src/codec.cpp:82-86).
 void check_disjoint(const void* a, uint64_t as, const void* b, uint64_t bs) {
@@ -240,10 +253,16 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     // with the fused repair emitter on brick pairs the finest level's high
     // bands stay unquantized in coeff; that kernel requantizes them on load
     const bool requant = fsegs && pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
+    // With a fused finest level the reconstruction buffer (which shares the
+    // staging slots) is never written, so the group pass runs on the side
+    // stream beside the inverse levels, whose coarse passes leave most of the
+    // device idle; the archive sections wait for it.
+    const SideStream* side = fsegs ? &w.side : nullptr;
+    const SideJoin join{side};
     if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz)) {
         StageClock c("quantize_encode", s);
         quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,
-                                 status, s, requant);
+                                 status, s, requant, side);
     } else {
         const BrickGeometry bg = brick_geometry(g.d, g.bz);
         const uint16_t* rank = w.ranks(g.d, g.bz, s);
@@ -252,7 +271,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
             quantize_bricks<T>(p.coeff, bg, quantum, rank, p.mag, p.sign, qo, status, s);
         }
         StageClock c("wzp_encode", s);
-        wzp_encode(p.mag, p.sign, n, archive + 32, capacity - 32, wsc, &res->wzp_bytes, status, s);
+        wzp_encode(p.mag, p.sign, n, archive + 32, capacity - 32, wsc, &res->wzp_bytes, status, s, side);
     }
     // finest inverse level with native repair detection fused in when that
     // level runs as a fused kernel; otherwise reconstruct, then a repair pass
@@ -264,6 +283,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         StageClock c("repairs", s);
         detect_repairs<T>(input, p.rec, n, abs_bound, rp, status, s);
     }
+    if (side) hip_check(hipStreamWaitEvent(s, side->done, 0), "stream wait");
     const uint8_t flags = uint8_t((g.dyadic ? 1u : 0u) | (sizeof(T) == 8 ? 4u : 0u) | (3u << 3));
     {
         StageClock c("sections", s);
@@ -329,7 +349,7 @@ struct DecodeSource {
     uint8_t* scratch;
     EncodeResult* result;
     EncodeResult* host;
-    DeviceWorkspace* cache;  // cached rank tables, or nullptr: rebuilt into `rank`
+    DeviceWorkspace* cache;  // cached rank tables and the side stream, or nullptr: ranks rebuilt into `rank`
     uint16_t* rank;
     const uint16_t* ranks(Dims d, uint32_t bz, hipStream_t s) const {
         if (cache) return cache->ranks(d, bz, s);
@@ -375,13 +395,16 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
         check_unique(archive + L.pi, np, n, sign, status, s);
     }
     Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}, 0.0};
+    // the WZP directory check runs on the side stream beside the chunk decoders
+    const SideStream* side = m.cache ? &m.cache->side : nullptr;
+    const SideJoin join{side};
     if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz)) {
         StageClock ck("decode_dequantize", s);
-        decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s);
+        decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s, side);
     } else {
         {
             StageClock ck("wzp_decode", s);
-            wzp_decode(archive + 32, wzp, n, mag, sign, status, s);
+            wzp_decode(archive + 32, wzp, n, mag, sign, status, s, side);
         }
         const BrickGeometry bg = brick_geometry(g.d, bz);
         const uint16_t* rank = m.ranks(g.d, bz, s);
@@ -398,6 +421,7 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
         apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);  // screens patched samples
         if (!screened) check_finite<T>(output, n, status, s);
     }
+    if (side) hip_check(hipStreamWaitEvent(s, side->done, 0), "stream wait");
     hip_check(hipMemcpyAsync(m.host, m.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s), "status copy");
     hip_check(hipStreamSynchronize(s), "decompress");
     hip_check(hipGetLastError(), "decompress launch");

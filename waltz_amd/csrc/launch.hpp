@@ -98,29 +98,41 @@ struct WzpScratch {
     uint32_t* size;            // chunks payload sizes
     uint32_t* ticket;          // 1 counter (zeroed)
 };
+// A second stream for the WZP group pass, so that it runs beside the work
+// queued after the encoder on the caller's stream: the pass waits for `ready`
+// (recorded after the encoder) and records `done`, which the caller's stream
+// must wait for before it reads the stream's bytes or total.
+struct SideStream {
+    hipStream_t stream;
+    hipEvent_t ready, done;
+};
 // Staging of the encoder: kChunkSlot bytes per chunk plus read-ahead padding.
 inline uint64_t wzp_staging_bytes(uint32_t chunks) { return uint64_t(chunks) * kChunkSlot + 64; }
 // Encodes mag/sign into out (WZP stream at out[0..]); payload capacity bound
 // `capacity` (bytes available from out). Total bytes -> *total (device u64).
 void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity,
-                WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s);
+                WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
+                const SideStream* pack = nullptr);
 // Checks the group directory of a WZP stream of `bytes` (>= its header)
 // bytes: spans inside the payload and tiling it exactly, else kCorrupt.
-// wzp_decode and decode_dequantize_pairs run it first.
-void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s);
+// wzp_decode and decode_dequantize_pairs run it first, on `side` when given
+// (beside the chunk decoders: both only read the stream; the caller's stream
+// must wait for side->done before it reads the status).
+void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s,
+                         const SideStream* side = nullptr);
 // Decodes a WZP stream of `bytes` bytes into packed mag/sign arrays.
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
-                hipStream_t s);
+                hipStream_t s, const SideStream* side = nullptr);
 // Paired-brick geometry only (pair_bricks_fit): quantization fused into the
 // encoder (coefficients dequantized in place, q-outliers appended to `qo`),
 // and dequantization fused into the decoder (writes the coefficient volume).
 template <class T>
 void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
                            WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
-                           bool coarse_box_only = false);
+                           bool coarse_box_only = false, const SideStream* pack = nullptr);
 template <class T>
 void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double quantum, T* coeff, uint32_t* status,
-                             hipStream_t s);
+                             hipStream_t s, const SideStream* side = nullptr);
 
 // sections.hip
 constexpr uint32_t kRepairTile = 4096;  // row-major indices per standalone repair segment

@@ -21,6 +21,7 @@
 // undone top-down with popcount prefix ranks, then BIT_2, then signs are
 // expanded under the nonzero mask. Every read is bounds-checked against the
 // stream; inconsistencies raise kCorrupt.
+#include "profile.hpp"
 #include "wzp_device.hpp"
 
 namespace wz {
@@ -540,20 +541,39 @@ __global__ void __launch_bounds__(kPackThreads) wzp_pack_kernel(uint8_t* __restr
     }
 }
 
+// The group pass after the encoder, on `s` or on the side stream.
+void launch_pack(uint8_t* out, uint64_t capacity, const WzpShape& ws, const WzpScratch& w, unsigned long long* total,
+                 uint32_t* status, hipStream_t s, const SideStream* side) {
+    hipStream_t ps = s;
+    if (side) {
+        hip_check(hipEventRecord(side->ready, s), "event record");
+        hip_check(hipStreamWaitEvent(side->stream, side->ready, 0), "stream wait");
+        ps = side->stream;
+    }
+    {
+        StageClock c("wzp_pack", ps);
+        hipLaunchKernelGGL(wzp_pack_kernel, dim3(ws.groups), dim3(kPackThreads), 0, ps, out, capacity, ws.chunks,
+                           ws.groups, w, total, status);
+    }
+    if (side) hip_check(hipEventRecord(side->done, ps), "event record");
+}
+
 void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity, WzpScratch w,
-                unsigned long long* total, uint32_t* status, hipStream_t s) {
+                unsigned long long* total, uint32_t* status, hipStream_t s, const SideStream* pack) {
     const WzpShape ws = wzp_shape(n);
-    if (ws.chunks == 0) return;
+    if (ws.chunks == 0) {
+        if (pack) hip_check(hipEventRecord(pack->done, s), "event record");
+        return;
+    }
     hipLaunchKernelGGL(wzp_encode_kernel<MagSource>, dim3(ws.chunks), dim3(kThreads), 0, s, MagSource{mag, sign}, n,
                        ws.chunks, ws.groups, out, capacity, w, total, status);
-    hipLaunchKernelGGL(wzp_pack_kernel, dim3(ws.groups), dim3(kPackThreads), 0, s, out, capacity, ws.chunks,
-                       ws.groups, w, total, status);
+    launch_pack(out, capacity, ws, w, total, status, s, pack);
 }
 
 template <class T>
 void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
                            WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
-                           bool coarse_box_only) {
+                           bool coarse_box_only, const SideStream* pack) {
     if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
     const uint32_t n = d.nx * d.ny * d.nz;
     const WzpShape ws = wzp_shape(n);
@@ -568,8 +588,7 @@ void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_
                              coarse_box_only ? half_len(d.nz) : d.nz};
     hipLaunchKernelGGL(wzp_encode_kernel<BrickSource<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, src, n, ws.chunks,
                        ws.groups, out, capacity, w, total, status);
-    hipLaunchKernelGGL(wzp_pack_kernel, dim3(ws.groups), dim3(kPackThreads), 0, s, out, capacity, ws.chunks,
-                       ws.groups, w, total, status);
+    launch_pack(out, capacity, ws, w, total, status, s, pack);
 }
 
 template <class Sink>
@@ -589,41 +608,52 @@ __global__ void __launch_bounds__(kDirThreads) wzp_directory_kernel(const uint8_
 
 // The stream's directory check runs beside its chunk decoders (both only
 // read the stream and report into `status`). bytes >= the stream header.
-void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s) {
+void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s,
+                         const SideStream* side) {
     const WzpShape ws = wzp_shape(n);
-    if (ws.chunks == 0) return;
+    if (ws.chunks == 0) {
+        if (side) hip_check(hipEventRecord(side->done, s), "event record");
+        return;
+    }
     if (bytes < ws.header) throw std::runtime_error("WZP archive size is outside its valid bounds");
-    hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, s, in, bytes, n, ws.chunks, ws.groups,
+    hipStream_t ds = s;
+    if (side) {
+        hip_check(hipEventRecord(side->ready, s), "event record");
+        hip_check(hipStreamWaitEvent(side->stream, side->ready, 0), "stream wait");
+        ds = side->stream;
+    }
+    hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, bytes, n, ws.chunks, ws.groups,
                        status);
+    if (side) hip_check(hipEventRecord(side->done, ds), "event record");
 }
 
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
-                hipStream_t s) {
+                hipStream_t s, const SideStream* side) {
     const WzpShape ws = wzp_shape(n);
+    wzp_check_directory(in, bytes, n, status, s, side);
     if (ws.chunks == 0) return;
-    wzp_check_directory(in, bytes, n, status, s);
     hipLaunchKernelGGL(wzp_decode_kernel<MagSink>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n, ws.chunks,
                        ws.groups, MagSink{mag, sign}, status);
 }
 
 template <class T>
 void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double quantum, T* coeff, uint32_t* status,
-                             hipStream_t s) {
+                             hipStream_t s, const SideStream* side) {
     if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
     const uint32_t n = d.nx * d.ny * d.nz;
     const WzpShape ws = wzp_shape(n);
-    wzp_check_directory(in, bytes, n, status, s);
+    wzp_check_directory(in, bytes, n, status, s, side);
     hipLaunchKernelGGL(wzp_decode_kernel<BrickSink<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n,
                        ws.chunks, ws.groups, BrickSink<T>{coeff, d.nx, d.ny, T(quantum)}, status);
 }
 
 template void quantize_encode_pairs<float>(float*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
-                                           unsigned long long*, uint32_t*, hipStream_t, bool);
+                                           unsigned long long*, uint32_t*, hipStream_t, bool, const SideStream*);
 template void quantize_encode_pairs<double>(double*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
-                                            unsigned long long*, uint32_t*, hipStream_t, bool);
+                                            unsigned long long*, uint32_t*, hipStream_t, bool, const SideStream*);
 template void decode_dequantize_pairs<float>(const uint8_t*, uint64_t, Dims, double, float*, uint32_t*,
-                                             hipStream_t);
+                                             hipStream_t, const SideStream*);
 template void decode_dequantize_pairs<double>(const uint8_t*, uint64_t, Dims, double, double*, uint32_t*,
-                                              hipStream_t);
+                                              hipStream_t, const SideStream*);
 
 }  // namespace wz

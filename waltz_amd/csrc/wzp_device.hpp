@@ -419,12 +419,24 @@ WZ_D void check_directory(const uint8_t* __restrict__ in, uint64_t bytes, uint32
         const uint32_t g = base + t;
         uint32_t L = 0, O = 0;
         if (g < groups) {
-            for (uint32_t j = 0; j < kGroup; ++j) {
-                const uint32_t c = g * kGroup + j;
-                if (c >= chunks) break;
-                const uint32_t mc = load24(meta + uint64_t(c) * 3);
+            auto add = [&](uint32_t c, uint32_t mc) {
                 const uint32_t code = mc & kRawCode, sc = mc >> 14;
                 if (code) L += (code == kRawCode ? 2 * chunk_elements(n, c) : code) + (sc ? sc : kSignSlot);
+            };
+            if ((g + 1) * kGroup <= chunks && !(reinterpret_cast<uintptr_t>(meta) & 3u)) {
+                // a whole group: its 48 meta bytes as 12 aligned words
+                const uint32_t* mw = reinterpret_cast<const uint32_t*>(meta) + uint64_t(g) * 12;
+                uint32_t w[12];
+#pragma unroll
+                for (int k = 0; k < 12; ++k) w[k] = mw[k];
+#pragma unroll
+                for (uint32_t j = 0; j < kGroup; ++j) {
+                    const uint32_t k = 3 * j / 4, sh8 = 8 * (3 * j % 4);
+                    const unsigned long long pair = (k + 1 < 12 ? (unsigned long long)w[k + 1] << 32 : 0ull) | w[k];
+                    add(g * kGroup + j, uint32_t(pair >> sh8) & 0xFFFFFFu);
+                }
+            } else {
+                for (uint32_t c = g * kGroup; c < chunks; ++c) add(c, load24(meta + uint64_t(c) * 3));
             }
             O = load32(in + uint64_t(g) * 4);
             len[g] = L;

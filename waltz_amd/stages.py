@@ -15,25 +15,36 @@ def _dims(t):
     return nx, ny, nz
 
 
+def _scratch(volume):
+    # two lifting scratch volumes from torch's stream-ordered allocator: freed
+    # here, they are reused only by work queued after this call's kernels
+    return torch.empty_like(volume), torch.empty_like(volume)
+
+
 def transform(volume, levels_xy, levels_z, dyadic, inverse, out=None):
-    """CDF 9/7 lifting (forward: samples -> coefficients; inverse: back)."""
+    """CDF 9/7 lifting (forward: samples -> coefficients; inverse: back),
+    enqueued on the current stream without synchronising."""
     volume = volume.contiguous()
     out = torch.empty_like(volume) if out is None else out
     nx, ny, nz = _dims(volume)
-    check(lib().waltz_amd_transform(_W[volume.dtype], int(inverse), volume.data_ptr(), out.data_ptr(), nx, ny, nz,
-                                    levels_xy, levels_z, int(dyadic), _stream(volume.device)), "transform")
+    t0, t1 = _scratch(volume)
+    check(lib().waltz_amd_transform_async(_W[volume.dtype], int(inverse), volume.data_ptr(), out.data_ptr(),
+                                          t0.data_ptr(), t1.data_ptr(), nx, ny, nz, levels_xy, levels_z, int(dyadic),
+                                          _stream(volume.device)), "transform")
     return out
 
 
 def transform_adjoint(volume, levels_xy, levels_z, dyadic, of_inverse, out=None):
     """Transpose of transform() with the same geometry: of_inverse=False maps a
     coefficient gradient to samples (DWT^T), True maps a sample gradient to
-    coefficients (IDWT^T)."""
+    coefficients (IDWT^T). Enqueued on the current stream without synchronising."""
     volume = volume.contiguous()
     out = torch.empty_like(volume) if out is None else out
     nx, ny, nz = _dims(volume)
-    check(lib().waltz_amd_transform_adjoint(_W[volume.dtype], int(of_inverse), volume.data_ptr(), out.data_ptr(), nx,
-                                            ny, nz, levels_xy, levels_z, int(dyadic), _stream(volume.device)),
+    t0, t1 = _scratch(volume)
+    check(lib().waltz_amd_transform_adjoint_async(_W[volume.dtype], int(of_inverse), volume.data_ptr(),
+                                                  out.data_ptr(), t0.data_ptr(), t1.data_ptr(), nx, ny, nz, levels_xy,
+                                                  levels_z, int(dyadic), _stream(volume.device)),
           "transform_adjoint")
     return out
 

@@ -45,3 +45,24 @@ def sinusoids(shape, count=8, seed=20261202, noise=0.01, dtype=torch.float32, de
     gn = torch.Generator(device=device).manual_seed(int(seed) + 1)
     f += noise * torch.randn((nz, ny, nx), generator=gn, device=device, dtype=torch.float64)
     return f.to(dtype).contiguous()
+
+
+def fronts(shape, seed=20261202, count=3, amplitude=40.0, noise=3.0, dtype=torch.float64, device="cuda"):
+    """Smooth field with sharp fronts: sum of `count` seeded tanh fronts
+    amplitude * tanh((n_i . r - c_i) / w_i) (unit normals n_i, offsets c_i
+    inside the box, widths w_i of 2-8 samples) plus `noise` times a unit
+    variance k^-3 Gaussian random field; range about [-50, 50]."""
+    nz, ny, nx = shape
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    z = torch.arange(nz, device=device, dtype=torch.float64).view(nz, 1, 1)
+    y = torch.arange(ny, device=device, dtype=torch.float64).view(1, ny, 1)
+    x = torch.arange(nx, device=device, dtype=torch.float64).view(1, 1, nx)
+    f = torch.zeros((nz, ny, nx), device=device, dtype=torch.float64)
+    for _ in range(count):
+        v = torch.randn(3, generator=g, dtype=torch.float64)
+        v = v / v.norm()
+        c = float(torch.rand((), generator=g, dtype=torch.float64)) * float(v[0] * nx + v[1] * ny + v[2] * nz)
+        w = 2.0 + 6.0 * float(torch.rand((), generator=g, dtype=torch.float64))
+        f += amplitude / count * torch.tanh((float(v[0]) * x + float(v[1]) * y + float(v[2]) * z - c) / w)
+    f += noise * grf(shape, slope=-3.0, seed=int(seed) + 1, device=device).to(torch.float64)
+    return f.to(dtype).contiguous()
