@@ -99,6 +99,26 @@ int waltz_amd_batch_decode(waltz_amd_batch* batch, const void* packed, size_t pa
                            void* output, void* workspace, size_t workspace_bytes, void* stream);
 int waltz_amd_batch_check(waltz_amd_batch* batch, void* stream);
 
+/* Batched compress with the same plan (no reference counterpart: the
+ * reference compresses one field per call, src/waltz.cu:327-609). `input`
+ * holds every job's field concatenated in job order (output_elements()
+ * samples, the layout batch_decode writes), compressed with one absolute
+ * bound; archive j is written at a 16-byte aligned slot of `archives`
+ * (archive_capacity() bytes) and `spans` (device, jobs x little-endian uint64
+ * {offset, size}) receives each archive's place and length (size 0 for a job
+ * that failed), i.e. exactly what batch_decode takes. Every archive is the
+ * canonical one the single-field compress writes. Work: one launch per stage
+ * for all jobs; the call only enqueues work on `stream` (no synchronisation,
+ * allocation or memset), so it can be captured and replayed. compress_check
+ * synchronises `stream` and reports the first failed job ("job j: ...").
+ * The plan must not run a decode and a compress at the same time. */
+size_t waltz_amd_batch_compress_workspace_bytes(const waltz_amd_batch* batch);
+size_t waltz_amd_batch_archive_capacity(const waltz_amd_batch* batch);
+int waltz_amd_batch_compress(waltz_amd_batch* batch, const void* input, double abs_bound, void* archives,
+                             size_t archives_bytes, void* spans, void* workspace, size_t workspace_bytes,
+                             void* stream);
+int waltz_amd_batch_compress_check(waltz_amd_batch* batch, void* stream);
+
 /* Per-stage timing with HIP events on the pipeline's stream (off by default).
  * enable returns the previous state; report writes a JSON object
  * {"stage": [total_ms, launches], ...} accumulated since the last report and

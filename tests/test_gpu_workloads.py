@@ -6,7 +6,7 @@
   (cpu/tests/interop_gpu.cu:216-246 of the reference: the CPU codec is canonical).
 - 256 float32 chunks of uneven shape (seeded mix of 32^3, 64x64x32, 96x64x48,
   128x128x64; each its own k^-5/3 field), ABS 1e-3: every archive equals the
-  oracle's, and every tile of the batched decode (eager, and replayed from a
+  oracle's (one call per chunk and one batched compress of all of them), and every tile of the batched decode (eager, and replayed from a
   captured graph between eager work) equals the oracle's decode of its archive
   (python/tests/test_gpu_loader.py:104-125: batch == per-chunk CPU decode).
 - 8 x 256^3 float32 fields, REL 1e-3: the transpose identity of the inverse
@@ -72,7 +72,7 @@ def test_batched_256_chunks_match_oracle(wz, oracle):
     from waltz_amd import batch
     shapes = _batch_shapes()
     assert {s for s in shapes} == {(32, 32, 32), (32, 64, 64), (48, 64, 96), (64, 128, 128)}
-    archives, hosts = [], []
+    archives, hosts, fields = [], [], []
     for j, s in enumerate(shapes):
         f = synthetic.grf(s, slope=-5.0 / 3.0, seed=20261202 + 4 + 1000 * (j + 1))
         a = waltz_amd.compress(f, abs_error=1e-3).clone()
@@ -80,7 +80,17 @@ def test_batched_256_chunks_match_oracle(wz, oracle):
         assert h == oracle.compress(f.cpu().numpy(), 1e-3), (j, s)
         archives.append(a)
         hosts.append(h)
+        fields.append(f)
     _say(f"{len(shapes)} archives byte-identical to the oracle")
+    # the same 256 fields through the batched compress: the same archives
+    plan = batch.BatchDecoder(shapes, torch.float32)
+    barch, bspans = plan.compress(plan.flatten(fields), abs_error=1e-3)
+    sp = bspans.cpu().numpy()
+    for j, h in enumerate(hosts):
+        got = barch[int(sp[2 * j]):int(sp[2 * j]) + int(sp[2 * j + 1])].cpu().numpy().tobytes()
+        assert got == h, (j, shapes[j])
+    _say("batched compress: 256 archives byte-identical to the oracle")
+    del barch
     wants = [oracle.decompress(h, s, np.float32) for h, s in zip(hosts, shapes)]
     packed, spans = batch.pack(archives)
     dec = batch.BatchDecoder(shapes, torch.float32)

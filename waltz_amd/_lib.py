@@ -53,6 +53,10 @@ _SIGS.update({
     "waltz_amd_batch_output_elements": (_SZ, [_P]),
     "waltz_amd_batch_decode": (C.c_int, [_P, _P, _SZ, _P, _P, _P, _SZ, _P]),
     "waltz_amd_batch_check": (C.c_int, [_P, _P]),
+    "waltz_amd_batch_compress_workspace_bytes": (_SZ, [_P]),
+    "waltz_amd_batch_archive_capacity": (_SZ, [_P]),
+    "waltz_amd_batch_compress": (C.c_int, [_P, _P, C.c_double, _P, _SZ, _P, _P, _SZ, _P]),
+    "waltz_amd_batch_compress_check": (C.c_int, [_P, _P]),
     "waltz_amd_transform_adjoint": (C.c_int, [C.c_int, C.c_int, _P, _P, _U32, _U32, _U32, _U, _U, C.c_int, _P]),
     "waltz_amd_transform_async": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _U32, _U32, _U32, _U, _U, C.c_int,
                                             _P]),

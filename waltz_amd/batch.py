@@ -1,13 +1,14 @@
-"""Batched decode of many archives (mixed tile shapes) on the GPU, one launch
-per pipeline stage for the whole batch, optionally replayed as a captured
-graph. The reference's loader entry point for one tile shape,
-waltz_gpu_decode_batch (python/pywaltz/gpu_dataloader.py:34-36), is exported
-by the same library.
+"""Batched compress and decode of many fields / archives of mixed tile
+shapes on the GPU, one launch per pipeline stage for the whole batch,
+optionally replayed as a captured graph. The reference's loader entry point
+for one tile shape, waltz_gpu_decode_batch
+(python/pywaltz/gpu_dataloader.py:34-36), is exported by the same library.
 
     dec = BatchDecoder([(32, 32, 32), (48, 64, 96)], torch.float32)
-    packed, spans = pack([archive0, archive1])        # device tensors
-    flat = dec.decode(packed, spans)                  # tiles back to back
+    archives, spans = dec.compress(dec.flatten([f0, f1]), abs_error=1e-3)
+    flat = dec.decode(archives, spans)                # tiles back to back
     tiles = dec.views(flat)
+    packed, spans = pack([archive0, archive1])        # archives made one by one
 """
 import ctypes as C
 
@@ -78,6 +79,44 @@ class BatchDecoder:
     def check(self):
         if lib().waltz_amd_batch_check(self._h, _stream(self.device)):
             raise WaltzError(f"batch decode: {lib().waltz_amd_batch_error().decode(errors='replace')}")
+
+    def flatten(self, fields):
+        """The jobs' fields (one per shape, in order) as one flat tensor."""
+        if len(fields) != len(self.shapes):
+            raise ValueError("one field per job")
+        for f, s in zip(fields, self.shapes):
+            if tuple(f.shape) != s or f.dtype != self.dtype:
+                raise ValueError("field shape or dtype differs from the plan")
+        return torch.cat([f.reshape(-1) for f in fields])
+
+    def compress(self, flat, abs_error, archives=None, spans=None, check=True):
+        """Compress every job's field (flat: numel samples, jobs in order) into
+        archive slots of one uint8 device buffer; returns (archives, spans)
+        with spans the int64 (offset, size) table decode() takes. With
+        check=False the work is only enqueued (graph capture); call
+        compress_check() afterwards."""
+        if flat.numel() != self.numel or flat.dtype != self.dtype or not flat.is_contiguous():
+            raise ValueError("input must be a contiguous tensor of numel samples")
+        cap = int(lib().waltz_amd_batch_archive_capacity(self._h))
+        if archives is None:
+            archives = torch.zeros(cap, dtype=torch.uint8, device=self.device)
+        if spans is None:
+            spans = torch.zeros(2 * len(self.shapes), dtype=torch.int64, device=self.device)
+        ws = int(lib().waltz_amd_batch_compress_workspace_bytes(self._h))
+        if getattr(self, "cworkspace", None) is None or self.cworkspace.numel() < ws:
+            self.cworkspace = torch.empty(ws, dtype=torch.uint8, device=self.device)
+        rc = lib().waltz_amd_batch_compress(self._h, flat.data_ptr(), float(abs_error), archives.data_ptr(),
+                                            archives.numel(), spans.data_ptr(), self.cworkspace.data_ptr(),
+                                            self.cworkspace.numel(), _stream(self.device))
+        if rc:
+            raise WaltzError(f"batch compress: {lib().waltz_amd_batch_error().decode(errors='replace')}")
+        if check:
+            self.compress_check()
+        return archives, spans
+
+    def compress_check(self):
+        if lib().waltz_amd_batch_compress_check(self._h, _stream(self.device)):
+            raise WaltzError(f"batch compress: {lib().waltz_amd_batch_error().decode(errors='replace')}")
 
     def views(self, flat):
         out, at = [], 0

@@ -39,8 +39,10 @@
 #include "../../include/waltz_gpu_batch.h"
 #include "common.hpp"
 #include "launch.hpp"
+#include "bricks.hpp"
 #include "lifting.hpp"
-#include "wzp_device.hpp"
+#include "sections.hpp"
+#include "wzp_encode.hpp"
 
 namespace wz {
 namespace {
@@ -401,6 +403,291 @@ __global__ void __launch_bounds__(256) batch_rpatch_kernel(const uint8_t* __rest
     }
 }
 
+// ------------------------------------------------------------------ batched compress
+// The single-field compress pipeline (codec.hip) with one launch per stage for
+// every job of the batch: forward passes, brick quantization, WZP chunks, WZP
+// groups, inverse passes into a reconstruction bank, repair tiles, rank
+// prefixes, layout + header per job, placement of both side lists. Per-job
+// offsets into the work banks and into the caller's archive buffer come from
+// the plan; failures are recorded per job (EncodeResult::status).
+struct JobComp {
+    uint64_t arc, cap;      // the job's archive slot in the output buffer (bytes)
+    uint64_t staging;       // byte offset of its WZP staging slots
+    uint64_t qlist;         // entry offset of its q-outlier list
+    uint64_t rlist;         // entry offset of its repair segments
+    uint64_t words;         // word offset of its flag bitmaps (16-byte aligned)
+    uint64_t prefix;        // word offset of its rank scratch
+    uint32_t qcap, tiles;   // q-outlier capacity, repair tiles of kRepairTile
+    uint32_t chunk0, group0, tile0, pblocks;  // global first chunk / group / tile; prefix blocks
+    uint32_t qblocks, pad;  // q-outlier placement blocks
+};
+
+// Device pointers of one batched compress call.
+struct CompBufs {
+    const void* input;      // jobs' fields, concatenated (JobShape::out offsets)
+    void* home;             // coefficients
+    void* tmp0;             // lifting scratch; then u16 magnitudes
+    void* tmp1;             // lifting scratch; then sign words
+    void* recon;            // reconstruction (the input's layout)
+    uint8_t* staging;
+    uint32_t *qbm, *rbm;    // flag bitmaps
+    uint32_t *qidx, *qval;  // q-outlier lists
+    uint32_t* ridx;         // repair segments: indices
+    void* rval;             // repair segments: corrections
+    uint32_t *rcount, *qcount, *prefix, *sizes, *ticket;
+    unsigned long long* desc;
+    EncodeResult* res;
+    uint8_t* out;           // archive slots
+    uint64_t* spans;        // jobs x {offset, size}
+};
+
+// One tile of one line of a forward axis pass (transform.hip's lift_tile,
+// forward: reflected halo, band-split outputs).
+template <class T, int Axis, int Tile>
+WZ_D bool forward_line(const T* __restrict__ src, T* __restrict__ dst, Dims full, Dims act, uint32_t ntiles,
+                       uint32_t i, bool screen) {
+    constexpr int P = Tile / 2 + 4;
+    uint32_t x = 0, y = 0, z = 0, t;
+    if constexpr (Axis == 0) {
+        t = i % ntiles;
+        const uint32_t r = i / ntiles;
+        y = r % act.ny;
+        z = r / act.ny;
+    } else if constexpr (Axis == 1) {
+        x = i % act.nx;
+        const uint32_t r = i / act.nx;
+        t = r % ntiles;
+        z = r / ntiles;
+    } else {
+        x = i % act.nx;
+        const uint32_t r = i / act.nx;
+        y = r % act.ny;
+        t = r / act.ny;
+    }
+    const uint64_t plane = uint64_t(full.nx) * full.ny;
+    const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
+    const uint64_t stride = Axis == 0 ? 1u : Axis == 1 ? uint64_t(full.nx) : plane;
+    const uint64_t base = uint64_t(z) * plane + uint64_t(y) * full.nx + x;
+    const int32_t start = int32_t(t) * Tile;
+    T e[P], o[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int32_t p = start - 4 + 2 * j;
+        e[j] = src[base + uint64_t(reflect_pos(p, n)) * stride];
+        o[j] = src[base + uint64_t(reflect_pos(p + 1, n)) * stride];
+    }
+    bool ok = true;
+    if (screen) {  // the input screen of the first pass (cpu/src/codec.cpp:183)
+#pragma unroll
+        for (int j = 0; j < P; ++j) ok = ok && __builtin_isfinite(e[j]) && __builtin_isfinite(o[j]);
+    }
+    lift_forward<T, P>(e, o);
+    const uint32_t h = half_len(n);
+#pragma unroll
+    for (int j = 2; j < P - 2; ++j) {
+        const uint32_t p = uint32_t(start) + 2u * uint32_t(j - 2);
+        if (p < n) dst[base + uint64_t(p >> 1) * stride] = e[j];
+        if (p + 1 < n) dst[base + uint64_t(h + (p >> 1)) * stride] = o[j];
+    }
+    return ok;
+}
+
+enum : uint8_t { kInput = 4, kRecon = 5 };
+
+template <class T>
+__global__ void __launch_bounds__(256) batch_fwd_pass_kernel(const JobShape* __restrict__ shape,
+                                                             const Pass* __restrict__ pass,
+                                                             const uint32_t* __restrict__ pfx, uint32_t jobs,
+                                                             CompBufs b) {
+    constexpr int ZTile = sizeof(T) == 4 ? 16 : 8;
+    const uint32_t j = job_of(pfx, jobs, blockIdx.x);
+    const Pass ps = pass[j];
+    const uint32_t i = (blockIdx.x - pfx[j]) * 256u + threadIdx.x;
+    if (ps.kind == kPassNone) return;
+    const JobShape& sh = shape[j];
+    auto bank = [&](uint8_t id) -> T* {
+        if (id == kInput) return const_cast<T*>(static_cast<const T*>(b.input)) + sh.out;
+        T* p = static_cast<T*>(id == kHome ? b.home : id == kTmp0 ? b.tmp0 : b.tmp1);
+        return p + sh.bank;
+    };
+    const T* src = bank(ps.src);
+    T* dst = bank(ps.dst);
+    const bool screen = ps.src == kInput;
+    bool ok = true;
+    if (i < ps.work) {
+        switch (ps.kind) {
+            case kPassX: ok = forward_line<T, 0, 8>(src, dst, sh.d, ps.act, ps.ntiles, i, screen); break;
+            case kPassY: ok = forward_line<T, 1, 8>(src, dst, sh.d, ps.act, ps.ntiles, i, screen); break;
+            // the Z tile make_pass sized the pass for (forward tiles change no bit)
+            case kPassZ: ok = forward_line<T, 2, ZTile>(src, dst, sh.d, ps.act, ps.ntiles, i, screen); break;
+            default: {
+                const uint32_t x = i % ps.act.nx, r = i / ps.act.nx, y = r % ps.act.ny, z = r / ps.act.ny;
+                const uint64_t at = (uint64_t(z) * sh.d.ny + y) * sh.d.nx + x;
+                dst[at] = src[at];
+            }
+        }
+    }
+    if (__ballot(!ok) && (threadIdx.x & 63u) == 0) atomicOr(&b.res[j].status, uint32_t(kNonFiniteInput));
+}
+
+// Packed rank inside a brick by the recursive octant rule (cpu/src/codec.cpp:96-107).
+struct MortonRank {
+    WZ_D uint32_t operator()(const Brick& k, uint32_t, uint32_t x, uint32_t y, uint32_t z) const {
+        return morton_rank(x, y, z, k.bx, k.by, k.bz);
+    }
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) batch_quantize_kernel(const JobShape* __restrict__ shape,
+                                                             const JobComp* __restrict__ comp,
+                                                             const uint32_t* __restrict__ brick_pfx, uint32_t jobs,
+                                                             double quantum, CompBufs b) {
+    const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    const BrickGeometry g{sh.d, sh.bz, sh.nbx, sh.nby, sh.nbz};
+    const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
+    quantize_brick(static_cast<T*>(b.home) + sh.bank, g, T(quantum), T(1.0 / quantum), MortonRank{},
+                   static_cast<uint16_t*>(b.tmp0) + sh.mag, static_cast<uint32_t*>(b.tmp1) + sh.sign, qo,
+                   &b.res[j].status, blockIdx.x - brick_pfx[j]);
+}
+
+WZ_D WzpScratch job_wzp(const CompBufs& b, const JobComp& jc) {
+    return WzpScratch{b.desc + jc.group0, b.staging + jc.staging, b.sizes + jc.chunk0, nullptr};
+}
+
+__global__ void __launch_bounds__(256) batch_encode_kernel(const JobShape* __restrict__ shape,
+                                                           const JobComp* __restrict__ comp,
+                                                           const uint32_t* __restrict__ chunk_pfx, uint32_t jobs,
+                                                           CompBufs b) {
+    const uint32_t j = job_of(chunk_pfx, jobs, blockIdx.x);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    const MagSource src{static_cast<const uint16_t*>(b.tmp0) + sh.mag, static_cast<const uint32_t*>(b.tmp1) + sh.sign};
+    encode_chunk(src, blockIdx.x - chunk_pfx[j], sh.n, sh.groups, b.out + jc.arc + 32, jc.cap - 32, job_wzp(b, jc),
+                 &b.res[j].status);
+}
+
+// Groups of all jobs in one ticket order (job by job, group by group), so
+// every group's predecessors in its own stream are held by running workgroups.
+__global__ void __launch_bounds__(kPackThreads) batch_pack_kernel(const JobShape* __restrict__ shape,
+                                                                  const JobComp* __restrict__ comp,
+                                                                  const uint32_t* __restrict__ group_pfx,
+                                                                  uint32_t jobs, CompBufs b) {
+    __shared__ uint32_t g_sh;
+    if (threadIdx.x == 0) g_sh = atomicAdd(b.ticket, 1u);
+    __syncthreads();
+    const uint32_t j = job_of(group_pfx, jobs, g_sh);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    pack_group(g_sh - group_pfx[j], b.out + jc.arc + 32, jc.cap - 32, sh.chunks, sh.groups, job_wzp(b, jc),
+               &b.res[j].wzp_bytes, &b.res[j].status);
+}
+
+template <class T>
+WZ_D SegList job_repairs(const CompBufs& b, const JobComp& jc) {
+    return SegList{b.rbm + jc.words, b.ridx + jc.rlist, static_cast<T*>(b.rval) + jc.rlist, b.rcount + jc.tile0,
+                   jc.tiles, kRepairTile};
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) batch_repair_kernel(const JobShape* __restrict__ shape,
+                                                           const JobComp* __restrict__ comp,
+                                                           const uint32_t* __restrict__ tile_pfx, uint32_t jobs,
+                                                           double bound, CompBufs b) {
+    const uint32_t j = job_of(tile_pfx, jobs, blockIdx.x);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    repair_tile(static_cast<const T*>(b.input) + sh.out, static_cast<const T*>(b.recon) + sh.out, sh.n, bound,
+                job_repairs<T>(b, jc), blockIdx.x - tile_pfx[j], &b.res[j].status);
+}
+
+// Rank scratch of a job at jc.prefix: [repair wpre | repair bsum | q wpre | q bsum]
+// (finish_scratch_words(n) words, each part a multiple of 64 words).
+struct JobRanks {
+    Ranks r, q;
+    uint32_t *rbsum, *qbsum;
+    uint32_t words, blocks;
+};
+WZ_D JobRanks job_ranks(const CompBufs& b, const JobShape& sh, const JobComp& jc) {
+    const uint32_t words = (sh.n + 31) / 32, blocks = (words + kPrefixWords - 1) / kPrefixWords;
+    const uint64_t pw = (uint64_t(words) + 63) & ~uint64_t(63), pb = (uint64_t(blocks) + 63) & ~uint64_t(63);
+    uint32_t* base = b.prefix + jc.prefix;
+    JobRanks k;
+    k.rbsum = base + pw;
+    k.qbsum = base + pw + pb + pw;
+    k.r = Ranks{b.rbm + jc.words, base, k.rbsum};
+    k.q = Ranks{b.qbm + jc.words, base + pw + pb, k.qbsum};
+    k.words = words;
+    k.blocks = blocks;
+    return k;
+}
+
+// Word prefixes of both flag bitmaps of every job: block k of the launch is
+// prefix block (k / 2 - pblock_pfx[j]) of job j, bitmap k % 2.
+__global__ void __launch_bounds__(256) batch_prefix_kernel(const JobShape* __restrict__ shape,
+                                                           const JobComp* __restrict__ comp,
+                                                           const uint32_t* __restrict__ pblock_pfx, uint32_t jobs,
+                                                           CompBufs b) {
+    const uint32_t k = blockIdx.x >> 1, which = blockIdx.x & 1u;
+    const uint32_t j = job_of(pblock_pfx, jobs, k);
+    const JobShape& sh = shape[j];
+    const JobRanks r = job_ranks(b, sh, comp[j]);
+    const uint32_t blk = k - pblock_pfx[j];
+    if (which == 0) word_prefix_block(r.r.bm, r.words, const_cast<uint32_t*>(r.r.wpre), r.rbsum, blk);
+    else if (b.qcount[j] != 0) word_prefix_block(r.q.bm, r.words, const_cast<uint32_t*>(r.q.wpre), r.qbsum, blk);
+}
+
+// One 1024-thread workgroup per job: block sums, layout, header, padding.
+template <class T>
+__global__ void __launch_bounds__(1024) batch_layout_kernel(const JobShape* __restrict__ shape,
+                                                            const JobComp* __restrict__ comp, double quantum,
+                                                            CompBufs b) {
+    const uint32_t j = blockIdx.x;
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    const JobRanks r = job_ranks(b, sh, jc);
+    const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
+    const uint8_t flags = uint8_t((sh.dyadic ? 1u : 0u) | (sizeof(T) == 8 ? 4u : 0u) | (3u << 3));
+    scan_layout<T>(b.out + jc.arc, jc.cap, sh.n, r.rbsum, r.qbsum, r.blocks, &b.res[j].pad, qo, flags,
+                   uint8_t(sh.lx), uint8_t(sh.lz), uint8_t(sh.bz), quantum, &b.res[j]);
+    if (threadIdx.x == 0) {
+        b.spans[2 * j] = jc.arc;
+        b.spans[2 * j + 1] = b.res[j].status ? 0ull : b.res[j].end;
+    }
+}
+
+// Both side lists of every job to their index-order slots: per job, tiles x
+// kPlaceSlices workgroups for the repairs, then qblocks for the q-outliers.
+template <class T>
+__global__ void __launch_bounds__(256) batch_place_kernel(const JobShape* __restrict__ shape,
+                                                          const JobComp* __restrict__ comp,
+                                                          const uint32_t* __restrict__ place_pfx, uint32_t jobs,
+                                                          CompBufs b) {
+    const uint32_t j = job_of(place_pfx, jobs, blockIdx.x);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    const EncodeResult* res = &b.res[j];
+    if (res->status) return;
+    const JobRanks r = job_ranks(b, sh, jc);
+    const uint32_t g = blockIdx.x - place_pfx[j], rblocks = jc.tiles * kPlaceSlices;
+    uint8_t* archive = b.out + jc.arc;
+    if (g < rblocks) {
+        place_repairs<T>(archive, job_repairs<T>(b, jc), r.r, res, g / kPlaceSlices, g % kPlaceSlices);
+        return;
+    }
+    const uint32_t np = res->np, nq = res->nq;
+    const Layout L = archive_layout(res->wzp_bytes, np, uint64_t(np) * sizeof(T), nq);
+    const uint32_t* qi = b.qidx + jc.qlist;
+    const uint32_t* qv = b.qval + jc.qlist;
+    for (uint32_t e = (g - rblocks) * 256u + threadIdx.x; e < nq; e += jc.qblocks * 256u) {
+        const uint32_t idx = qi[e];
+        const uint32_t pos = r.q.of(idx);
+        reinterpret_cast<uint32_t*>(archive + L.qi)[pos] = idx;
+        reinterpret_cast<uint32_t*>(archive + L.qv)[pos] = qv[e];
+    }
+}
+
 // ------------------------------------------------------------------ host plan
 Dims halve(Dims d, bool xy, bool z) {
     if (xy) {
@@ -480,6 +767,62 @@ std::vector<Pass> inverse_schedule(const JobShape& sh, unsigned width) {
     return out;
 }
 
+// The forward schedule of transform_forward (cpu/src/transform.cpp:425-456) as
+// out-of-place passes: the first reads the job's input field, each level
+// ends in the home bank (a level of one pass that starts in home goes through
+// tmp0 and is copied back). Forward tiles do not change a bit.
+std::vector<Pass> forward_schedule(const JobShape& sh, unsigned width) {
+    std::vector<Pass> out;
+    struct Level {
+        Dims act;
+        std::vector<uint8_t> axes;
+    };
+    std::vector<Level> levels;
+    if (sh.dyadic) {
+        Dims act = sh.d;
+        for (unsigned l = 0; l < sh.lx; ++l) {
+            levels.push_back({act, {kPassY, kPassX, kPassZ}});
+            act = halve(act, true, true);
+        }
+    } else {
+        Dims act = sh.d;
+        for (unsigned l = 0; l < sh.lz; ++l) {
+            levels.push_back({act, {kPassZ}});
+            act = halve(act, false, true);
+        }
+        act = sh.d;
+        for (unsigned l = 0; l < sh.lx; ++l) {
+            levels.push_back({act, {kPassY, kPassX}});
+            act = halve(act, true, false);
+        }
+    }
+    if (levels.empty()) {
+        out.push_back(make_pass(kPassCopy, kInput, kHome, sh.d, width));
+        return out;
+    }
+    uint8_t src = kInput;
+    for (const Level& L : levels) {
+        for (size_t a = 0; a < L.axes.size(); ++a) {
+            const bool last = a + 1 == L.axes.size();
+            uint8_t dst = last ? uint8_t(kHome) : (src == kTmp0 ? uint8_t(kTmp1) : uint8_t(kTmp0));
+            if (dst == src) dst = kTmp0;
+            out.push_back(make_pass(L.axes[a], src, dst, L.act, width));
+            src = dst;
+        }
+        if (src != kHome) {
+            out.push_back(make_pass(kPassCopy, src, kHome, L.act, width));
+            src = kHome;
+        }
+    }
+    return out;
+}
+
+// Byte offsets of one batched compress call's workspace (all 256-byte aligned).
+struct CompressLayout {
+    uint64_t home, tmp0, tmp1, recon, staging, qbm, rbm, qidx, qval, ridx, rval, rcount, prefix, sizes, desc;
+    uint64_t zero0, qcount, ticket, res, zero_end, total;
+};
+
 struct HostPlan {
     unsigned width = 4;
     uint32_t jobs = 0;
@@ -488,6 +831,13 @@ struct HostPlan {
     std::vector<std::vector<Pass>> step;      // step[s][job]
     std::vector<std::vector<uint32_t>> pfx;   // block prefixes per step
     uint64_t bank_elems = 0, mag_elems = 0, sign_words = 0, out_elems = 0;
+    // compress
+    std::vector<std::vector<Pass>> fstep;     // forward passes, fstep[s][job]
+    std::vector<std::vector<uint32_t>> fpfx;
+    std::vector<JobComp> comp;
+    std::vector<uint32_t> group_pfx, tile_pfx, pblock_pfx, place_pfx;
+    uint64_t archive_bytes = 0;               // archive slots of all jobs
+    CompressLayout cl{};
 };
 
 // Geometry checks of the single-field codec (codec.hip geometry()).
@@ -510,6 +860,106 @@ JobShape job_shape(Dims d, unsigned width) {
     s.nbz = (d.nz + s.bz - 1) / s.bz;
     s.clip = d;
     return s;
+}
+
+// Stage tables and workspace layout of the batched compress (for the same
+// job shapes; inputs use the JobShape::out offsets).
+void finish_compress_plan(HostPlan& p) {
+    const uint32_t J = p.jobs;
+    const unsigned w = p.width;
+    std::vector<std::vector<Pass>> sched(J);
+    size_t steps = 0;
+    p.comp.assign(J, JobComp{});
+    p.group_pfx.assign(J + 1, 0);
+    p.tile_pfx.assign(J + 1, 0);
+    p.pblock_pfx.assign(J + 1, 0);
+    p.place_pfx.assign(J + 1, 0);
+    uint64_t arc = 0, staging = 0, qlist = 0, words = 0, prefix = 0, groups = 0, tiles = 0, pblocks = 0, places = 0;
+    for (uint32_t j = 0; j < J; ++j) {
+        const JobShape& s = p.shape[j];
+        JobComp& c = p.comp[j];
+        c.arc = arc;
+        c.cap = max_compressed_bytes(s.d, w);
+        arc += (c.cap + 15) & ~uint64_t(15);
+        c.staging = staging;
+        staging += uint64_t(s.chunks) * kChunkSlot;
+        c.qcap = s.n <= (1u << 20) ? s.n : (1u << 20);
+        c.qlist = qlist;
+        qlist += c.qcap;
+        c.tiles = (s.n + kRepairTile - 1) / kRepairTile;
+        c.words = words;
+        const uint64_t wds = (uint64_t(s.n) + 31) / 32;
+        words += (wds + 3) & ~uint64_t(3);
+        c.prefix = prefix;
+        prefix += finish_scratch_words(s.n);
+        c.chunk0 = p.chunk_pfx[j];
+        c.group0 = uint32_t(groups);
+        c.tile0 = uint32_t(tiles);
+        c.pblocks = uint32_t((wds + kPrefixWords - 1) / kPrefixWords);
+        c.qblocks = std::max(1u, std::min(256u, (c.qcap + 255) / 256));
+        c.rlist = uint64_t(c.tile0) * kRepairTile;
+        p.group_pfx[j] = c.group0;
+        p.tile_pfx[j] = c.tile0;
+        p.pblock_pfx[j] = uint32_t(pblocks);
+        p.place_pfx[j] = uint32_t(places);
+        groups += s.groups;
+        tiles += c.tiles;
+        pblocks += c.pblocks;
+        places += uint64_t(c.tiles) * kPlaceSlices + c.qblocks;
+        sched[j] = forward_schedule(s, w);
+        steps = std::max(steps, sched[j].size());
+    }
+    if (groups > 0x7FFFFFFFull || tiles > 0x7FFFFFFFull || 2 * pblocks > 0x7FFFFFFFull || places > 0x7FFFFFFFull)
+        throw std::invalid_argument("batch launch limit");
+    p.group_pfx[J] = uint32_t(groups);
+    p.tile_pfx[J] = uint32_t(tiles);
+    p.pblock_pfx[J] = uint32_t(pblocks);
+    p.place_pfx[J] = uint32_t(places);
+    p.archive_bytes = arc;
+    p.fstep.assign(steps, std::vector<Pass>(J));
+    p.fpfx.assign(steps, std::vector<uint32_t>(J + 1, 0));
+    for (size_t st = 0; st < steps; ++st) {
+        uint64_t blocks = 0;
+        for (uint32_t j = 0; j < J; ++j) {
+            Pass ps{};
+            ps.kind = kPassNone;
+            if (st < sched[j].size()) ps = sched[j][st];
+            p.fstep[st][j] = ps;
+            p.fpfx[st][j] = uint32_t(blocks);
+            if (ps.kind != kPassNone) blocks += (uint64_t(ps.work) + 255) / 256;
+        }
+        if (blocks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
+        p.fpfx[st][J] = uint32_t(blocks);
+    }
+    CompressLayout& L = p.cl;
+    uint64_t at = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = at;
+        at += (bytes + 255) & ~uint64_t(255);
+        return o;
+    };
+    const uint64_t bank = p.bank_elems * w;
+    L.home = take(bank);
+    L.tmp0 = take(std::max(bank, p.mag_elems * 2));
+    L.tmp1 = take(std::max(bank, p.sign_words * 4));
+    L.recon = take(p.out_elems * w);
+    L.staging = take(staging + 64);
+    L.rbm = take(words * 4);
+    L.qidx = take(qlist * 4);
+    L.qval = take(qlist * 4);
+    L.ridx = take(tiles * kRepairTile * 4);
+    L.rval = take(tiles * kRepairTile * w);
+    L.rcount = take(tiles * 4);
+    L.prefix = take(prefix * 4);
+    L.sizes = take(uint64_t(p.chunk_pfx[J]) * 4);
+    L.zero0 = at;  // zero-initialised from here on
+    L.qbm = take(words * 4);
+    L.desc = take(groups * 8);
+    L.qcount = take(uint64_t(J) * 4);
+    L.ticket = take(4);
+    L.res = take(uint64_t(J) * sizeof(EncodeResult));
+    L.zero_end = at;
+    L.total = at;
 }
 
 // Finishes a plan whose shapes (and output placement) are set: bank offsets,
@@ -557,6 +1007,7 @@ void finish_plan(HostPlan& p) {
         if (blocks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
         p.pfx[st][J] = uint32_t(blocks);
     }
+    finish_compress_plan(p);
 }
 
 // Device copy of a plan: tables in one allocation owned by the plan.
@@ -569,6 +1020,12 @@ struct DevicePlan {
     Pass* pass = nullptr;        // steps x jobs
     uint32_t* pass_pfx = nullptr;  // steps x (jobs + 1)
     uint32_t* host_status = nullptr;  // pinned
+    // compress tables
+    JobComp* comp = nullptr;
+    uint32_t *group_pfx = nullptr, *tile_pfx = nullptr, *pblock_pfx = nullptr, *place_pfx = nullptr;
+    Pass* fpass = nullptr;          // forward steps x jobs
+    uint32_t* fpass_pfx = nullptr;  // forward steps x (jobs + 1)
+    EncodeResult* last_res = nullptr;  // results of the last compress (device, in its workspace)
     std::string error;
 
     ~DevicePlan() {
@@ -590,6 +1047,10 @@ struct DevicePlan {
         const size_t o_cp = carve(4 * (J + 1)), o_bp = carve(4 * (J + 1)), o_np = carve(4 * (J + 1)),
                      o_nq = carve(4 * (J + 1)), o_st = carve(4), o_pass = carve(sizeof(Pass) * J * S),
                      o_pp = carve(4 * (J + 1) * S);
+        const size_t FS = p.fstep.size();
+        const size_t o_comp = carve(sizeof(JobComp) * J), o_gp = carve(4 * (J + 1)), o_tp = carve(4 * (J + 1)),
+                     o_pbp = carve(4 * (J + 1)), o_plp = carve(4 * (J + 1)), o_fpass = carve(sizeof(Pass) * J * FS),
+                     o_fpp = carve(4 * (J + 1) * FS);
         hip_check(hipMalloc(reinterpret_cast<void**>(&mem), used), "batch plan allocation");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_status), 4), "batch status");
         shape = reinterpret_cast<JobShape*>(mem + o_shape);
@@ -601,6 +1062,13 @@ struct DevicePlan {
         status = reinterpret_cast<uint32_t*>(mem + o_st);
         pass = reinterpret_cast<Pass*>(mem + o_pass);
         pass_pfx = reinterpret_cast<uint32_t*>(mem + o_pp);
+        comp = reinterpret_cast<JobComp*>(mem + o_comp);
+        group_pfx = reinterpret_cast<uint32_t*>(mem + o_gp);
+        tile_pfx = reinterpret_cast<uint32_t*>(mem + o_tp);
+        pblock_pfx = reinterpret_cast<uint32_t*>(mem + o_pbp);
+        place_pfx = reinterpret_cast<uint32_t*>(mem + o_plp);
+        fpass = reinterpret_cast<Pass*>(mem + o_fpass);
+        fpass_pfx = reinterpret_cast<uint32_t*>(mem + o_fpp);
         std::vector<Pass> flat;
         std::vector<uint32_t> flat_pfx;
         for (size_t st = 0; st < S; ++st) {
@@ -615,7 +1083,103 @@ struct DevicePlan {
             hip_check(hipMemcpyAsync(pass_pfx, flat_pfx.data(), 4 * flat_pfx.size(), hipMemcpyHostToDevice, s),
                       "plan");
         }
+        std::vector<Pass> fflat;
+        std::vector<uint32_t> fflat_pfx;
+        for (size_t st = 0; st < FS; ++st) {
+            fflat.insert(fflat.end(), p.fstep[st].begin(), p.fstep[st].end());
+            fflat_pfx.insert(fflat_pfx.end(), p.fpfx[st].begin(), p.fpfx[st].end());
+        }
+        hip_check(hipMemcpyAsync(comp, p.comp.data(), sizeof(JobComp) * J, hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(group_pfx, p.group_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(tile_pfx, p.tile_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(pblock_pfx, p.pblock_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(place_pfx, p.place_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        if (FS) {
+            hip_check(hipMemcpyAsync(fpass, fflat.data(), sizeof(Pass) * fflat.size(), hipMemcpyHostToDevice, s),
+                      "plan");
+            hip_check(hipMemcpyAsync(fpass_pfx, fflat_pfx.data(), 4 * fflat_pfx.size(), hipMemcpyHostToDevice, s),
+                      "plan");
+        }
         hip_check(hipStreamSynchronize(s), "plan upload");
+    }
+
+    // Enqueues the batched compress on `s`: no synchronisation, allocation or
+    // memset (graph-capturable). `ws` holds host.cl.total bytes.
+    template <class T>
+    void enqueue_compress(const T* input, double bound, uint8_t* out, uint64_t* spans, uint8_t* ws,
+                          hipStream_t s) {
+        const HostPlan& p = host;
+        const uint32_t J = p.jobs;
+        const CompressLayout& L = p.cl;
+        const double quantum = 1.5 * bound;
+        CompBufs b{};
+        b.input = input;
+        b.home = ws + L.home;
+        b.tmp0 = ws + L.tmp0;
+        b.tmp1 = ws + L.tmp1;
+        b.recon = ws + L.recon;
+        b.staging = ws + L.staging;
+        b.qbm = reinterpret_cast<uint32_t*>(ws + L.qbm);
+        b.rbm = reinterpret_cast<uint32_t*>(ws + L.rbm);
+        b.qidx = reinterpret_cast<uint32_t*>(ws + L.qidx);
+        b.qval = reinterpret_cast<uint32_t*>(ws + L.qval);
+        b.ridx = reinterpret_cast<uint32_t*>(ws + L.ridx);
+        b.rval = ws + L.rval;
+        b.rcount = reinterpret_cast<uint32_t*>(ws + L.rcount);
+        b.qcount = reinterpret_cast<uint32_t*>(ws + L.qcount);
+        b.prefix = reinterpret_cast<uint32_t*>(ws + L.prefix);
+        b.sizes = reinterpret_cast<uint32_t*>(ws + L.sizes);
+        b.ticket = reinterpret_cast<uint32_t*>(ws + L.ticket);
+        b.desc = reinterpret_cast<unsigned long long*>(ws + L.desc);
+        b.res = reinterpret_cast<EncodeResult*>(ws + L.res);
+        b.out = out;
+        b.spans = spans;
+        last_res = b.res;
+        zero_words(ws + L.zero0, (L.zero_end - L.zero0) / 4, s);
+        zero_words(run, uint64_t(J) * sizeof(JobRun) / 4, s);  // every job decodable for the inverse passes
+        for (size_t st = 0; st < p.fstep.size(); ++st) {
+            const uint32_t blocks = p.fpfx[st][J];
+            if (blocks)
+                hipLaunchKernelGGL((batch_fwd_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape,
+                                   fpass + st * J, fpass_pfx + st * (J + 1), J, b);
+        }
+        zero_words(b.tmp1, p.sign_words, s);  // sign words: bricks OR their edge words
+        if (p.brick_pfx[J])
+            hipLaunchKernelGGL((batch_quantize_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, comp,
+                               brick_pfx, J, quantum, b);
+        if (p.chunk_pfx[J])
+            hipLaunchKernelGGL(batch_encode_kernel, dim3(p.chunk_pfx[J]), dim3(256), 0, s, shape, comp, chunk_pfx, J,
+                               b);
+        if (p.group_pfx[J])
+            hipLaunchKernelGGL(batch_pack_kernel, dim3(p.group_pfx[J]), dim3(kPackThreads), 0, s, shape, comp,
+                               group_pfx, J, b);
+        const Banks bk{b.home, b.tmp0, b.tmp1, b.recon};
+        for (size_t st = 0; st < p.step.size(); ++st) {
+            const uint32_t blocks = p.pfx[st][J];
+            if (blocks)
+                hipLaunchKernelGGL((batch_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape, run, pass + st * J,
+                                   pass_pfx + st * (J + 1), J, bk, T(0), status);
+        }
+        if (p.tile_pfx[J])
+            hipLaunchKernelGGL((batch_repair_kernel<T>), dim3(p.tile_pfx[J]), dim3(256), 0, s, shape, comp, tile_pfx,
+                               J, bound, b);
+        if (p.pblock_pfx[J])
+            hipLaunchKernelGGL(batch_prefix_kernel, dim3(2 * p.pblock_pfx[J]), dim3(256), 0, s, shape, comp,
+                               pblock_pfx, J, b);
+        hipLaunchKernelGGL((batch_layout_kernel<T>), dim3(J), dim3(1024), 0, s, shape, comp, quantum, b);
+        hipLaunchKernelGGL((batch_place_kernel<T>), dim3(p.place_pfx[J]), dim3(256), 0, s, shape, comp, place_pfx,
+                           J, b);
+    }
+
+    // Per-job results of the last compress (synchronises `s`).
+    std::vector<EncodeResult> compress_results(hipStream_t s) const {
+        hip_check(hipGetLastError(), "batch launch");
+        std::vector<EncodeResult> r(host.jobs);
+        if (!last_res) throw std::runtime_error("no batched compress enqueued");
+        hip_check(hipMemcpyAsync(r.data(), last_res, sizeof(EncodeResult) * host.jobs, hipMemcpyDeviceToHost, s),
+                  "batch results");
+        hip_check(hipStreamSynchronize(s), "batch compress");
+        return r;
     }
 
     // Enqueues the whole decode on `s`: no synchronisation, allocation or memset.
@@ -820,6 +1384,73 @@ const char* waltz_amd_batch_error(void) { return wz::g_batch_error.c_str(); }
 void waltz_amd_batch_destroy(waltz_amd_batch* b) { delete b; }
 
 size_t waltz_amd_batch_workspace_bytes(const waltz_amd_batch* b) { return b ? size_t(3 * b->plan.bank_bytes()) : 0; }
+
+size_t waltz_amd_batch_compress_workspace_bytes(const waltz_amd_batch* b) {
+    return b ? size_t(b->plan.host.cl.total) : 0;
+}
+
+size_t waltz_amd_batch_archive_capacity(const waltz_amd_batch* b) { return b ? size_t(b->plan.host.archive_bytes) : 0; }
+
+int waltz_amd_batch_compress(waltz_amd_batch* b, const void* input, double abs_bound, void* archives,
+                             size_t archives_bytes, void* spans, void* workspace, size_t workspace_bytes,
+                             void* stream) {
+    try {
+        if (!b || !input || !archives || !spans || !workspace) throw std::invalid_argument("null argument");
+        const wz::HostPlan& p = b->plan.host;
+        if (!(abs_bound > 0) || !std::isfinite(abs_bound))
+            throw std::invalid_argument("positive finite absolute error required");
+        const double quantum = 1.5 * abs_bound;
+        const bool f32 = p.width == 4;
+        const bool q_ok = f32 ? (float(quantum) > 0 && std::isfinite(float(quantum)) && std::isfinite(float(1.0 / quantum)))
+                              : (quantum > 0 && std::isfinite(quantum) && std::isfinite(1.0 / quantum));
+        if (!q_ok) throw std::invalid_argument("quantum not representable in native type");
+        if (archives_bytes < p.archive_bytes) throw std::invalid_argument("archive capacity too small");
+        if (workspace_bytes < p.cl.total) throw std::invalid_argument("short batch workspace");
+        if (reinterpret_cast<uintptr_t>(archives) % 16 || reinterpret_cast<uintptr_t>(workspace) % 256 ||
+            reinterpret_cast<uintptr_t>(input) % p.width || reinterpret_cast<uintptr_t>(spans) % 8)
+            throw std::invalid_argument("misaligned batch buffers");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        uint8_t* out = static_cast<uint8_t*>(archives);
+        uint8_t* ws = static_cast<uint8_t*>(workspace);
+        if (f32)
+            b->plan.enqueue_compress<float>(static_cast<const float*>(input), abs_bound, out,
+                                            static_cast<uint64_t*>(spans), ws, s);
+        else
+            b->plan.enqueue_compress<double>(static_cast<const double*>(input), abs_bound, out,
+                                             static_cast<uint64_t*>(spans), ws, s);
+        return 0;
+    } catch (const std::exception& e) {
+        wz::g_batch_error = e.what();
+    } catch (...) {
+        wz::g_batch_error = "unknown batch error";
+    }
+    return -1;
+}
+
+int waltz_amd_batch_compress_check(waltz_amd_batch* b, void* stream) {
+    try {
+        if (!b) throw std::invalid_argument("null batch");
+        const std::vector<wz::EncodeResult> r = b->plan.compress_results(static_cast<hipStream_t>(stream));
+        for (size_t j = 0; j < r.size(); ++j) {
+            const uint32_t st = r[j].status;
+            if (!st) continue;
+            const char* what = (st & wz::kNonFiniteInput)   ? "nonfinite input is unsupported"
+                               : (st & wz::kQuantDomain)    ? "coefficient exceeds portable integer conversion domain"
+                               : (st & wz::kRepairBound)    ? "native repair cannot meet strict bound or nonfinite reconstruction"
+                               : (st & wz::kSideCapacity)   ? "portable GPU side-channel capacity reached"
+                               : (st & wz::kWzpCapacity)    ? "archive capacity too small"
+                               : (st & wz::kLookbackTimeout) ? "WZP lookback did not complete"
+                                                            : "Waltz device status error";
+            throw std::runtime_error("job " + std::to_string(j) + ": " + what);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        wz::g_batch_error = e.what();
+    } catch (...) {
+        wz::g_batch_error = "unknown batch error";
+    }
+    return -1;
+}
 
 size_t waltz_amd_batch_output_elements(const waltz_amd_batch* b) { return b ? size_t(b->plan.host.out_elems) : 0; }
 
