@@ -55,6 +55,7 @@ struct JobShape {
     Dims d;
     uint32_t lx, lz, bz, dyadic;
     uint32_t n, chunks, groups, nbx, nby, nbz;
+    uint32_t rank;      // the job's Morton rank table set (one per distinct geometry)
     uint64_t bank;      // element offset in each work bank
     uint64_t mag;       // u16 offset of the magnitudes (in the tmp0 bank)
     uint64_t sign;      // word offset of the sign words (in the tmp1 bank)
@@ -214,33 +215,56 @@ __global__ void __launch_bounds__(256) batch_wzp_kernel(const uint8_t* __restric
                  MagSink{mag + sh.mag, sign + sh.sign}, status);
 }
 
-// Dequantization of one brick (quantize.hip's brick order and packed base;
-// rank from the recursive octant rule, cpu/src/codec.cpp:96-150).
+// Rank table sets of a plan: per distinct (geometry, z band) the packed rank
+// of every local index of the eight brick edge shapes (quantize.hip's tables).
+constexpr uint64_t kRankSet = 2ull * 8 * 8192;  // u16 entries per set (rank + inverse)
+
+struct SetRank {
+    const uint16_t* set;
+    WZ_D uint32_t operator()(const Brick& k, uint32_t i, uint32_t, uint32_t, uint32_t) const {
+        return set[k.shape * 8192 + i];
+    }
+};
+
+// Dequantization of one brick (quantize.hip's brick order, packed base and ranks).
 template <class T>
 __global__ void __launch_bounds__(256) batch_dequant_kernel(const JobShape* __restrict__ shape,
                                                             const JobRun* __restrict__ run,
                                                             const uint32_t* __restrict__ brick_pfx, uint32_t jobs,
+                                                            const uint16_t* __restrict__ ranks,
                                                             const uint16_t* __restrict__ mag,
                                                             const uint32_t* __restrict__ sign, T* __restrict__ home) {
     const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
     const JobRun& r = run[j];
     if (r.state != kJobOk) return;
     const JobShape& sh = shape[j];
-    const uint32_t b = blockIdx.x - brick_pfx[j];
-    const uint32_t bxi = b % sh.nbx, rest = b / sh.nbx, byi = rest % sh.nby, bzi = rest / sh.nby;
-    const uint32_t x0 = bxi * 16, y0 = byi * 16, z0 = bzi * sh.bz;
-    const uint32_t bx = min(16u, sh.d.nx - x0), by = min(16u, sh.d.ny - y0), bzz = min(sh.bz, sh.d.nz - z0);
-    const uint64_t base = uint64_t(z0) * sh.d.nx * sh.d.ny + uint64_t(y0) * sh.d.nx * bzz + uint64_t(x0) * by * bzz;
+    const BrickGeometry g{sh.d, sh.bz, sh.nbx, sh.nby, sh.nbz};
+    const Brick k = brick_of(g, blockIdx.x - brick_pfx[j]);
     const T q = T(r.quantum);
     const uint16_t* m = mag + sh.mag;
     const uint32_t* sg = sign + sh.sign;
+    const uint16_t* rank = ranks + uint64_t(sh.rank) * kRankSet + k.shape * 8192;
     T* c = home + sh.bank;
-    const uint32_t vol = bx * by * bzz;
-    for (uint32_t i = threadIdx.x; i < vol; i += 256) {
-        const uint32_t x = i % bx, y = (i / bx) % by, z = i / (bx * by);
-        const uint64_t p = base + morton_rank(x, y, z, bx, by, bzz);
-        const T v = T(m[p]) * q;  // cpu/src/codec.cpp:146-147
-        c[(uint64_t(z0 + z) * sh.d.ny + (y0 + y)) * sh.d.nx + x0 + x] = ((sg[p >> 5] >> (p & 31u)) & 1u) ? -v : v;
+    const uint32_t vol = k.bx * k.by * k.bz;
+    constexpr uint32_t kBurst = 8;  // all loads of a burst issued before its stores
+    for (uint32_t i0 = 0; i0 < vol; i0 += kBurst * 256u) {
+        uint32_t mv[kBurst], sv[kBurst], pv[kBurst];
+#pragma unroll
+        for (uint32_t u = 0; u < kBurst; ++u) {
+            const uint32_t i = i0 + u * 256u + threadIdx.x;
+            const uint64_t p = k.base + rank[i < vol ? i : 0u];
+            pv[u] = uint32_t(p & 31u);
+            mv[u] = m[p];
+            sv[u] = sg[p >> 5];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBurst; ++u) {
+            const uint32_t i = i0 + u * 256u + threadIdx.x;
+            if (i >= vol) continue;
+            const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
+            const T v = T(mv[u]) * q;  // cpu/src/codec.cpp:146-147
+            c[(uint64_t(k.z0 + z) * sh.d.ny + (k.y0 + y)) * sh.d.nx + k.x0 + x] = ((sv[u] >> pv[u]) & 1u) ? -v : v;
+        }
     }
 }
 
@@ -530,24 +554,19 @@ __global__ void __launch_bounds__(256) batch_fwd_pass_kernel(const JobShape* __r
     if (__ballot(!ok) && (threadIdx.x & 63u) == 0) atomicOr(&b.res[j].status, uint32_t(kNonFiniteInput));
 }
 
-// Packed rank inside a brick by the recursive octant rule (cpu/src/codec.cpp:96-107).
-struct MortonRank {
-    WZ_D uint32_t operator()(const Brick& k, uint32_t, uint32_t x, uint32_t y, uint32_t z) const {
-        return morton_rank(x, y, z, k.bx, k.by, k.bz);
-    }
-};
-
 template <class T>
 __global__ void __launch_bounds__(256) batch_quantize_kernel(const JobShape* __restrict__ shape,
                                                              const JobComp* __restrict__ comp,
                                                              const uint32_t* __restrict__ brick_pfx, uint32_t jobs,
-                                                             double quantum, CompBufs b) {
+                                                             const uint16_t* __restrict__ ranks, double quantum,
+                                                             CompBufs b) {
     const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     const BrickGeometry g{sh.d, sh.bz, sh.nbx, sh.nby, sh.nbz};
     const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
-    quantize_brick(static_cast<T*>(b.home) + sh.bank, g, T(quantum), T(1.0 / quantum), MortonRank{},
+    quantize_brick(static_cast<T*>(b.home) + sh.bank, g, T(quantum), T(1.0 / quantum),
+                   SetRank{ranks + uint64_t(sh.rank) * kRankSet},
                    static_cast<uint16_t*>(b.tmp0) + sh.mag, static_cast<uint32_t*>(b.tmp1) + sh.sign, qo,
                    &b.res[j].status, blockIdx.x - brick_pfx[j]);
 }
@@ -688,6 +707,190 @@ __global__ void __launch_bounds__(256) batch_place_kernel(const JobShape* __rest
     }
 }
 
+// ------------------------------------------------------------------ LDS-staged passes
+// Two pass kinds that stage a whole piece of the box in LDS and run several
+// lifting passes there, reading and writing the box once:
+//   kPassXYPlane  one workgroup per z plane of the box: the X and Y passes of
+//                 `levels` XY levels (Pass::pad) in canonical order (inverse:
+//                 coarse to fine, X then Y; forward: fine to coarse, Y then X);
+//   kPassZCol     one workgroup per row of 64 columns: the Z passes of
+//                 `levels` Z levels (the plane schedule's whole Z phase).
+// Tiles and arithmetic are those of the per-axis passes (lift_tile), so every
+// value is the same. In place is allowed (src may equal dst).
+enum : uint8_t { kPassXYPlane = 6, kPassZCol = 7 };
+constexpr uint32_t kLdsThreads = 512;
+constexpr uint32_t kColumns = 64;          // columns of a kPassZCol workgroup
+// Samples of the one LDS buffer a workgroup stages (odd plane pitches: rows
+// of up to 128 + 1 samples for the large variant).
+template <class T, bool Large>
+struct PlaneCap {
+    static constexpr uint32_t value = (Large ? 66560u : 16640u) / sizeof(T);
+};
+template <class T>
+constexpr uint32_t zcol_cap() { return PlaneCap<T, true>::value / kColumns; }  // z extent of a column row
+
+// Banks of the LDS passes: input (compress forward), the three work banks, the
+// output (decode) or reconstruction (compress inverse), and where failures go.
+struct StageBanks {
+    const void* input;
+    void* home;
+    void* tmp0;
+    void* tmp1;
+    void* out;
+    EncodeResult* res;  // compress: per-job status; decode: null (status word)
+};
+
+template <class T>
+WZ_D T* stage_bank(const StageBanks& b, const JobShape& sh, uint8_t id) {
+    if (id == kInput) return const_cast<T*>(static_cast<const T*>(b.input)) + sh.out;
+    return static_cast<T*>(id == kHome ? b.home : id == kTmp0 ? b.tmp0 : b.tmp1) + sh.bank;
+}
+
+// One lifting pass in place over lines of n samples in LDS (line l starts at
+// l * pitch with unit stride for Ax = 0, at l with stride `pitch` otherwise).
+// Lines go in groups small enough that every thread holds the (even, odd)
+// pairs of its K tiles: all of a group's loads happen before its stores
+// (the band split / interleave writes inside the same lines); groups touch
+// disjoint lines. Consecutive lanes take consecutive lines of one tile
+// column, so with an odd row pitch they hit distinct LDS banks.
+template <class T, bool Inv, int Ax, int Tl>
+WZ_D void lds_pass(T* A, uint32_t pitch, uint32_t n, uint32_t lines) {
+    constexpr int P = Tl / 2 + 4;
+    constexpr int K = (sizeof(T) == 4 ? 4 : 2) / (Tl > 8 ? 2 : 1);  // <= 128 VGPRs: two workgroups per CU
+    const uint32_t t = threadIdx.x, nt = (n + Tl - 1) / Tl;
+    const uint32_t per = (K * kLdsThreads) / nt;  // lines of a group (>= 1: nt <= K * threads)
+    const uint32_t h = half_len(n);
+    for (uint32_t l0 = 0; l0 < lines; l0 += per) {
+        const uint32_t gl = min(per, lines - l0), items = gl * nt;
+        T e[K][P], o[K][P];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t w = t + uint32_t(k) * kLdsThreads;
+            if (w >= items) continue;
+            const uint32_t line = l0 + w % gl, tile = w / gl;
+            const uint32_t base = Ax == 0 ? line * pitch : line, stride = Ax == 0 ? 1u : pitch;
+            const int32_t start = int32_t(tile) * Tl;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int32_t q = start - 4 + 2 * j;
+                e[k][j] = A[base + (Inv ? split_pos(q, n) : reflect_pos(q, n)) * stride];
+                o[k][j] = A[base + (Inv ? split_pos(q + 1, n) : reflect_pos(q + 1, n)) * stride];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t w = t + uint32_t(k) * kLdsThreads;
+            if (w >= items) continue;
+            const uint32_t line = l0 + w % gl, tile = w / gl;
+            const uint32_t base = Ax == 0 ? line * pitch : line, stride = Ax == 0 ? 1u : pitch;
+            const uint32_t start = tile * Tl;
+            if constexpr (Inv) lift_inverse<T, P, Ax != 0>(e[k], o[k]);
+            else lift_forward<T, P>(e[k], o[k]);
+#pragma unroll
+            for (int j = 2; j < P - 2; ++j) {
+                const uint32_t q = start + 2u * uint32_t(j - 2);
+                if (q < n) A[base + (Inv ? q : q >> 1) * stride] = e[k][j];
+                if (q + 1 < n) A[base + (Inv ? q + 1 : h + (q >> 1)) * stride] = o[k][j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class T, bool Large, bool Inverse>
+__global__ void __launch_bounds__(kLdsThreads) batch_lds_kernel(const JobShape* __restrict__ shape,
+                                                                const JobRun* __restrict__ run,
+                                                                const Pass* __restrict__ pass,
+                                                                const uint32_t* __restrict__ pfx, uint32_t jobs,
+                                                                StageBanks b, T fill, uint32_t* status) {
+    constexpr uint32_t Cap = PlaneCap<T, Large>::value;
+    constexpr int ZTile = Inverse ? (sizeof(T) == 4 ? 16 : 8) : 8;
+    __shared__ T A[Cap];
+    const uint32_t j = job_of(pfx, jobs, blockIdx.x);
+    const Pass ps = pass[j];
+    if (ps.kind != kPassXYPlane && ps.kind != kPassZCol) return;
+    const uint32_t state = run ? run[j].state : uint32_t(kJobOk);
+    if (state == kJobBad || (state == kJobHole && ps.dst != kOutput)) return;
+    const JobShape& sh = shape[j];
+    const uint32_t t = threadIdx.x, item = blockIdx.x - pfx[j];
+    const Dims full = sh.d, act = ps.act;
+    const uint64_t plane = uint64_t(full.nx) * full.ny;
+    const T* src = stage_bank<T>(b, sh, ps.src);
+    const bool screen = ps.src == kInput;
+    const bool xy = ps.kind == kPassXYPlane;
+    // the staged piece: plane z (rows of act.nx samples at an odd pitch), or
+    // columns x0.. of row y (rows of up to 64 z-ordered columns)
+    const uint32_t z = item, y = xy ? 0u : item % act.ny, x0 = xy ? 0u : (item / act.ny) * kColumns;
+    const uint32_t cols = xy ? act.nx : min(kColumns, act.nx - x0);
+    const uint32_t rows = xy ? act.ny : act.nz;
+    const uint32_t pitch = xy ? (act.nx | 1u) : kColumns;
+    const uint32_t cells = cols * rows;
+    auto gat = [&](uint32_t c, uint32_t r) -> uint64_t {  // global index of staged cell (c, r)
+        return xy ? uint64_t(z) * plane + uint64_t(r) * full.nx + c
+                  : uint64_t(r) * plane + uint64_t(y) * full.nx + x0 + c;
+    };
+    // staging: every thread issues kBurst loads before it writes any of them to LDS
+    constexpr uint32_t kBurst = 8;
+    bool ok = true;
+    for (uint32_t i0 = 0; i0 < cells; i0 += kBurst * kLdsThreads) {
+        T v[kBurst];
+#pragma unroll
+        for (uint32_t k = 0; k < kBurst; ++k) {
+            const uint32_t i = i0 + k * kLdsThreads + t;
+            v[k] = i < cells ? src[gat(i % cols, i / cols)] : T(0);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBurst; ++k) {
+            const uint32_t i = i0 + k * kLdsThreads + t;
+            if (i < cells) {
+                ok = ok && (!screen || __builtin_isfinite(v[k]));
+                A[(i / cols) * pitch + i % cols] = v[k];
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t levels = ps.pad;
+    uint32_t ex[5], ey[5];  // extents of each level inside the piece
+    ex[0] = xy ? act.nx : act.nz;
+    ey[0] = act.ny;
+    for (uint32_t l = 1; l < 5; ++l) {
+        ex[l] = half_len(ex[l - 1]);
+        ey[l] = half_len(ey[l - 1]);
+    }
+    for (uint32_t k = 0; k < levels; ++k) {
+        const uint32_t l = Inverse ? levels - 1 - k : k;
+        if (xy) {
+            if constexpr (Inverse) {
+                lds_pass<T, true, 0, 8>(A, pitch, ex[l], ey[l]);  // X (unfolded), rows of the level box
+                lds_pass<T, true, 1, 8>(A, pitch, ey[l], ex[l]);  // Y (folded), its columns
+            } else {
+                lds_pass<T, false, 1, 8>(A, pitch, ey[l], ex[l]);  // Y
+                lds_pass<T, false, 0, 8>(A, pitch, ex[l], ey[l]);  // X
+            }
+        } else {
+            lds_pass<T, Inverse, 2, ZTile>(A, pitch, ex[l], cols);  // Z (folded) of every column
+        }
+    }
+    bool fin = true;
+    if (ps.dst == kOutput) {
+        const Put<T> put{nullptr, static_cast<T*>(b.out), &sh, state == kJobHole, fill};
+        for (uint32_t i = t; i < cells; i += kLdsThreads) {
+            const uint32_t c = i % cols, r = i / cols;
+            const uint32_t X = xy ? c : x0 + c, Y = xy ? r : y, Zc = xy ? z : r;
+            fin = put(gat(c, r), X, Y, Zc, A[r * pitch + c]) && fin;
+        }
+        if (__ballot(!fin) && (t & 63u) == 0 && status) atomicOr(status, uint32_t(kNonFiniteOutput));
+    } else {
+        T* dst = stage_bank<T>(b, sh, ps.dst);
+        for (uint32_t i = t; i < cells; i += kLdsThreads) {
+            const uint32_t c = i % cols, r = i / cols;
+            dst[gat(c, r)] = A[r * pitch + c];
+        }
+    }
+    if (screen && __ballot(!ok) && (t & 63u) == 0 && b.res) atomicOr(&b.res[j].status, uint32_t(kNonFiniteInput));
+}
+
 // ------------------------------------------------------------------ host plan
 Dims halve(Dims d, bool xy, bool z) {
     if (xy) {
@@ -717,22 +920,61 @@ Pass make_pass(uint8_t kind, uint8_t src, uint8_t dst, Dims act, unsigned width)
     return p;
 }
 
-// The inverse schedule of transform_inverse, as out-of-place passes through
-// the three banks, the last one into the output.
+// Whether a box's XY planes / Z columns fit the LDS-staged passes.
+bool plane_fits(Dims act, unsigned width) {
+    // rows at an odd pitch; every tile column of a line fits one lds_pass group
+    const uint64_t cap = width == 4 ? PlaneCap<float, true>::value : PlaneCap<double, true>::value;
+    const uint64_t k = width == 4 ? 4 : 2;
+    return uint64_t(act.nx | 1u) * act.ny <= cap && (act.nx + 7) / 8 <= k * kLdsThreads &&
+           (act.ny + 7) / 8 <= k * kLdsThreads;
+}
+bool zcol_fits(Dims act, unsigned width) { return act.nz <= (width == 4 ? zcol_cap<float>() : zcol_cap<double>()); }
+
+Pass lds_pass(uint8_t kind, uint8_t src, uint8_t dst, Dims act, uint32_t levels) {
+    Pass p{};
+    p.kind = kind;
+    p.src = src;
+    p.dst = dst;
+    p.pad = uint8_t(levels);
+    p.act = act;
+    p.ntiles = 1;
+    p.work = kind == kPassXYPlane ? act.nz : act.ny * ((act.nx + kColumns - 1) / kColumns);  // workgroups
+    return p;
+}
+
+// The inverse schedule of transform_inverse (cpu/src/transform.cpp:409-447):
+// every level ends in the home bank, the very last pass writes the output.
+// A dyadic level is its Z pass and one XY-plane pass (X then Y) when the
+// level's planes fit LDS; the plane schedule is one XY-plane pass for all XY
+// levels and one Z-column pass for all Z levels when they fit. Otherwise per
+// axis passes through the scratch banks (copied back to home).
 std::vector<Pass> inverse_schedule(const JobShape& sh, unsigned width) {
     std::vector<Pass> out;
-    struct Level {
+    if (sh.lx == 0 && sh.lz == 0) {
+        out.push_back(make_pass(kPassCopy, kHome, kOutput, sh.d, width));
+        return out;
+    }
+    struct Op {
+        uint8_t kind;
         Dims act;
-        std::vector<uint8_t> axes;
+        uint32_t levels;
     };
-    std::vector<Level> levels;
+    std::vector<std::vector<Op>> levels;  // ops of each level (one level ends in home)
+    auto axes = [&](Dims act, std::initializer_list<uint8_t> ks) {
+        std::vector<Op> v;
+        for (uint8_t k : ks) v.push_back({k, act, 1});
+        return v;
+    };
     if (sh.dyadic) {
         Dims sizes[5], act = sh.d;
         for (unsigned l = 0; l < sh.lx; ++l) {
             sizes[l] = act;
             act = halve(act, true, true);
         }
-        for (int l = int(sh.lx) - 1; l >= 0; --l) levels.push_back({sizes[l], {kPassZ, kPassX, kPassY}});
+        for (int l = int(sh.lx) - 1; l >= 0; --l) {
+            if (plane_fits(sizes[l], width)) levels.push_back({{kPassZ, sizes[l], 1}, {kPassXYPlane, sizes[l], 1}});
+            else levels.push_back(axes(sizes[l], {kPassZ, kPassX, kPassY}));
+        }
     } else {
         Dims xy[5], zs[5], act = sh.d;
         for (unsigned l = 0; l < sh.lx; ++l) {
@@ -744,77 +986,133 @@ std::vector<Pass> inverse_schedule(const JobShape& sh, unsigned width) {
             zs[l] = act;
             act = halve(act, false, true);
         }
-        for (int l = int(sh.lx) - 1; l >= 0; --l) levels.push_back({xy[l], {kPassX, kPassY}});
-        for (int l = int(sh.lz) - 1; l >= 0; --l) levels.push_back({zs[l], {kPassZ}});
-    }
-    if (levels.empty()) {
-        out.push_back(make_pass(kPassCopy, kHome, kOutput, sh.d, width));
-        return out;
+        if (sh.lx && plane_fits(sh.d, width)) levels.push_back({{kPassXYPlane, sh.d, sh.lx}});
+        else
+            for (int l = int(sh.lx) - 1; l >= 0; --l) levels.push_back(axes(xy[l], {kPassX, kPassY}));
+        if (sh.lz && zcol_fits(sh.d, width)) levels.push_back({{kPassZCol, sh.d, sh.lz}});
+        else
+            for (int l = int(sh.lz) - 1; l >= 0; --l) levels.push_back(axes(zs[l], {kPassZ}));
     }
     for (size_t k = 0; k < levels.size(); ++k) {
         const bool last = k + 1 == levels.size();
-        const Level& L = levels[k];
         const uint8_t fin = last ? uint8_t(kOutput) : uint8_t(kHome);
         uint8_t src = kHome;
-        for (size_t a = 0; a < L.axes.size(); ++a) {
-            uint8_t dst = a + 1 == L.axes.size() ? fin : uint8_t(a & 1 ? kTmp1 : kTmp0);
-            if (dst == src) dst = kTmp0;
-            out.push_back(make_pass(L.axes[a], src, dst, L.act, width));
+        const std::vector<Op>& L = levels[k];
+        for (size_t a = 0; a < L.size(); ++a) {
+            const Op& op = L[a];
+            const bool lds = op.kind == kPassXYPlane || op.kind == kPassZCol;
+            uint8_t dst;
+            if (a + 1 == L.size()) dst = fin;
+            else dst = src == kTmp0 ? uint8_t(kTmp1) : uint8_t(kTmp0);
+            if (dst == src && !lds) dst = src == kTmp0 ? uint8_t(kTmp1) : uint8_t(kTmp0);
+            out.push_back(lds ? lds_pass(op.kind, src, dst, op.act, op.levels) : make_pass(op.kind, src, dst, op.act, width));
             src = dst;
         }
-        if (src != fin) out.push_back(make_pass(kPassCopy, src, fin, L.act, width));
+        if (src != fin) out.push_back(make_pass(kPassCopy, src, fin, L.back().act, width));
     }
     return out;
 }
 
-// The forward schedule of transform_forward (cpu/src/transform.cpp:425-456) as
-// out-of-place passes: the first reads the job's input field, each level
-// ends in the home bank (a level of one pass that starts in home goes through
-// tmp0 and is copied back). Forward tiles do not change a bit.
+// The forward schedule of transform_forward (cpu/src/transform.cpp:425-456):
+// the first pass reads the job's input, every level ends in the home bank.
+// A dyadic level is one XY-plane pass (Y then X) and its Z pass; the plane
+// schedule is one Z-column pass and one XY-plane pass, when they fit LDS.
+// Forward tiles do not change a bit.
 std::vector<Pass> forward_schedule(const JobShape& sh, unsigned width) {
     std::vector<Pass> out;
-    struct Level {
-        Dims act;
-        std::vector<uint8_t> axes;
-    };
-    std::vector<Level> levels;
-    if (sh.dyadic) {
-        Dims act = sh.d;
-        for (unsigned l = 0; l < sh.lx; ++l) {
-            levels.push_back({act, {kPassY, kPassX, kPassZ}});
-            act = halve(act, true, true);
-        }
-    } else {
-        Dims act = sh.d;
-        for (unsigned l = 0; l < sh.lz; ++l) {
-            levels.push_back({act, {kPassZ}});
-            act = halve(act, false, true);
-        }
-        act = sh.d;
-        for (unsigned l = 0; l < sh.lx; ++l) {
-            levels.push_back({act, {kPassY, kPassX}});
-            act = halve(act, true, false);
-        }
-    }
-    if (levels.empty()) {
+    if (sh.lx == 0 && sh.lz == 0) {
         out.push_back(make_pass(kPassCopy, kInput, kHome, sh.d, width));
         return out;
     }
+    struct Op {
+        uint8_t kind;
+        Dims act;
+        uint32_t levels;
+    };
+    std::vector<std::vector<Op>> levels;
+    if (sh.dyadic) {
+        Dims act = sh.d;
+        for (unsigned l = 0; l < sh.lx; ++l) {
+            if (plane_fits(act, width)) levels.push_back({{kPassXYPlane, act, 1}, {kPassZ, act, 1}});
+            else levels.push_back({{kPassY, act, 1}, {kPassX, act, 1}, {kPassZ, act, 1}});
+            act = halve(act, true, true);
+        }
+    } else {
+        if (sh.lz && zcol_fits(sh.d, width)) {
+            levels.push_back({{kPassZCol, sh.d, sh.lz}});
+        } else {
+            Dims act = sh.d;
+            for (unsigned l = 0; l < sh.lz; ++l) {
+                levels.push_back({{kPassZ, act, 1}});
+                act = halve(act, false, true);
+            }
+        }
+        if (sh.lx && plane_fits(sh.d, width)) {
+            levels.push_back({{kPassXYPlane, sh.d, sh.lx}});
+        } else {
+            Dims act = sh.d;
+            for (unsigned l = 0; l < sh.lx; ++l) {
+                levels.push_back({{kPassY, act, 1}, {kPassX, act, 1}});
+                act = halve(act, true, false);
+            }
+        }
+    }
     uint8_t src = kInput;
-    for (const Level& L : levels) {
-        for (size_t a = 0; a < L.axes.size(); ++a) {
-            const bool last = a + 1 == L.axes.size();
-            uint8_t dst = last ? uint8_t(kHome) : (src == kTmp0 ? uint8_t(kTmp1) : uint8_t(kTmp0));
-            if (dst == src) dst = kTmp0;
-            out.push_back(make_pass(L.axes[a], src, dst, L.act, width));
+    for (const std::vector<Op>& L : levels) {
+        for (size_t a = 0; a < L.size(); ++a) {
+            const Op& op = L[a];
+            const bool lds = op.kind == kPassXYPlane || op.kind == kPassZCol;
+            uint8_t dst;
+            if (a + 1 == L.size()) dst = kHome;
+            else dst = src == kTmp0 ? uint8_t(kTmp1) : uint8_t(kTmp0);
+            if (dst == src && !lds) dst = src == kTmp0 ? uint8_t(kTmp1) : uint8_t(kTmp0);
+            out.push_back(lds ? lds_pass(op.kind, src, dst, op.act, op.levels) : make_pass(op.kind, src, dst, op.act, width));
             src = dst;
         }
         if (src != kHome) {
-            out.push_back(make_pass(kPassCopy, src, kHome, L.act, width));
+            out.push_back(make_pass(kPassCopy, src, kHome, L.back().act, width));
             src = kHome;
         }
     }
     return out;
+}
+
+// Per-step launch tables: every job's pass at step st, the block prefixes of
+// the per-axis kernel (256-thread blocks of work items) and of the LDS kernel
+// (one workgroup per plane / column row), and whether that step's LDS pieces
+// need the large LDS variant.
+void build_steps(const std::vector<std::vector<Pass>>& sched, uint32_t J, unsigned width,
+                 std::vector<std::vector<Pass>>& step, std::vector<std::vector<uint32_t>>& pfx,
+                 std::vector<std::vector<uint32_t>>& lpfx, std::vector<uint8_t>& large) {
+    size_t steps = 0;
+    for (const auto& v : sched) steps = std::max(steps, v.size());
+    step.assign(steps, std::vector<Pass>(J));
+    pfx.assign(steps, std::vector<uint32_t>(J + 1, 0));
+    lpfx.assign(steps, std::vector<uint32_t>(J + 1, 0));
+    large.assign(steps, 0);
+    const uint64_t small = width == 4 ? PlaneCap<float, false>::value : PlaneCap<double, false>::value;
+    for (size_t st = 0; st < steps; ++st) {
+        uint64_t blocks = 0, lblocks = 0;
+        for (uint32_t j = 0; j < J; ++j) {
+            Pass ps{};
+            ps.kind = kPassNone;
+            if (st < sched[j].size()) ps = sched[j][st];
+            step[st][j] = ps;
+            pfx[st][j] = uint32_t(blocks);
+            lpfx[st][j] = uint32_t(lblocks);
+            if (ps.kind == kPassXYPlane || ps.kind == kPassZCol) {
+                lblocks += ps.work;
+                const uint64_t piece = ps.kind == kPassXYPlane ? uint64_t(ps.act.nx | 1u) * ps.act.ny
+                                                               : uint64_t(ps.act.nz) * kColumns;
+                if (piece > small) large[st] = 1;
+            } else if (ps.kind != kPassNone) {
+                blocks += (uint64_t(ps.work) + 255) / 256;
+            }
+        }
+        if (blocks > 0x7FFFFFFFull || lblocks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
+        pfx[st][J] = uint32_t(blocks);
+        lpfx[st][J] = uint32_t(lblocks);
+    }
 }
 
 // Byte offsets of one batched compress call's workspace (all 256-byte aligned).
@@ -829,15 +1127,19 @@ struct HostPlan {
     std::vector<JobShape> shape;
     std::vector<uint32_t> chunk_pfx, brick_pfx;
     std::vector<std::vector<Pass>> step;      // step[s][job]
-    std::vector<std::vector<uint32_t>> pfx;   // block prefixes per step
+    std::vector<std::vector<uint32_t>> pfx;   // block prefixes per step (per-axis kernel)
+    std::vector<std::vector<uint32_t>> lpfx;  // block prefixes per step (LDS kernel)
+    std::vector<uint8_t> large;               // step needs the large LDS variant
     uint64_t bank_elems = 0, mag_elems = 0, sign_words = 0, out_elems = 0;
     // compress
     std::vector<std::vector<Pass>> fstep;     // forward passes, fstep[s][job]
-    std::vector<std::vector<uint32_t>> fpfx;
+    std::vector<std::vector<uint32_t>> fpfx, flpfx;
+    std::vector<uint8_t> flarge;
     std::vector<JobComp> comp;
     std::vector<uint32_t> group_pfx, tile_pfx, pblock_pfx, place_pfx;
     uint64_t archive_bytes = 0;               // archive slots of all jobs
     CompressLayout cl{};
+    std::vector<std::pair<Dims, uint32_t>> rank_geo;  // distinct (geometry, z band) of the rank table sets
 };
 
 // Geometry checks of the single-field codec (codec.hip geometry()).
@@ -916,21 +1218,8 @@ void finish_compress_plan(HostPlan& p) {
     p.pblock_pfx[J] = uint32_t(pblocks);
     p.place_pfx[J] = uint32_t(places);
     p.archive_bytes = arc;
-    p.fstep.assign(steps, std::vector<Pass>(J));
-    p.fpfx.assign(steps, std::vector<uint32_t>(J + 1, 0));
-    for (size_t st = 0; st < steps; ++st) {
-        uint64_t blocks = 0;
-        for (uint32_t j = 0; j < J; ++j) {
-            Pass ps{};
-            ps.kind = kPassNone;
-            if (st < sched[j].size()) ps = sched[j][st];
-            p.fstep[st][j] = ps;
-            p.fpfx[st][j] = uint32_t(blocks);
-            if (ps.kind != kPassNone) blocks += (uint64_t(ps.work) + 255) / 256;
-        }
-        if (blocks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
-        p.fpfx[st][J] = uint32_t(blocks);
-    }
+    (void)steps;
+    build_steps(sched, J, w, p.fstep, p.fpfx, p.flpfx, p.flarge);
     CompressLayout& L = p.cl;
     uint64_t at = 0;
     auto take = [&](uint64_t bytes) {
@@ -971,8 +1260,16 @@ void finish_plan(HostPlan& p) {
     std::vector<std::vector<Pass>> sched(J);
     size_t steps = 0;
     uint64_t bank = 0, mag = 0, sign = 0, chunks = 0, bricks = 0;
+    p.rank_geo.clear();
     for (uint32_t j = 0; j < J; ++j) {
         JobShape& s = p.shape[j];
+        uint32_t set = 0;
+        while (set < p.rank_geo.size() &&
+               !(p.rank_geo[set].first.nx == s.d.nx && p.rank_geo[set].first.ny == s.d.ny &&
+                 p.rank_geo[set].first.nz == s.d.nz && p.rank_geo[set].second == s.bz))
+            ++set;
+        if (set == p.rank_geo.size()) p.rank_geo.push_back({s.d, s.bz});
+        s.rank = set;
         s.bank = bank;
         s.mag = mag;
         s.sign = sign;
@@ -992,21 +1289,8 @@ void finish_plan(HostPlan& p) {
     p.bank_elems = bank;
     p.mag_elems = mag;
     p.sign_words = sign;
-    p.step.assign(steps, std::vector<Pass>(J));
-    p.pfx.assign(steps, std::vector<uint32_t>(J + 1, 0));
-    for (size_t st = 0; st < steps; ++st) {
-        uint64_t blocks = 0;
-        for (uint32_t j = 0; j < J; ++j) {
-            Pass ps{};
-            ps.kind = kPassNone;
-            if (st < sched[j].size()) ps = sched[j][st];
-            p.step[st][j] = ps;
-            p.pfx[st][j] = uint32_t(blocks);
-            if (ps.kind != kPassNone) blocks += (uint64_t(ps.work) + 255) / 256;
-        }
-        if (blocks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
-        p.pfx[st][J] = uint32_t(blocks);
-    }
+    (void)steps;
+    build_steps(sched, J, p.width, p.step, p.pfx, p.lpfx, p.large);
     finish_compress_plan(p);
 }
 
@@ -1025,12 +1309,16 @@ struct DevicePlan {
     uint32_t *group_pfx = nullptr, *tile_pfx = nullptr, *pblock_pfx = nullptr, *place_pfx = nullptr;
     Pass* fpass = nullptr;          // forward steps x jobs
     uint32_t* fpass_pfx = nullptr;  // forward steps x (jobs + 1)
+    uint32_t* lds_pfx = nullptr;    // inverse steps x (jobs + 1), LDS kernel
+    uint32_t* flds_pfx = nullptr;   // forward steps x (jobs + 1), LDS kernel
     EncodeResult* last_res = nullptr;  // results of the last compress (device, in its workspace)
+    uint16_t* ranks = nullptr;         // kRankSet entries per distinct geometry
     std::string error;
 
     ~DevicePlan() {
-        if (mem) hipFree(mem);
-        if (host_status) hipHostFree(host_status);
+        if (mem) (void)hipFree(mem);
+        if (ranks) (void)hipFree(ranks);
+        if (host_status) (void)hipHostFree(host_status);
     }
 
     void upload(hipStream_t s) {
@@ -1050,7 +1338,7 @@ struct DevicePlan {
         const size_t FS = p.fstep.size();
         const size_t o_comp = carve(sizeof(JobComp) * J), o_gp = carve(4 * (J + 1)), o_tp = carve(4 * (J + 1)),
                      o_pbp = carve(4 * (J + 1)), o_plp = carve(4 * (J + 1)), o_fpass = carve(sizeof(Pass) * J * FS),
-                     o_fpp = carve(4 * (J + 1) * FS);
+                     o_fpp = carve(4 * (J + 1) * FS), o_lp = carve(4 * (J + 1) * S), o_flp = carve(4 * (J + 1) * FS);
         hip_check(hipMalloc(reinterpret_cast<void**>(&mem), used), "batch plan allocation");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_status), 4), "batch status");
         shape = reinterpret_cast<JobShape*>(mem + o_shape);
@@ -1069,11 +1357,14 @@ struct DevicePlan {
         place_pfx = reinterpret_cast<uint32_t*>(mem + o_plp);
         fpass = reinterpret_cast<Pass*>(mem + o_fpass);
         fpass_pfx = reinterpret_cast<uint32_t*>(mem + o_fpp);
+        lds_pfx = reinterpret_cast<uint32_t*>(mem + o_lp);
+        flds_pfx = reinterpret_cast<uint32_t*>(mem + o_flp);
         std::vector<Pass> flat;
-        std::vector<uint32_t> flat_pfx;
+        std::vector<uint32_t> flat_pfx, flat_lpfx;
         for (size_t st = 0; st < S; ++st) {
             flat.insert(flat.end(), p.step[st].begin(), p.step[st].end());
             flat_pfx.insert(flat_pfx.end(), p.pfx[st].begin(), p.pfx[st].end());
+            flat_lpfx.insert(flat_lpfx.end(), p.lpfx[st].begin(), p.lpfx[st].end());
         }
         hip_check(hipMemcpyAsync(shape, p.shape.data(), sizeof(JobShape) * J, hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(chunk_pfx, p.chunk_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
@@ -1082,12 +1373,15 @@ struct DevicePlan {
             hip_check(hipMemcpyAsync(pass, flat.data(), sizeof(Pass) * flat.size(), hipMemcpyHostToDevice, s), "plan");
             hip_check(hipMemcpyAsync(pass_pfx, flat_pfx.data(), 4 * flat_pfx.size(), hipMemcpyHostToDevice, s),
                       "plan");
+            hip_check(hipMemcpyAsync(lds_pfx, flat_lpfx.data(), 4 * flat_lpfx.size(), hipMemcpyHostToDevice, s),
+                      "plan");
         }
         std::vector<Pass> fflat;
-        std::vector<uint32_t> fflat_pfx;
+        std::vector<uint32_t> fflat_pfx, fflat_lpfx;
         for (size_t st = 0; st < FS; ++st) {
             fflat.insert(fflat.end(), p.fstep[st].begin(), p.fstep[st].end());
             fflat_pfx.insert(fflat_pfx.end(), p.fpfx[st].begin(), p.fpfx[st].end());
+            fflat_lpfx.insert(fflat_lpfx.end(), p.flpfx[st].begin(), p.flpfx[st].end());
         }
         hip_check(hipMemcpyAsync(comp, p.comp.data(), sizeof(JobComp) * J, hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(group_pfx, p.group_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
@@ -1099,8 +1393,28 @@ struct DevicePlan {
                       "plan");
             hip_check(hipMemcpyAsync(fpass_pfx, fflat_pfx.data(), 4 * fflat_pfx.size(), hipMemcpyHostToDevice, s),
                       "plan");
+            hip_check(hipMemcpyAsync(flds_pfx, fflat_lpfx.data(), 4 * fflat_lpfx.size(), hipMemcpyHostToDevice, s),
+                      "plan");
         }
+        hip_check(hipMalloc(reinterpret_cast<void**>(&ranks), std::max<size_t>(1, p.rank_geo.size()) * kRankSet * 2),
+                  "batch rank tables");
+        for (size_t k = 0; k < p.rank_geo.size(); ++k)
+            build_rank_tables(p.rank_geo[k].first, p.rank_geo[k].second, ranks + k * kRankSet,
+                              ranks + k * kRankSet + 8 * 8192, s);
         hip_check(hipStreamSynchronize(s), "plan upload");
+    }
+
+    // The LDS-staged passes of one step (blocks workgroups), small or large variant.
+    template <class T, bool Inverse>
+    void launch_lds(uint32_t blocks, bool large_lds, const JobRun* r, const Pass* ps, const uint32_t* pf,
+                    const StageBanks& sb, T fill, uint32_t* st, hipStream_t s) const {
+        if (!blocks) return;
+        if (large_lds)
+            hipLaunchKernelGGL((batch_lds_kernel<T, true, Inverse>), dim3(blocks), dim3(kLdsThreads), 0, s, shape, r,
+                               ps, pf, host.jobs, sb, fill, st);
+        else
+            hipLaunchKernelGGL((batch_lds_kernel<T, false, Inverse>), dim3(blocks), dim3(kLdsThreads), 0, s, shape,
+                               r, ps, pf, host.jobs, sb, fill, st);
     }
 
     // Enqueues the batched compress on `s`: no synchronisation, allocation or
@@ -1137,16 +1451,19 @@ struct DevicePlan {
         last_res = b.res;
         zero_words(ws + L.zero0, (L.zero_end - L.zero0) / 4, s);
         zero_words(run, uint64_t(J) * sizeof(JobRun) / 4, s);  // every job decodable for the inverse passes
+        const StageBanks fsb{input, b.home, b.tmp0, b.tmp1, nullptr, b.res};
         for (size_t st = 0; st < p.fstep.size(); ++st) {
             const uint32_t blocks = p.fpfx[st][J];
             if (blocks)
                 hipLaunchKernelGGL((batch_fwd_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape,
                                    fpass + st * J, fpass_pfx + st * (J + 1), J, b);
+            launch_lds<T, false>(p.flpfx[st][J], p.flarge[st], nullptr, fpass + st * J, flds_pfx + st * (J + 1), fsb,
+                                 T(0), nullptr, s);
         }
         zero_words(b.tmp1, p.sign_words, s);  // sign words: bricks OR their edge words
         if (p.brick_pfx[J])
             hipLaunchKernelGGL((batch_quantize_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, comp,
-                               brick_pfx, J, quantum, b);
+                               brick_pfx, J, ranks, quantum, b);
         if (p.chunk_pfx[J])
             hipLaunchKernelGGL(batch_encode_kernel, dim3(p.chunk_pfx[J]), dim3(256), 0, s, shape, comp, chunk_pfx, J,
                                b);
@@ -1154,11 +1471,14 @@ struct DevicePlan {
             hipLaunchKernelGGL(batch_pack_kernel, dim3(p.group_pfx[J]), dim3(kPackThreads), 0, s, shape, comp,
                                group_pfx, J, b);
         const Banks bk{b.home, b.tmp0, b.tmp1, b.recon};
+        const StageBanks isb{nullptr, b.home, b.tmp0, b.tmp1, b.recon, nullptr};
         for (size_t st = 0; st < p.step.size(); ++st) {
             const uint32_t blocks = p.pfx[st][J];
             if (blocks)
                 hipLaunchKernelGGL((batch_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape, run, pass + st * J,
                                    pass_pfx + st * (J + 1), J, bk, T(0), status);
+            launch_lds<T, true>(p.lpfx[st][J], p.large[st], run, pass + st * J, lds_pfx + st * (J + 1), isb, T(0),
+                                status, s);
         }
         if (p.tile_pfx[J])
             hipLaunchKernelGGL((batch_repair_kernel<T>), dim3(p.tile_pfx[J]), dim3(256), 0, s, shape, comp, tile_pfx,
@@ -1196,15 +1516,18 @@ struct DevicePlan {
                                J, static_cast<uint16_t*>(bk.tmp0), static_cast<uint32_t*>(bk.tmp1), status);
         if (p.brick_pfx[J])
             hipLaunchKernelGGL((batch_dequant_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, run,
-                               brick_pfx, J, static_cast<const uint16_t*>(bk.tmp0),
+                               brick_pfx, J, ranks, static_cast<const uint16_t*>(bk.tmp0),
                                static_cast<const uint32_t*>(bk.tmp1), static_cast<T*>(bk.home));
         hipLaunchKernelGGL((batch_qpatch_kernel<T>), dim3(256), dim3(256), 0, s, packed, shape, run, nq_pfx, J,
                            static_cast<T*>(bk.home), status);
+        const StageBanks sb{nullptr, bk.home, bk.tmp0, bk.tmp1, bk.out, nullptr};
         for (size_t st = 0; st < p.step.size(); ++st) {
             const uint32_t blocks = p.pfx[st][J];
-            if (!blocks) continue;
-            hipLaunchKernelGGL((batch_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape, run,
-                               pass + st * J, pass_pfx + st * (J + 1), J, bk, T(fill), status);
+            if (blocks)
+                hipLaunchKernelGGL((batch_pass_kernel<T>), dim3(blocks), dim3(256), 0, s, shape, run,
+                                   pass + st * J, pass_pfx + st * (J + 1), J, bk, T(fill), status);
+            launch_lds<T, true>(p.lpfx[st][J], p.large[st], run, pass + st * J, lds_pfx + st * (J + 1), sb, T(fill),
+                                status, s);
         }
         hipLaunchKernelGGL((batch_rpatch_kernel<T>), dim3(256), dim3(256), 0, s, packed, shape, run, np_pfx, J,
                            static_cast<T*>(bk.out), status);

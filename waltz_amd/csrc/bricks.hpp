@@ -42,11 +42,29 @@ WZ_D void quantize_brick(T* __restrict__ coeff, const BrickGeometry& g, T q, T i
     __syncthreads();
     const uint32_t shift = uint32_t(k.base & 31u);
     bool bad = false;
-    for (uint32_t i = threadIdx.x; i < vol; i += 256) {
-        const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
-        const uint64_t at = (uint64_t(k.z0 + z) * g.d.ny + (k.y0 + y)) * g.d.nx + k.x0 + x;
-        const uint32_t r = rank(k, i, x, y, z);
-        const T scaled = coeff[at] * inv;
+    // a burst of kBurst samples per thread: their coefficient and rank loads
+    // are all issued before the first store (the stores could alias the
+    // loads of the next samples, so the compiler would not hoist them)
+    constexpr uint32_t kBurst = 8;
+    for (uint32_t i0 = 0; i0 < vol; i0 += kBurst * 256u) {
+      T cv[kBurst];
+      uint32_t rv[kBurst];
+      uint64_t av[kBurst];
+#pragma unroll
+      for (uint32_t u = 0; u < kBurst; ++u) {
+        const uint32_t i = i0 + u * 256u + threadIdx.x;
+        const uint32_t ic = i < vol ? i : 0u;
+        const uint32_t x = ic % k.bx, y = (ic / k.bx) % k.by, z = ic / (k.bx * k.by);
+        av[u] = (uint64_t(k.z0 + z) * g.d.ny + (k.y0 + y)) * g.d.nx + k.x0 + x;
+        rv[u] = rank(k, ic, x, y, z);
+        cv[u] = coeff[av[u]];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kBurst; ++u) {
+        if (i0 + u * 256u + threadIdx.x >= vol) continue;
+        const uint64_t at = av[u];
+        const uint32_t r = rv[u];
+        const T scaled = cv[u] * inv;
         // cpu/src/quantize_avx2.cpp:45-62: finite, |scaled| < 2^63, llrint, INT32 saturation.
         if (!__builtin_isfinite(scaled) || !(abs_t(scaled) < T(9223372036854775808.0))) bad = true;
         const T rounded = rint_t(scaled);
@@ -64,6 +82,7 @@ WZ_D void quantize_brick(T* __restrict__ coeff, const BrickGeometry& g, T q, T i
                 static_cast<uint32_t*>(qo.payload)[slot] = uint32_t(m);
             }
         }
+      }
     }
     if (bad) atomicOr(status, uint32_t(kQuantDomain));
     __syncthreads();

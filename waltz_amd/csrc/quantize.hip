@@ -74,12 +74,26 @@ __global__ void __launch_bounds__(256) dequantize_kernel(const uint16_t* __restr
                                                          const uint16_t* __restrict__ rank, T* __restrict__ out) {
     const Brick k = brick_of(g, blockIdx.x);
     const uint32_t vol = k.bx * k.by * k.bz;
-    for (uint32_t i = threadIdx.x; i < vol; i += 256) {
-        const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
-        const uint64_t at = (uint64_t(k.z0 + z) * g.d.ny + (k.y0 + y)) * g.d.nx + k.x0 + x;
-        const uint64_t p = k.base + rank[k.shape * 8192 + i];
-        const T v = T(mag[p]) * q;  // cpu/src/codec.cpp:146-147
-        out[at] = ((sign[p >> 5] >> (p & 31u)) & 1u) ? -v : v;
+    constexpr uint32_t kBurst = 8;  // all loads of a burst issued before its stores
+    for (uint32_t i0 = 0; i0 < vol; i0 += kBurst * 256u) {
+        uint32_t mv[kBurst], sv[kBurst], pv[kBurst];
+#pragma unroll
+        for (uint32_t u = 0; u < kBurst; ++u) {
+            const uint32_t i = i0 + u * 256u + threadIdx.x;
+            const uint64_t p = k.base + rank[k.shape * 8192 + (i < vol ? i : 0u)];
+            pv[u] = uint32_t(p & 31u);
+            mv[u] = mag[p];
+            sv[u] = sign[p >> 5];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBurst; ++u) {
+            const uint32_t i = i0 + u * 256u + threadIdx.x;
+            if (i >= vol) continue;
+            const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
+            const uint64_t at = (uint64_t(k.z0 + z) * g.d.ny + (k.y0 + y)) * g.d.nx + k.x0 + x;
+            const T v = T(mv[u]) * q;  // cpu/src/codec.cpp:146-147
+            out[at] = ((sv[u] >> pv[u]) & 1u) ? -v : v;
+        }
     }
 }
 
