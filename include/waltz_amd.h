@@ -51,10 +51,11 @@ int waltz_amd_transform_adjoint_async(int width, int of_inverse, const void* inp
                                       void* tmp1, uint32_t nx, uint32_t ny, uint32_t nz, unsigned levels_xy,
                                       unsigned levels_z, int dyadic, void* stream);
 
-/* Bit-neutral launch tuning of the dyadic schedule (no reference
- * counterpart): autotune times the fused level kernels of both directions at
- * each candidate z segment for this geometry and type on this device and keeps
- * the fastest (process-wide, per geometry); set_tuning fixes the number of
+/* Bit-neutral launch tuning (no reference counterpart): autotune times the
+ * fused level kernels of both directions at each candidate z segment for this
+ * geometry and type on this device, then the per-axis passes' line tiles and
+ * LDS pair limit, and keeps the fastest (process-wide, per geometry and
+ * direction); set_tuning fixes the number of
  * fused levels and their five z segments (positive multiples of 16) for one
  * direction; clear_tuning restores the built-in heuristic; tuning_report
  * writes the table as JSON (returns its full length). Results never depend on
@@ -62,6 +63,13 @@ int waltz_amd_transform_adjoint_async(int width, int of_inverse, const void* inp
 int waltz_amd_autotune(int width, uint32_t nx, uint32_t ny, uint32_t nz, void* stream);
 int waltz_amd_set_tuning(int width, uint32_t nx, uint32_t ny, uint32_t nz, int inverse, unsigned fused_levels,
                          const uint32_t* z_segments);
+/* The per-axis passes' share of the tuning (the plane schedule, and dyadic
+ * levels that are not fused): line tiles of the X, Y and Z passes (8, 16 or
+ * 32; an inverse direction uses only its X tile, its Y and Z tiles being part
+ * of the format) and the largest plane, in samples, whose X and Y passes run
+ * as one LDS-staged launch (0, 1024 or 4096). Results never depend on them. */
+int waltz_amd_set_pass_tuning(int width, uint32_t nx, uint32_t ny, uint32_t nz, int inverse, const uint32_t* tiles,
+                              uint32_t lds_pair_max);
 void waltz_amd_clear_tuning(void);
 size_t waltz_amd_tuning_report(char* buf, size_t cap);
 

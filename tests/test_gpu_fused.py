@@ -89,3 +89,48 @@ def test_autotuned_choice_is_bit_neutral(wz, oracle):
         check_codec(waltz_amd, oracle, v, 1e-3)
     finally:
         stages.clear_tuning()
+
+
+# per-axis pass choices: forward X/Y/Z line tiles, inverse X tile, LDS pair limit
+PASS_TUNINGS = [((8, 8, 8), 0), ((16, 32, 16), 1024), ((32, 16, 32), 4096), ((32, 32, 8), 0), ((16, 8, 32), 4096)]
+
+
+@pytest.mark.parametrize("pt", PASS_TUNINGS, ids=lambda p: "t" + "-".join(map(str, p[0])) + f"_pair{p[1]}")
+@pytest.mark.parametrize("case", [((24, 100, 120), "plane", np.float64), ((40, 68, 72), "plane", np.float32),
+                                  ((80, 100, 120), "dyadic-passes", np.float32)], ids=lambda c: c[1] + str(c[0][0]))
+def test_forced_pass_tunings(wz, oracle, case, pt):
+    """Line tiles and the LDS pair limit of the per-axis passes change no bit:
+    the plane schedule (all axis passes) and a dyadic field with every level
+    left to the passes (no fused level)."""
+    waltz_amd, stages = wz
+    shape, kind, dtype = case
+    tiles, pair = pt
+    if kind == "dyadic-passes":
+        for inverse in (False, True):
+            stages.set_tuning(shape, TDT[dtype], inverse, 0, (16, 16, 16, 16, 16))
+    for inverse in (False, True):
+        stages.set_pass_tuning(shape, TDT[dtype], inverse, tiles, pair)
+    try:
+        from waltz_amd.autograd import default_levels
+        lxy, lz, dyadic = default_levels(shape)
+        v = field(shape, "smooth", dtype) + field(shape, "noise", dtype, seed=5) / 16
+        c = stages.transform(torch.from_numpy(v).cuda(), lxy, lz, dyadic, inverse=False).cpu().numpy()
+        assert same_bits(c, oracle.transform(v, lxy, lz, dyadic, False))
+        r = stages.transform(torch.from_numpy(v).cuda(), lxy, lz, dyadic, inverse=True).cpu().numpy()
+        assert same_bits(r, oracle.transform(v, lxy, lz, dyadic, True))
+        check_codec(waltz_amd, oracle, v, 1e-3)
+    finally:
+        stages.clear_tuning()
+
+
+def test_autotuned_plane_schedule_is_bit_neutral(wz, oracle):
+    waltz_amd, stages = wz
+    shape = (24, 100, 120)
+    stages.autotune(shape, torch.float64)
+    rep = stages.tuning_report()
+    assert any(r["shape_xyz"] == [120, 100, 24] and "pass_tiles_xyz" in r for r in rep)
+    v = field(shape, "smooth", np.float64)
+    try:
+        check_codec(waltz_amd, oracle, v, 1e-6)
+    finally:
+        stages.clear_tuning()

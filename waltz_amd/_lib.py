@@ -65,6 +65,7 @@ _SIGS.update({
     "waltz_amd_autotune": (C.c_int, [C.c_int, _U32, _U32, _U32, _P]),
     "waltz_amd_set_tuning": (C.c_int, [C.c_int, _U32, _U32, _U32, C.c_int, _U, C.POINTER(_U32)]),
     "waltz_amd_clear_tuning": (None, []),
+    "waltz_amd_set_pass_tuning": (C.c_int, [C.c_int, _U32, _U32, _U32, C.c_int, C.POINTER(_U32), _U32]),
     "waltz_amd_tuning_report": (_SZ, [C.c_char_p, _SZ]),
 })
 OPTIONAL = {

@@ -1,11 +1,14 @@
-// Autotuner of the bit-neutral launch choices of the dyadic schedule (no
-// reference counterpart). For one geometry and type it times, on this device
-// and a synthetic finite field, every level's fused kernel of both directions
-// at each candidate z segment, keeps the fastest per level, and then picks the
+// Autotuner of the bit-neutral launch choices (no reference counterpart). For
+// one geometry and type it times, on this device and a synthetic finite
+// field: for the dyadic schedule every level's fused kernel of both
+// directions at each candidate z segment (the fastest per level), then the
 // number of leading levels to run fused (the rest as axis passes) from whole
-// transforms. Every candidate produces the same bits (a z segment only
-// partitions the z range among workgroups; fused and pass forms are bitwise
-// equal), so the choice changes speed only.
+// transforms; for every schedule the per-axis passes' line tiles (X, Y, Z;
+// the inverse only X) and the largest plane whose X and Y passes run as one
+// LDS-staged launch. Every candidate produces the same bits (a z segment only
+// partitions the z range among workgroups, a line tile only the line among
+// threads, fused and pass forms are bitwise equal), so the choice changes
+// speed only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,13 +78,57 @@ std::vector<uint32_t> candidates(uint32_t nz) {
 }
 }  // namespace
 
+// The per-axis passes' line tiles and LDS pair limit for one direction, by
+// coordinate descent over whole transforms (each parameter in turn over its
+// candidates, the others fixed; a change has to win by 1.5%).
+template <class T>
+void tune_passes(Dims full, unsigned lxy, unsigned lz, bool dyadic, bool inverse, T* x, T* c, T* t0, T* t1,
+                 Timer& timer, hipStream_t s) {
+    Tuning t = tuning_for(full, lxy, sizeof(T), inverse);
+    auto measure = [&](const Tuning& cand) {
+        set_tuning(full, sizeof(T), inverse, cand);
+        if (inverse) return timer.best_ms([&] { transform_inverse<T>(c, x, t0, t1, full, lxy, lz, dyadic, s); }, s, 5);
+        return timer.best_ms([&] { transform_forward<T>(x, c, t0, t1, full, lxy, lz, dyadic, nullptr, s); }, s, 5);
+    };
+    float best = measure(t);
+    const uint32_t tiles[3] = {8, 16, 32}, pairs[3] = {0, 1024, 4096};
+    for (int param = 0; param < 4; ++param) {
+        if (inverse && (param == 1 || param == 2)) continue;  // canonical inverse Y / Z tiles
+        for (int k = 0; k < 3; ++k) {
+            Tuning cand = t;
+            if (param < 3) cand.tile[param] = tiles[k];
+            else cand.pair_max = pairs[k];
+            if ((param < 3 && cand.tile[param] == t.tile[param]) || (param == 3 && cand.pair_max == t.pair_max))
+                continue;
+            const float ms = measure(cand);
+            if (ms < best * 0.985f) {
+                best = ms;
+                t = cand;
+            }
+        }
+    }
+    set_tuning(full, sizeof(T), inverse, t);
+}
+
 template <class T>
 void autotune(Dims full, hipStream_t s) {
     const unsigned lxy = dwt_levels(full.nx < full.ny ? full.nx : full.ny), lz = dwt_levels(full.nz);
-    if (lxy != lz || lxy == 0) return;  // the plane schedule runs axis passes only
-    if (!fused_plane_fits(full.nx, full.ny, sizeof(T))) return;  // so do planes of 4 GiB and more
-    const Tuning base = heuristic_tuning(full, lxy);
+    if (lxy == 0 && lz == 0) return;
+    const bool dyadic = lxy == lz;
     const uint64_t n = uint64_t(full.nx) * full.ny * full.nz;
+    if (!dyadic || !fused_plane_fits(full.nx, full.ny, sizeof(T))) {  // per-axis passes only
+        DeviceBuffer a(n * sizeof(T)), b(n * sizeof(T)), c(n * sizeof(T)), d(n * sizeof(T));
+        hipLaunchKernelGGL((tune_fill<T>), dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, static_cast<T*>(a.p), n);
+        Timer timer;
+        T *x = static_cast<T*>(a.p), *co = static_cast<T*>(b.p), *t0 = static_cast<T*>(c.p), *t1 = static_cast<T*>(d.p);
+        tune_passes<T>(full, lxy, lz, dyadic, false, x, co, t0, t1, timer, s);
+        transform_forward<T>(x, co, t0, t1, full, lxy, lz, dyadic, nullptr, s);
+        tune_passes<T>(full, lxy, lz, dyadic, true, x, co, t0, t1, timer, s);
+        hip_check(hipGetLastError(), "autotune launch");
+        hip_check(hipStreamSynchronize(s), "autotune");
+        return;
+    }
+    const Tuning base = heuristic_tuning(full, lxy);
     DeviceBuffer in(n * sizeof(T)), coeff(n * sizeof(T)), low(n * sizeof(T));
     hipLaunchKernelGGL((tune_fill<T>), dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, static_cast<T*>(in.p), n);
     hipLaunchKernelGGL((tune_fill<T>), dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, static_cast<T*>(coeff.p), n);
@@ -162,10 +209,16 @@ void autotune(Dims full, hipStream_t s) {
     }
     fw.fused = fuse_f;
     iv.fused = fuse_i;
-    hip_check(hipGetLastError(), "autotune launch");
-    hip_check(hipStreamSynchronize(s), "autotune");
     set_tuning(full, sizeof(T), false, fw);
     set_tuning(full, sizeof(T), true, iv);
+    // levels left to the per-axis passes: their tiles and LDS pair limit
+    if (fuse_f < lxy) tune_passes<T>(full, lxy, lz, true, false, x, c, t0, t1, timer, s);
+    if (fuse_i < lxy) {
+        transform_forward<T>(x, c, t0, t1, full, lxy, lz, true, nullptr, s);
+        tune_passes<T>(full, lxy, lz, true, true, x, c, t0, t1, timer, s);
+    }
+    hip_check(hipGetLastError(), "autotune launch");
+    hip_check(hipStreamSynchronize(s), "autotune");
 }
 
 template void autotune<float>(Dims, hipStream_t);

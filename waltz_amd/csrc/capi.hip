@@ -373,10 +373,24 @@ int waltz_amd_set_tuning(int width, uint32_t nx, uint32_t ny, uint32_t nz, int i
     return guarded([&] {
         if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
         if (!z_segments) throw std::invalid_argument("null z segments");
-        wz::Tuning t{};
+        const wz::Dims d = dims_of(nx, ny, nz);
+        wz::Tuning t = wz::tuning_for(d, wz::dwt_levels(nx < ny ? nx : ny), unsigned(width), inverse != 0);
         t.fused = fused_levels;
         for (int l = 0; l < 5; ++l) t.sz[l] = z_segments[l];
-        wz::set_tuning(dims_of(nx, ny, nz), unsigned(width), inverse != 0, t);
+        wz::set_tuning(d, unsigned(width), inverse != 0, t);
+    });
+}
+
+int waltz_amd_set_pass_tuning(int width, uint32_t nx, uint32_t ny, uint32_t nz, int inverse, const uint32_t* tiles,
+                              uint32_t lds_pair_max) {
+    return guarded([&] {
+        if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
+        if (!tiles) throw std::invalid_argument("null tiles");
+        const wz::Dims d = dims_of(nx, ny, nz);
+        wz::Tuning t = wz::tuning_for(d, wz::dwt_levels(nx < ny ? nx : ny), unsigned(width), inverse != 0);
+        for (int a = 0; a < 3; ++a) t.tile[a] = tiles[a];
+        t.pair_max = lds_pair_max;
+        wz::set_tuning(d, unsigned(width), inverse != 0, t);
     });
 }
 

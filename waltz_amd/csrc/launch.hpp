@@ -57,6 +57,13 @@ void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width
 struct Tuning {
     unsigned fused;
     uint32_t sz[5];
+    // per-axis passes (the plane schedule, and the dyadic levels not fused):
+    // line tile of the X, Y, Z passes (8, 16 or 32; forward passes take any,
+    // the inverse only its X tile: its Y and Z tiles are part of the format),
+    // and the largest plane (samples) whose X and Y passes run as one
+    // LDS-staged launch (0, 1024 or 4096)
+    uint32_t tile[3];
+    uint32_t pair_max;
 };
 Tuning tuning_for(Dims full, unsigned lxy, unsigned width, bool inverse);
 Tuning heuristic_tuning(Dims full, unsigned lxy);

@@ -66,11 +66,10 @@ __global__ void __launch_bounds__(256) axis_pass(const T* __restrict__ src, T* _
 // one launch: one workgroup per z plane stages the plane in LDS, runs the
 // first pass LDS to LDS and the second LDS to dst. The same tile lifting as
 // axis_pass, so the same bits.
-constexpr uint32_t kPlaneMax = 32 * 32;  // samples of a staged plane
-template <class T, bool Inverse, int A0, int A1>
+template <class T, bool Inverse, int A0, int A1, uint32_t kPlaneMax>
 __global__ void __launch_bounds__(256) plane_pair_pass(const T* __restrict__ src, T* __restrict__ dst, Dims full,
                                                        Dims act) {
-    __shared__ T P0[kPlaneMax], P1[kPlaneMax];
+    __shared__ T P0[kPlaneMax], P1[kPlaneMax];  // kPlaneMax: samples of a staged plane
     const uint32_t z = blockIdx.x, t = threadIdx.x;
     const uint64_t plane = uint64_t(full.nx) * full.ny, zb = uint64_t(z) * plane;
     const uint32_t cells = act.nx * act.ny;
@@ -122,25 +121,50 @@ void launch_axis(const T* src, T* dst, Dims full, Dims act, uint32_t* status, hi
                        src, dst, full, act, ntiles, uint32_t(total), status);
 }
 
+// The line tile is free for forward passes and the inverse X pass (it only
+// decides how a line is cut among threads; every tile recomputes its halo);
+// the inverse Y and Z tiles are the canonical ones.
+template <class T, bool Inverse, int Axis>
+void launch_axis_tile(const T* src, T* dst, Dims full, Dims act, uint32_t tile, uint32_t* status, hipStream_t s) {
+    if (tile == 32) launch_axis<T, Inverse, Axis, 32>(src, dst, full, act, status, s);
+    else if (tile == 16) launch_axis<T, Inverse, Axis, 16>(src, dst, full, act, status, s);
+    else launch_axis<T, Inverse, Axis, 8>(src, dst, full, act, status, s);
+}
+
 template <class T, bool Inverse>
-void axis(const T* src, T* dst, Dims full, Dims act, int axis, uint32_t* status, hipStream_t s) {
+void axis(const T* src, T* dst, Dims full, Dims act, int axis, const uint32_t* tiles, uint32_t* status,
+          hipStream_t s) {
     constexpr int ZTileInv = sizeof(T) == 4 ? 16 : 8;
-    if (axis == 0) launch_axis<T, Inverse, 0, 8>(src, dst, full, act, status, s);
-    else if (axis == 1) launch_axis<T, Inverse, 1, 8>(src, dst, full, act, status, s);
+    if (axis == 0) launch_axis_tile<T, Inverse, 0>(src, dst, full, act, tiles[0], status, s);
+    else if (Inverse && axis == 1) launch_axis<T, Inverse, 1, 8>(src, dst, full, act, status, s);
+    else if (axis == 1) launch_axis_tile<T, Inverse, 1>(src, dst, full, act, tiles[1], status, s);
     else if (Inverse) launch_axis<T, Inverse, 2, ZTileInv>(src, dst, full, act, status, s);
-    else launch_axis<T, Inverse, 2, 8>(src, dst, full, act, status, s);
+    else launch_axis_tile<T, Inverse, 2>(src, dst, full, act, tiles[2], status, s);
 }
 
 // X and Y passes of a box whose planes fit the LDS stage, in one launch.
+// limit (tuned): planes up to 1024 or 4096 samples, 0 = never; by default
+// 32 x 32 (larger planes kept a workgroup busy for longer than two launches of
+// the separate passes took at 64 x 64 on this device).
 template <class T, bool Inverse>
-bool plane_pair(const T* src, T* dst, Dims full, Dims act, int a0, int a1, hipStream_t s) {
-    // planes of up to 32 x 32: larger planes keep a workgroup busy for longer
-    // than two launches of the separate passes take (measured at 64 x 64)
-    if (uint64_t(act.nx) * act.ny > 1024 || act.nz == 0 || a0 + a1 != 1) return false;
-    if (a0 == 0)
-        hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 0, 1>), dim3(act.nz), dim3(256), 0, s, src, dst, full, act);
-    else
-        hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 1, 0>), dim3(act.nz), dim3(256), 0, s, src, dst, full, act);
+bool plane_pair(const T* src, T* dst, Dims full, Dims act, int a0, int a1, uint32_t limit, hipStream_t s) {
+    const uint64_t cells = uint64_t(act.nx) * act.ny;
+    if (cells > limit || act.nz == 0 || a0 + a1 != 1) return false;
+    if (cells <= 1024) {
+        if (a0 == 0)
+            hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 0, 1, 1024>), dim3(act.nz), dim3(256), 0, s, src, dst,
+                               full, act);
+        else
+            hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 1, 0, 1024>), dim3(act.nz), dim3(256), 0, s, src, dst,
+                               full, act);
+    } else {
+        if (a0 == 0)
+            hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 0, 1, 4096>), dim3(act.nz), dim3(256), 0, s, src, dst,
+                               full, act);
+        else
+            hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 1, 0, 4096>), dim3(act.nz), dim3(256), 0, s, src, dst,
+                               full, act);
+    }
     return true;
 }
 
@@ -164,6 +188,7 @@ struct Passes {
     T* last_dst;
     uint32_t* status;
     hipStream_t s;
+    Tuning tune;  // tiles and LDS pair limit of the per-axis passes
     bool first = true;
 
     // Passes: src -> tmp0 -> tmp1 -> ... -> final. A single pass whose source
@@ -182,10 +207,10 @@ struct Passes {
             T* const other = src == tmp[0] ? tmp[1] : tmp[0];  // never the source
             T* dst = last == count - 1 ? final_dst : other;
             if (dst == src) dst = other;  // in place at the end: through scratch, copied back below
-            if (pair && plane_pair<T, Inverse>(src, dst, full, act, axes[k], axes[k + 1], s)) {
+            if (pair && plane_pair<T, Inverse>(src, dst, full, act, axes[k], axes[k + 1], tune.pair_max, s)) {
                 ++k;
             } else {
-                axis<T, Inverse>(src, dst, full, act, axes[k], screen ? status : nullptr, s);
+                axis<T, Inverse>(src, dst, full, act, axes[k], tune.tile, screen ? status : nullptr, s);
             }
             src = dst;
         }
@@ -237,6 +262,8 @@ std::map<TuneKey, Tuning> g_tune;
 Tuning heuristic_impl(Dims full, unsigned lxy) {
     Tuning t{};
     t.fused = fused_levels(full, lxy);
+    t.tile[0] = t.tile[1] = t.tile[2] = 8;
+    t.pair_max = 1024;
     Dims n = full;
     for (unsigned l = 0; l < 5; ++l) {
         t.sz[l] = z_segment(n);
@@ -275,6 +302,11 @@ void set_tuning(Dims full, unsigned width, bool inverse, const Tuning& t) {
     for (unsigned l = 0; l < 5; ++l)
         if (t.sz[l] < 16 || t.sz[l] % 16) throw std::invalid_argument("z segment must be a positive multiple of 16");
     if (t.fused > 5) throw std::invalid_argument("at most five fused levels");
+    for (unsigned a = 0; a < 3; ++a)
+        if (t.tile[a] != 8 && t.tile[a] != 16 && t.tile[a] != 32)
+            throw std::invalid_argument("pass tile must be 8, 16 or 32");
+    if (t.pair_max != 0 && t.pair_max != 1024 && t.pair_max != 4096)
+        throw std::invalid_argument("LDS pair limit must be 0, 1024 or 4096");
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune[TuneKey{full.nx, full.ny, full.nz, width, inverse ? 1u : 0u}] = t;
 }
@@ -288,7 +320,8 @@ std::string tuning_report() {
                "],\"width\":" + std::to_string(k.width) + ",\"inverse\":" + (k.inverse ? "true" : "false") +
                ",\"fused_levels\":" + std::to_string(t.fused) + ",\"z_segment\":[";
         for (unsigned l = 0; l < 5; ++l) out += (l ? "," : "") + std::to_string(t.sz[l]);
-        out += "]}";
+        out += "],\"pass_tiles_xyz\":[" + std::to_string(t.tile[0]) + "," + std::to_string(t.tile[1]) + "," +
+               std::to_string(t.tile[2]) + "],\"lds_pair_max\":" + std::to_string(t.pair_max) + "}";
     }
     return out + "]";
 }
@@ -406,7 +439,7 @@ void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width
 template <class T>
 void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz,
                        bool dyadic, uint32_t* status, hipStream_t s) {
-    Passes<T, false> p{coeff, {tmp0, tmp1}, input, nullptr, status, s};
+    Passes<T, false> p{coeff, {tmp0, tmp1}, input, nullptr, status, s, tuning_for(full, lxy, sizeof(T), false)};
     if (lxy == 0 && lz == 0) {
         if (input != coeff) copy_box<T>(input, coeff, full, full, s);
         return;
@@ -448,7 +481,8 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
 template <class T>
 bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
                        hipStream_t s, const Level0Emit<T>* emit) {
-    Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s};
+    Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s,
+                      tuning_for(full, lxy, sizeof(T), true)};
     if (lxy == 0 && lz == 0) {
         if (output != coeff) copy_box<T>(coeff, output, full, full, s);
         return false;

@@ -105,6 +105,14 @@ def set_tuning(shape, dtype, inverse, fused_levels, z_segments):
     check(lib().waltz_amd_set_tuning(_W[dtype], nx, ny, nz, int(inverse), int(fused_levels), seg), "set_tuning")
 
 
+def set_pass_tuning(shape, dtype, inverse, tiles, lds_pair_max):
+    """Fix the per-axis passes' X/Y/Z line tiles and LDS pair limit for one direction."""
+    nz, ny, nx = shape
+    t = (C.c_uint32 * 3)(*[int(v) for v in tiles])
+    check(lib().waltz_amd_set_pass_tuning(_W[dtype], nx, ny, nz, int(inverse), t, int(lds_pair_max)),
+          "set_pass_tuning")
+
+
 def clear_tuning():
     lib().waltz_amd_clear_tuning()
 
