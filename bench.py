@@ -31,6 +31,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-only", action="store_true", help="few calls, for rocprofv3 runs")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and autograd legs")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched leg, for rocprofv3 runs")
     return ap.parse_args()
 
 
@@ -74,7 +75,7 @@ def batched_shapes():
     return [kinds[k] for k in rng.integers(0, len(kinds), 256)]
 
 
-def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps):
+def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps, single_field_compress_gbps):
     """256 float32 chunks of uneven shape (seeded mix of 32^3, 64x64x32,
     96x64x48, 128x128x64; each its own k^-5/3 field), ABS 1e-3: decoded one
     archive per call through the plain API, then in one batched pass (eager
@@ -124,6 +125,28 @@ def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps):
     identical = identical and bool(torch.equal(out_plain, out_batch))
     nbytes = dec.numel * 4
     g_batch, g_graph, g_plain = nbytes / t_batch / 1e9, nbytes / t_graph / 1e9, nbytes / t_plain / 1e9
+
+    # compress: one call per chunk, one batched call, the batched call replayed as a graph
+    flat = dec.flatten(fields)
+
+    def plain_c():
+        for f in fields:
+            waltz_amd.compress(f, abs_error=1e-3)
+
+    barch, bspans = dec.compress(flat, abs_error=1e-3)
+    same = all(bool(torch.equal(barch[int(o):int(o) + int(n)], a))
+               for (o, n), a in zip(bspans.view(-1, 2).tolist(), archives))
+    cgraph = torch.cuda.CUDAGraph()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        with torch.cuda.graph(cgraph, stream=stream):
+            dec.compress(flat, abs_error=1e-3, archives=barch, spans=bspans, check=False)
+    torch.cuda.current_stream().wait_stream(stream)
+    tc_plain = timed(plain_c, 2)
+    tc_batch = timed(lambda: dec.compress(flat, abs_error=1e-3, archives=barch, spans=bspans), 5)
+    tc_graph = timed(cgraph.replay, 5)
+    dec.compress_check()
+    c_batch, c_graph, c_plain = nbytes / tc_batch / 1e9, nbytes / tc_graph / 1e9, nbytes / tc_plain / 1e9
     return {"chunks": len(shapes), "samples": dec.numel, "archive_bytes": int(packed.numel()),
             "bound": "ABS 1e-3",
             "per_chunk_calls_GBps": round(g_plain, 2), "batched_GBps": round(g_batch, 2),
@@ -131,7 +154,12 @@ def batched_leg(torch, waltz_amd, synthetic, single_field_decompress_gbps):
             "graph_over_per_chunk": round(g_graph / g_plain, 2),
             "single_field_decompress_GBps_same_bound": round(single_field_decompress_gbps, 2),
             "graph_over_single_field_decompress": round(g_graph / single_field_decompress_gbps, 3),
-            "bit_identical_to_per_chunk_decode": identical}, [f.cpu().numpy() for f in fields]
+            "bit_identical_to_per_chunk_decode": identical,
+            "compress": {"per_chunk_calls_GBps": round(c_plain, 2), "batched_GBps": round(c_batch, 2),
+                         "graph_replay_GBps": round(c_graph, 2),
+                         "single_field_compress_GBps_same_bound": round(single_field_compress_gbps, 2),
+                         "graph_over_single_field_compress": round(c_graph / single_field_compress_gbps, 3),
+                         "archives_identical_to_per_chunk_compress": same}}, [f.cpu().numpy() for f in fields]
 
 
 def autograd_leg(torch, waltz_amd, synthetic):
@@ -335,6 +363,10 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.batched_only:
+        res, _ = batched_leg(torch, waltz_amd, synthetic, 1.0, 1.0)
+        print(json.dumps({"batched": res}), flush=True)
+        return
     S = args.size
     shape = (S, S, S)
     n = S ** 3
@@ -464,15 +496,18 @@ def main():
         decompress(archive)
         tcd.append(time.perf_counter() - b)
 
-    # single-field decode at the batched workload's bound (ABS 1e-3)
-    arch3 = compress(1e-3)
-    t3 = []
+    # single-field compress / decode at the batched workload's bound (ABS 1e-3)
+    t3c, t3 = [], []
     for _ in range(3):
         torch.cuda.synchronize()
         a = time.perf_counter()
+        arch3 = compress(1e-3)
+        b = time.perf_counter()
         decompress(arch3)
-        t3.append(time.perf_counter() - a)
+        t3.append(time.perf_counter() - b)
+        t3c.append(b - a)
     d_gbps_abs1e3 = field_bytes / statistics.median(t3) / 1e9
+    c_gbps_abs1e3 = field_bytes / statistics.median(t3c) / 1e9
     del arch3
 
     # pre-quantization coefficients in the input's precision vs an FP64 transform
@@ -519,7 +554,7 @@ def main():
     batch_host = None
     if not args.no_extra:
         log("batched leg ...")
-        result["batched"], batch_host = batched_leg(torch, waltz_amd, synthetic, d_gbps_abs1e3)
+        result["batched"], batch_host = batched_leg(torch, waltz_amd, synthetic, d_gbps_abs1e3, c_gbps_abs1e3)
         log("batched:", json.dumps(result["batched"]))
         log("autograd leg ...")
         result["autograd"] = autograd_leg(torch, waltz_amd, synthetic)
