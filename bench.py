@@ -39,16 +39,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, requant):
+def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, writeback, folded):
     """Algorithmic HBM bytes of one pipeline stage (minimum dataflow).
 
-    requant: the finest inverse level requantizes its high bands on load, so
-    the quantizing encoder writes dequantized coefficients back only inside
-    the coarser levels' box (n/8 samples)."""
+    writeback: what the quantizing encoder writes back of the dequantized
+    coefficients: "all", "box" (the coarser levels' box, n/8 samples, when
+    the finest inverse level requantizes its high bands on load) or "none"
+    (the coarse levels requantize on load as well).
+    folded: the decoder's finest inverse level applies the repair list."""
     if name == "inverse_L0_repairs":
         # coefficients (7 bands + the coarser box) and the original field read,
         # flag bitmap and repair records written
         return 2 * w * n + n / 8 + np_repairs * (4 + w)
+    if name == "inverse_L0" and folded:
+        return 2 * w * n + np_repairs * (4 + w)  # coefficients read, field written, repair list read
+    if name == "patch_keys":
+        return np_repairs * (4 + w)  # the repair list read
+    if name == "coarse_requant":
+        return 2 * w * sizes[2]  # the level-2 box read and written compact
     if name.startswith(("forward_L", "inverse_L")):
         lvl = int(name.split("_L")[1][0])
         nb = sizes[lvl]
@@ -58,7 +66,7 @@ def stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, requant):
         "wzp_encode": 2 * n + n / 8 + wzp_bytes,    # read mags+signs, write stream
         # read coeff, write the dequantized coefficients (all, or the coarser
         # levels' box) and the chunk payloads into their staging slots
-        "quantize_encode": w * n + (w * n / 8 if requant else w * n) + wzp_bytes,
+        "quantize_encode": w * n + {"all": w * n, "box": w * n / 8, "none": 0}[writeback] + wzp_bytes,
         "wzp_pack": 2 * wzp_bytes,                   # payloads from the staging slots into the archive
         "repairs": 2 * w * n,                        # read reconstruction + original
         "wzp_decode": wzp_bytes + 2 * n + n / 8,    # read stream, write mags+signs
@@ -256,7 +264,7 @@ KERNELS_FILE = ROOT / "profiles" / "codec_512cube_f32_rel1e-3_kernels.md"
 # the kernel each timed stage launches (as rocprofv3 names it)
 STAGE_KERNEL = {
     "inverse_L0_repairs": "wz::inv_level_kernel<float, 2, true, true>",
-    "inverse_L0": "wz::inv_level_kernel<float, 1, false, true>",
+    "inverse_L0": "wz::inv_level_kernel<float, 1, false, true, true>",
     "forward_L0": "wz::fwd_level_kernel<float>",
     "decode_dequantize": "wz::wzp_decode_kernel<wz::BrickSink<float> >",
     "quantize_encode": "wz::wzp_encode_kernel<wz::BrickSource<float> >",
@@ -451,8 +459,10 @@ def main():
     np_repairs = int.from_bytes(header[24:28], "little")
     rows = []
     requant = "inverse_L0_repairs" in stages and "quantize_encode" in stages
+    writeback = "none" if "coarse_requant" in stages else "box" if requant else "all"
+    folded = "patch_keys" in stages
     for name, (ms, cnt) in stages.items():
-        ab = stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, requant)
+        ab = stage_alg_bytes(name, n, w, wzp_bytes, sizes, np_repairs, writeback, folded)
         rows.append((ms / max(cnt, 1), name, ab))
     rows.sort(reverse=True)
     dom_ms, dom_name, dom_bytes = next(r for r in rows if r[2])

@@ -91,6 +91,26 @@ def test_autotuned_choice_is_bit_neutral(wz, oracle):
         stages.clear_tuning()
 
 
+# brick-pair shapes with both leading inverse levels fused: compress runs the
+# coarse inverse levels beside the encoder on requantized loads
+SPLIT = [((64, 64, 64), 2, (16, 16, 16, 16, 16)), ((48, 80, 96), 3, (16, 16, 16, 16, 16)),
+         ((96, 48, 160), 4, (32, 16, 16, 16, 16))]
+
+
+@pytest.mark.parametrize("case", SPLIT, ids=lambda c: "x".join(map(str, c[0])) + f"_f{c[1]}")
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_split_compress_inverse(wz, oracle, dtype, case):
+    waltz_amd, stages = wz
+    shape, fused, sz = case
+    stages.set_tuning(shape, TDT[dtype], True, fused, sz)
+    try:
+        v = field(shape, "smooth", dtype) + field(shape, "outliers", dtype, seed=3) / 64
+        check_codec(waltz_amd, oracle, v, 1e-3)
+        check_codec(waltz_amd, oracle, v, 1e-5)
+    finally:
+        stages.clear_tuning()
+
+
 # per-axis pass choices: forward X/Y/Z line tiles, inverse X tile, LDS pair limit
 PASS_TUNINGS = [((8, 8, 8), 0), ((16, 32, 16), 1024), ((32, 16, 32), 4096), ((32, 32, 8), 0), ((16, 8, 32), 4096)]
 

@@ -9,7 +9,13 @@ single-bit and whole-byte flips, and on a two-group WZP stream mutated group
 offsets (off by one, swapped, past the payload, nonzero for an empty group),
 mutated chunk metas, and a valid reordering of the group payloads (accepted
 by both: the directory only has to tile the payload).
+
+Each case runs on both decode routes: the host reads the header first, or
+(brick-pair fields whose finest inverse level is fused, forced here on small
+fields) the header is checked on the device while the decode is already
+launched, and the host checks the header it reads back at the end.
 """
+import contextlib
 import struct
 
 import numpy as np
@@ -28,6 +34,25 @@ def wz():
     import waltz_amd
     waltz_amd.lib()
     return waltz_amd
+
+
+@contextlib.contextmanager
+def route(planned, shape, dtype):
+    """planned: force the fused finest inverse level for this shape, which
+    makes a brick-pair field take the device-planned decode."""
+    from waltz_amd import stages
+    if planned:
+        stages.set_tuning(shape, torch.float32 if dtype == np.float32 else torch.float64, True, 1,
+                          (16, 16, 16, 16, 16))
+    try:
+        yield
+    finally:
+        if planned:
+            stages.clear_tuning()
+
+
+ROUTES = [False, True]
+ROUTE_IDS = ["host-header", "planned"]
 
 
 def _gpu_decode(wz, archive, shape, dtype):
@@ -60,25 +85,29 @@ def _agree(wz, oracle, archive, shape, dtype, what):
     return want is not None
 
 
-def test_every_truncation(wz, oracle):
-    shape = (12, 14, 16)
+@pytest.mark.parametrize("planned", ROUTES, ids=ROUTE_IDS)
+def test_every_truncation(wz, oracle, planned):
+    shape = (16, 16, 32) if planned else (12, 14, 16)
     a = oracle.compress(field(shape, "smooth", np.float32), 1e-3)
-    assert _agree(wz, oracle, a, shape, np.float32, "whole")
-    for k in range(len(a)):
-        assert not _agree(wz, oracle, a[:k], shape, np.float32, f"truncated to {k}")
-    assert not _agree(wz, oracle, a + b"\0" * 8, shape, np.float32, "trailing bytes")
+    with route(planned, shape, np.float32):
+        assert _agree(wz, oracle, a, shape, np.float32, "whole")
+        for k in range(len(a)):
+            assert not _agree(wz, oracle, a[:k], shape, np.float32, f"truncated to {k}")
+        assert not _agree(wz, oracle, a + b"\0" * 8, shape, np.float32, "trailing bytes")
 
 
+@pytest.mark.parametrize("planned", ROUTES, ids=ROUTE_IDS)
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
-def test_header_byte_flips(wz, oracle, dtype):
-    shape = (16, 16, 16)
+def test_header_byte_flips(wz, oracle, dtype, planned):
+    shape = (16, 16, 32) if planned else (16, 16, 16)
     a = bytearray(oracle.compress(field(shape, "outliers", dtype), 1e-2))
     accepted = 0
-    for pos in range(32):
-        for mask in (0x01, 0x02, 0x10, 0x80, 0xFF):
-            m = bytearray(a)
-            m[pos] ^= mask
-            accepted += _agree(wz, oracle, m, shape, dtype, f"header byte {pos} ^ {mask:#x}")
+    with route(planned, shape, dtype):
+        for pos in range(32):
+            for mask in (0x01, 0x02, 0x10, 0x80, 0xFF):
+                m = bytearray(a)
+                m[pos] ^= mask
+                accepted += _agree(wz, oracle, m, shape, dtype, f"header byte {pos} ^ {mask:#x}")
     assert accepted  # some flips (quantum low bits, the z band) still decode
 
 
@@ -97,8 +126,14 @@ def _layout(a, n):
     return chunks, groups, wzp, header
 
 
-def test_directory_mutations(wz, oracle, two_groups):
+@pytest.mark.parametrize("planned", ROUTES, ids=ROUTE_IDS)
+def test_directory_mutations(wz, oracle, two_groups, planned):
     shape, a = two_groups
+    with route(planned, shape, np.float32):
+        _directory_mutations(wz, oracle, shape, a)
+
+
+def _directory_mutations(wz, oracle, shape, a):
     n = int(np.prod(shape))
     chunks, groups, wzp, header = _layout(a, n)
     assert groups == 2
@@ -124,8 +159,14 @@ def test_directory_mutations(wz, oracle, two_groups):
             _agree(wz, oracle, m, shape, np.float32, f"meta {c} {delta:+#x}")
 
 
-def test_reordered_groups_accepted(wz, oracle, two_groups):
+@pytest.mark.parametrize("planned", ROUTES, ids=ROUTE_IDS)
+def test_reordered_groups_accepted(wz, oracle, two_groups, planned):
     shape, a = two_groups
+    with route(planned, shape, np.float32):
+        _reordered_groups(wz, oracle, shape, a)
+
+
+def _reordered_groups(wz, oracle, shape, a):
     n = int(np.prod(shape))
     chunks, groups, wzp, header = _layout(a, n)
     base = 32 + header
@@ -143,9 +184,15 @@ def test_reordered_groups_accepted(wz, oracle, two_groups):
     assert not _agree(wz, oracle, m, shape, np.float32, "overlapping groups")
 
 
-def test_empty_group_offset(wz, oracle):
+@pytest.mark.parametrize("planned", ROUTES, ids=ROUTE_IDS)
+def test_empty_group_offset(wz, oracle, planned):
     shape = (48, 64, 64)  # two groups, every chunk empty: payload 0, both offsets 0
     a = bytearray(oracle.compress(np.zeros(shape, np.float32), 1e-2))
+    with route(planned, shape, np.float32):
+        _empty_group(wz, oracle, shape, a)
+
+
+def _empty_group(wz, oracle, shape, a):
     n = int(np.prod(shape))
     chunks, groups, wzp, header = _layout(a, n)
     assert wzp == header
@@ -154,3 +201,21 @@ def test_empty_group_offset(wz, oracle):
         m = bytearray(a)
         struct.pack_into("<I", m, 32 + 4 * g, 8)
         assert not _agree(wz, oracle, m, shape, np.float32, f"empty group {g} at offset 8")
+
+
+@pytest.mark.parametrize("planned", ROUTES, ids=ROUTE_IDS)
+def test_route_taken(wz, oracle, planned):
+    """The forced launch choice does send the small brick-pair field down the
+    planned decode (its header check is a stage of its own)."""
+    from waltz_amd import _lib
+    shape = (16, 16, 32)
+    a = oracle.compress(field(shape, "smooth", np.float32), 1e-3)
+    with route(planned, shape, np.float32):
+        _lib.profile(True)
+        try:
+            _lib.profile_report()
+            assert _agree(wz, oracle, a, shape, np.float32, "whole")
+            stages = _lib.profile_report()
+        finally:
+            _lib.profile(False)
+    assert ("decode_plan" in stages) == planned, sorted(stages)

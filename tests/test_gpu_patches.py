@@ -1,7 +1,10 @@
 """Side lists on decode: dense native repairs (whole 64-sample row blocks),
 side lists out of order (valid: decoded again with duplicate checks) and
 invalid entries. Archives are rewritten with other side lists and decoded
-against the oracle, which restates cpu/src/codec.cpp:260-330."""
+against the oracle, which restates cpu/src/codec.cpp:260-330. Each case runs
+with the launch choice these small fields get by default (per-axis passes,
+repairs patched by a separate pass) and with the finest inverse level forced
+to its fused kernel (repairs applied as it stores the samples)."""
 import numpy as np
 import pytest
 
@@ -60,6 +63,22 @@ def rebuild(s, pidx, perr, qidx=None, qval=None):
     return bytes(out)
 
 
+@pytest.fixture(params=["passes", "fused"])
+def finest(request, wz):
+    """How the finest inverse level runs for the shapes of the test."""
+    from waltz_amd import stages
+    forced = []
+
+    def use(shape, dtype):
+        if request.param == "fused":
+            stages.set_tuning(shape, TDT[dtype], True, 1, (16, 16, 16, 16, 16))
+            forced.append(shape)
+
+    yield use
+    if forced:
+        stages.clear_tuning()
+
+
 def decode(wz, raw, shape, dtype):
     a = torch.from_numpy(np.frombuffer(raw, np.uint8).copy()).cuda()
     return wz.decompress(a, shape=shape, dtype=TDT[dtype]).cpu().numpy()
@@ -97,7 +116,8 @@ SHAPES = [(64, 64, 64), (36, 40, 100), (48, 72, 136)]
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
-def test_dense_repairs(wz, oracle, shape, dtype):
+def test_dense_repairs(wz, oracle, finest, shape, dtype):
+    finest(shape, dtype)
     ref = base_archive(oracle, shape, dtype)
     s = sections(ref, np.dtype(dtype).itemsize)
     n = int(np.prod(shape))
@@ -108,9 +128,10 @@ def test_dense_repairs(wz, oracle, shape, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
-def test_unordered_lists_decode(wz, oracle, dtype):
+def test_unordered_lists_decode(wz, oracle, finest, dtype):
     """Out-of-order side lists without duplicates are valid archives."""
     shape = (64, 64, 64)
+    finest(shape, dtype)
     ref = base_archive(oracle, shape, dtype, spikes=True)
     s = sections(ref, np.dtype(dtype).itemsize)
     assert len(s["qidx"]) >= 2 and len(s["pidx"]) >= 2
@@ -144,10 +165,11 @@ def test_duplicate_index_rejected(wz, oracle, which):
 
 
 @pytest.mark.parametrize("where", ["early", "late", "index"])
-def test_bad_repair_rejected(wz, oracle, where):
+def test_bad_repair_rejected(wz, oracle, finest, where):
     """A non-finite correction (early or late in a dense row block) or an
     index beyond the field fails the decode."""
     shape = (64, 64, 64)
+    finest(shape, np.float32)
     ref = base_archive(oracle, shape, np.float32)
     s = sections(ref, 4)
     idx, err = dense_list(s, int(np.prod(shape)), shape[2], np.random.default_rng(3), np.float32)

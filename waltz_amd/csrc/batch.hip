@@ -56,6 +56,7 @@ struct JobShape {
     uint32_t lx, lz, bz, dyadic;
     uint32_t n, chunks, groups, nbx, nby, nbz;
     uint32_t rank;      // the job's Morton rank table set (one per distinct geometry)
+    uint32_t pairs;     // full 16^3 brick pairs: (de)quantization runs inside the WZP coder
     uint64_t bank;      // element offset in each work bank
     uint64_t mag;       // u16 offset of the magnitudes (in the tmp0 bank)
     uint64_t sign;      // word offset of the sign words (in the tmp1 bank)
@@ -202,17 +203,35 @@ __global__ void __launch_bounds__(1024) batch_plan_kernel(const uint8_t* __restr
     }
 }
 
+// Chunks of the jobs that are not brick pairs (mag_pfx counts only theirs):
+// magnitudes and signs, dequantized by batch_dequant_kernel.
 __global__ void __launch_bounds__(256) batch_wzp_kernel(const uint8_t* __restrict__ packed,
                                                         const JobShape* __restrict__ shape,
                                                         const JobRun* __restrict__ run,
-                                                        const uint32_t* __restrict__ chunk_pfx, uint32_t jobs,
+                                                        const uint32_t* __restrict__ mag_pfx, uint32_t jobs,
                                                         uint16_t* mag, uint32_t* sign, uint32_t* status) {
-    const uint32_t j = job_of(chunk_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of(mag_pfx, jobs, blockIdx.x);
     const JobRun& r = run[j];
     if (r.state != kJobOk) return;
     const JobShape& sh = shape[j];
-    decode_chunk(packed + r.arc + 32, r.wzp, sh.n, sh.chunks, sh.groups, blockIdx.x - chunk_pfx[j],
+    decode_chunk(packed + r.arc + 32, r.wzp, sh.n, sh.chunks, sh.groups, blockIdx.x - mag_pfx[j],
                  MagSink{mag + sh.mag, sign + sh.sign}, status);
+}
+
+// Chunks of the brick-pair jobs (pair_pfx): decoded straight into dequantized
+// coefficients, as the single-field decoder's BrickSink does.
+template <class T>
+__global__ void __launch_bounds__(256) batch_wzp_pairs_kernel(const uint8_t* __restrict__ packed,
+                                                              const JobShape* __restrict__ shape,
+                                                              const JobRun* __restrict__ run,
+                                                              const uint32_t* __restrict__ pair_pfx, uint32_t jobs,
+                                                              T* __restrict__ home, uint32_t* status) {
+    const uint32_t j = job_of(pair_pfx, jobs, blockIdx.x);
+    const JobRun& r = run[j];
+    if (r.state != kJobOk) return;
+    const JobShape& sh = shape[j];
+    decode_chunk(packed + r.arc + 32, r.wzp, sh.n, sh.chunks, sh.groups, blockIdx.x - pair_pfx[j],
+                 BrickSink<T>{home + sh.bank, sh.d.nx, sh.d.ny, T(r.quantum)}, status);
 }
 
 // Rank table sets of a plan: per distinct (geometry, z band) the packed rank
@@ -575,15 +594,35 @@ WZ_D WzpScratch job_wzp(const CompBufs& b, const JobComp& jc) {
     return WzpScratch{b.desc + jc.group0, b.staging + jc.staging, b.sizes + jc.chunk0, nullptr};
 }
 
+// Chunks of the jobs that are not brick pairs (mag_pfx counts only theirs):
+// the magnitudes and signs of batch_quantize_kernel.
 __global__ void __launch_bounds__(256) batch_encode_kernel(const JobShape* __restrict__ shape,
                                                            const JobComp* __restrict__ comp,
-                                                           const uint32_t* __restrict__ chunk_pfx, uint32_t jobs,
+                                                           const uint32_t* __restrict__ mag_pfx, uint32_t jobs,
                                                            CompBufs b) {
-    const uint32_t j = job_of(chunk_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of(mag_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     const MagSource src{static_cast<const uint16_t*>(b.tmp0) + sh.mag, static_cast<const uint32_t*>(b.tmp1) + sh.sign};
-    encode_chunk(src, blockIdx.x - chunk_pfx[j], sh.n, sh.groups, b.out + jc.arc + 32, jc.cap - 32, job_wzp(b, jc),
+    encode_chunk(src, blockIdx.x - mag_pfx[j], sh.n, sh.groups, b.out + jc.arc + 32, jc.cap - 32, job_wzp(b, jc),
+                 &b.res[j].status);
+}
+
+// Chunks of the brick-pair jobs (pair_pfx): quantized inside the encoder, as
+// the single-field BrickSource does, with the dequantized coefficients
+// written back over the whole job for the inverse passes.
+template <class T>
+__global__ void __launch_bounds__(256) batch_encode_pairs_kernel(const JobShape* __restrict__ shape,
+                                                                 const JobComp* __restrict__ comp,
+                                                                 const uint32_t* __restrict__ pair_pfx,
+                                                                 uint32_t jobs, double quantum, CompBufs b) {
+    const uint32_t j = job_of(pair_pfx, jobs, blockIdx.x);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
+    const BrickSource<T> src{static_cast<T*>(b.home) + sh.bank, sh.d.nx, sh.d.ny, T(quantum), T(1.0 / quantum), qo,
+                             sh.d.nx, sh.d.ny, sh.d.nz};
+    encode_chunk(src, blockIdx.x - pair_pfx[j], sh.n, sh.groups, b.out + jc.arc + 32, jc.cap - 32, job_wzp(b, jc),
                  &b.res[j].status);
 }
 
@@ -1126,6 +1165,8 @@ struct HostPlan {
     uint32_t jobs = 0;
     std::vector<JobShape> shape;
     std::vector<uint32_t> chunk_pfx, brick_pfx;
+    // chunks of the brick-pair jobs and of the others (WZP coder launches)
+    std::vector<uint32_t> pair_pfx, mag_pfx;
     std::vector<std::vector<Pass>> step;      // step[s][job]
     std::vector<std::vector<uint32_t>> pfx;   // block prefixes per step (per-axis kernel)
     std::vector<std::vector<uint32_t>> lpfx;  // block prefixes per step (LDS kernel)
@@ -1257,9 +1298,11 @@ void finish_plan(HostPlan& p) {
     const uint32_t J = p.jobs;
     p.chunk_pfx.assign(J + 1, 0);
     p.brick_pfx.assign(J + 1, 0);
+    p.pair_pfx.assign(J + 1, 0);
+    p.mag_pfx.assign(J + 1, 0);
     std::vector<std::vector<Pass>> sched(J);
     size_t steps = 0;
-    uint64_t bank = 0, mag = 0, sign = 0, chunks = 0, bricks = 0;
+    uint64_t bank = 0, mag = 0, sign = 0, chunks = 0, bricks = 0, pchunks = 0, mchunks = 0;
     p.rank_geo.clear();
     for (uint32_t j = 0; j < J; ++j) {
         JobShape& s = p.shape[j];
@@ -1278,14 +1321,26 @@ void finish_plan(HostPlan& p) {
         sign += (uint64_t(s.n) + 31) / 32;
         p.chunk_pfx[j] = uint32_t(chunks);
         p.brick_pfx[j] = uint32_t(bricks);
+        p.pair_pfx[j] = uint32_t(pchunks);
+        p.mag_pfx[j] = uint32_t(mchunks);
         chunks += s.chunks;
-        bricks += uint64_t(s.nbx) * s.nby * s.nbz;
+        // brick-pair jobs (de)quantize inside the WZP coder: no bricks in the
+        // separate passes
+        s.pairs = pair_bricks_fit(s.d.nx, s.d.ny, s.d.nz, s.bz) ? 1u : 0u;
+        if (s.pairs) {
+            pchunks += s.chunks;
+        } else {
+            mchunks += s.chunks;
+            bricks += uint64_t(s.nbx) * s.nby * s.nbz;
+        }
         sched[j] = inverse_schedule(s, p.width);
         steps = std::max(steps, sched[j].size());
     }
     if (chunks > 0x7FFFFFFFull || bricks > 0x7FFFFFFFull) throw std::invalid_argument("batch launch limit");
     p.chunk_pfx[J] = uint32_t(chunks);
     p.brick_pfx[J] = uint32_t(bricks);
+    p.pair_pfx[J] = uint32_t(pchunks);
+    p.mag_pfx[J] = uint32_t(mchunks);
     p.bank_elems = bank;
     p.mag_elems = mag;
     p.sign_words = sign;
@@ -1301,6 +1356,7 @@ struct DevicePlan {
     JobShape* shape = nullptr;
     JobRun* run = nullptr;
     uint32_t *chunk_pfx = nullptr, *brick_pfx = nullptr, *np_pfx = nullptr, *nq_pfx = nullptr, *status = nullptr;
+    uint32_t *pair_pfx = nullptr, *mag_pfx = nullptr;
     Pass* pass = nullptr;        // steps x jobs
     uint32_t* pass_pfx = nullptr;  // steps x (jobs + 1)
     uint32_t* host_status = nullptr;  // pinned
@@ -1332,6 +1388,7 @@ struct DevicePlan {
             return at;
         };
         const size_t o_shape = carve(sizeof(JobShape) * J), o_run = carve(sizeof(JobRun) * J);
+        const size_t o_pair = carve(4 * (J + 1)), o_magp = carve(4 * (J + 1));
         const size_t o_cp = carve(4 * (J + 1)), o_bp = carve(4 * (J + 1)), o_np = carve(4 * (J + 1)),
                      o_nq = carve(4 * (J + 1)), o_st = carve(4), o_pass = carve(sizeof(Pass) * J * S),
                      o_pp = carve(4 * (J + 1) * S);
@@ -1345,6 +1402,8 @@ struct DevicePlan {
         run = reinterpret_cast<JobRun*>(mem + o_run);
         chunk_pfx = reinterpret_cast<uint32_t*>(mem + o_cp);
         brick_pfx = reinterpret_cast<uint32_t*>(mem + o_bp);
+        pair_pfx = reinterpret_cast<uint32_t*>(mem + o_pair);
+        mag_pfx = reinterpret_cast<uint32_t*>(mem + o_magp);
         np_pfx = reinterpret_cast<uint32_t*>(mem + o_np);
         nq_pfx = reinterpret_cast<uint32_t*>(mem + o_nq);
         status = reinterpret_cast<uint32_t*>(mem + o_st);
@@ -1369,6 +1428,8 @@ struct DevicePlan {
         hip_check(hipMemcpyAsync(shape, p.shape.data(), sizeof(JobShape) * J, hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(chunk_pfx, p.chunk_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(brick_pfx, p.brick_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(pair_pfx, p.pair_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(mag_pfx, p.mag_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
         if (S) {
             hip_check(hipMemcpyAsync(pass, flat.data(), sizeof(Pass) * flat.size(), hipMemcpyHostToDevice, s), "plan");
             hip_check(hipMemcpyAsync(pass_pfx, flat_pfx.data(), 4 * flat_pfx.size(), hipMemcpyHostToDevice, s),
@@ -1464,9 +1525,11 @@ struct DevicePlan {
         if (p.brick_pfx[J])
             hipLaunchKernelGGL((batch_quantize_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, comp,
                                brick_pfx, J, ranks, quantum, b);
-        if (p.chunk_pfx[J])
-            hipLaunchKernelGGL(batch_encode_kernel, dim3(p.chunk_pfx[J]), dim3(256), 0, s, shape, comp, chunk_pfx, J,
-                               b);
+        if (p.mag_pfx[J])
+            hipLaunchKernelGGL(batch_encode_kernel, dim3(p.mag_pfx[J]), dim3(256), 0, s, shape, comp, mag_pfx, J, b);
+        if (p.pair_pfx[J])
+            hipLaunchKernelGGL((batch_encode_pairs_kernel<T>), dim3(p.pair_pfx[J]), dim3(256), 0, s, shape, comp,
+                               pair_pfx, J, quantum, b);
         if (p.group_pfx[J])
             hipLaunchKernelGGL(batch_pack_kernel, dim3(p.group_pfx[J]), dim3(kPackThreads), 0, s, shape, comp,
                                group_pfx, J, b);
@@ -1511,9 +1574,12 @@ struct DevicePlan {
         zero_words(status, 1, s);
         hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(1024), 0, s, packed, packed_bytes, spans, shape, run, J,
                            p.width, np_pfx, nq_pfx, status);
-        if (p.chunk_pfx[J])
-            hipLaunchKernelGGL(batch_wzp_kernel, dim3(p.chunk_pfx[J]), dim3(256), 0, s, packed, shape, run, chunk_pfx,
-                               J, static_cast<uint16_t*>(bk.tmp0), static_cast<uint32_t*>(bk.tmp1), status);
+        if (p.mag_pfx[J])
+            hipLaunchKernelGGL(batch_wzp_kernel, dim3(p.mag_pfx[J]), dim3(256), 0, s, packed, shape, run, mag_pfx, J,
+                               static_cast<uint16_t*>(bk.tmp0), static_cast<uint32_t*>(bk.tmp1), status);
+        if (p.pair_pfx[J])
+            hipLaunchKernelGGL((batch_wzp_pairs_kernel<T>), dim3(p.pair_pfx[J]), dim3(256), 0, s, packed, shape, run,
+                               pair_pfx, J, static_cast<T*>(bk.home), status);
         if (p.brick_pfx[J])
             hipLaunchKernelGGL((batch_dequant_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, run,
                                brick_pfx, J, ranks, static_cast<const uint16_t*>(bk.tmp0),

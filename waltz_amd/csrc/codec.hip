@@ -72,11 +72,13 @@ struct DeviceWorkspace {
     void init() {
         if (!rank) {
             hip_check(hipMalloc(reinterpret_cast<void**>(&rank), 2 * 8 * 8192 * sizeof(uint16_t)), "rank tables");
-            hip_check(hipMalloc(reinterpret_cast<void**>(&result), sizeof(EncodeResult)), "result");
-            hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_result), sizeof(EncodeResult)), "pinned result");
+            // one record for compress (EncodeResult) or decode (DecodeRecord)
+            hip_check(hipMalloc(reinterpret_cast<void**>(&result), kRecordBytes), "result");
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_result), kRecordBytes), "pinned result");
             hip_check(hipStreamCreateWithFlags(&side.stream, hipStreamNonBlocking), "side stream");
             hip_check(hipEventCreateWithFlags(&side.ready, hipEventDisableTiming), "event");
             hip_check(hipEventCreateWithFlags(&side.done, hipEventDisableTiming), "event");
+            hip_check(hipEventCreateWithFlags(&side.mid, hipEventDisableTiming), "event");
         }
     }
     const uint16_t* ranks(Dims d, uint32_t bz, hipStream_t s) {
@@ -259,10 +261,16 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     // device idle; the archive sections wait for it.
     const SideStream* side = fsegs ? &w.side : nullptr;
     const SideJoin join{side};
+    // When the coarser inverse levels requantize what they load as well (the
+    // two leading levels fused), the encoder writes nothing back. They run
+    // after the encoder, beside its group pass: beside the encoder they slowed
+    // it as much as they hid, and the group pass then collided with the
+    // finest level, which needs every CU's two workgroup slots.
+    const bool split = requant && coarse_split_fits(g.d, g.lx, sizeof(T));
     if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz)) {
         StageClock c("quantize_encode", s);
         quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,
-                                 status, s, requant, side);
+                                 status, s, split ? kWriteNone : requant ? kWriteCoarseBox : kWriteAll, side);
     } else {
         const BrickGeometry bg = brick_geometry(g.d, g.bz);
         const uint16_t* rank = w.ranks(g.d, g.bz, s);
@@ -278,7 +286,11 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     Level0Emit<T> emit{kEmitRepair, status, input, abs_bound, SegList{p.rbm, p.ridx, p.rerr, p.rcount, fsegs, fcap},
                        requant ? quantum : 0.0};
     SegList rp = emit.rep;
-    if (!transform_inverse<T>(p.coeff, p.rec, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s, fsegs ? &emit : nullptr)) {
+    if (split) {
+        inverse_coarse_requant<T>(p.coeff, p.t0, p.t1, g.d, g.lx, quantum, s);
+        inverse_finest_requant<T>(p.coeff, p.t0, g.d, g.lx, emit, s);
+    } else if (!transform_inverse<T>(p.coeff, p.rec, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s,
+                                     fsegs ? &emit : nullptr)) {
         rp = SegList{p.rbm, p.ridx, p.rerr, p.rcount, rtiles, kRepairTile};
         StageClock c("repairs", s);
         detect_repairs<T>(input, p.rec, n, abs_bound, rp, status, s);
@@ -343,12 +355,12 @@ uint64_t decode_scratch_bytes(uint32_t n, unsigned width) {
 namespace {
 
 // Where one decode's memory comes from: the device scratch (decode_scratch_bytes),
-// the rank tables for geometry d and band bz, the device result record and a
-// host copy of it (at least 48 bytes, also used for the 32-byte header).
+// the rank tables for geometry d and band bz, the device record and a host
+// copy of it.
 struct DecodeSource {
     uint8_t* scratch;
-    EncodeResult* result;
-    EncodeResult* host;
+    DecodeRecord* result;
+    DecodeRecord* host;
     DeviceWorkspace* cache;  // cached rank tables and the side stream, or nullptr: ranks rebuilt into `rank`
     uint16_t* rank;
     const uint16_t* ranks(Dims d, uint32_t bz, hipStream_t s) const {
@@ -389,18 +401,38 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
     uint16_t* mag = c.take<uint16_t>(n);
     uint32_t* sign = c.take<uint32_t>(words);
     uint32_t* status = &m.result->status;
-    zero_words(m.result, sizeof(EncodeResult) / 4, s);
+    zero_words(m.result, sizeof(DecodeRecord) / 4, s);
     if (!canonical) {  // duplicate indices fail (cpu/src/codec.cpp:260-275); `sign` is free until decode
         check_unique(archive + L.qi, nq, n, sign, status, s);
         check_unique(archive + L.pi, np, n, sign, status, s);
     }
     Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}, 0.0};
+    // a canonical repair list is applied as the finest fused level stores its
+    // samples (its key table in the magnitude scratch, free once the
+    // coefficients are dequantized); a list out of order comes back as
+    // kUnordered, and the general pass patches separately
+    const uint64_t keys = canonical ? patch_table_words(g.d, np, sizeof(T)) : 0;
+    if (keys) {
+        screen.pidx = reinterpret_cast<const uint32_t*>(archive + L.pi);
+        screen.perr = reinterpret_cast<const T*>(archive + L.pv);
+        screen.np = np;
+        screen.ptable = reinterpret_cast<uint32_t*>(mag);
+    }
     // the WZP directory check runs on the side stream beside the chunk decoders
     const SideStream* side = m.cache ? &m.cache->side : nullptr;
     const SideJoin join{side};
-    if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz)) {
+    const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz);
+    if (pairs) {
         StageClock ck("decode_dequantize", s);
         decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s, side);
+        if (keys && side) {
+            // the key table reads only the archive, and the brick-pair decoder
+            // leaves the magnitude scratch unused: built on the side stream
+            // after the directory check, beside the chunk decoders
+            repair_keys<T>(screen.pidx, screen.perr, np, g.d, screen.ptable, status, side->stream);
+            hip_check(hipEventRecord(side->mid, side->stream), "event record");
+            screen.ptable_built = true;
+        }
     } else {
         {
             StageClock ck("wzp_decode", s);
@@ -415,14 +447,93 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
         StageClock ck("qoutliers", s);
         apply_qoutliers<T>(coeff, archive + L.qi, archive + L.qv, nq, n, quantum, status, s);
     }
+    if (screen.ptable_built) hip_check(hipStreamWaitEvent(s, side->mid, 0), "stream wait");
     const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, dyadic, s, &screen);
     {
         StageClock ck("patches", s);
-        apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);  // screens patched samples
+        if (!screened || !keys)  // screens patched samples
+            apply_repairs<T>(output, archive + L.pi, archive + L.pv, np, n, status, s);
         if (!screened) check_finite<T>(output, n, status, s);
     }
     if (side) hip_check(hipStreamWaitEvent(s, side->done, 0), "stream wait");
-    hip_check(hipMemcpyAsync(m.host, m.result, sizeof(EncodeResult), hipMemcpyDeviceToHost, s), "status copy");
+    hip_check(hipMemcpyAsync(m.host, m.result, sizeof(DecodeRecord), hipMemcpyDeviceToHost, s), "status copy");
+    hip_check(hipStreamSynchronize(s), "decompress");
+    hip_check(hipGetLastError(), "decompress launch");
+    return m.host->status;
+}
+
+// A decode planned on the device: a kernel checks the header into the
+// device record and every stage reads the stream size, quantum and side
+// lists from it, so the call synchronises once, at its end, instead of
+// first waiting for the header. It assumes the canonical archive of the
+// geometry (the one compress writes); any other header leaves every planned
+// kernel idle (record not ok) and the caller decodes on the host-header path.
+// Applies to fields whose finest inverse level runs fused (it applies the
+// repairs, so every count the launch needs comes from the record).
+bool planned_decode_fits(const Geometry& g, uint64_t archive_bytes, unsigned width) {
+    return archive_bytes != ~uint64_t(0) && archive_bytes >= 32 && g.dyadic && g.lx >= 1 &&
+           tuning_for(g.d, g.lx, width, true).fused >= 1 && patch_table_words(g.d, g.n, width) != 0;
+}
+
+template <class T>
+uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Geometry& g, T* output,
+                        const DecodeSource& m, hipStream_t s) {
+    const uint32_t n = g.n;
+    Carve c{m.scratch};
+    T* coeff = c.take<T>(n);
+    T* t0 = c.take<T>(n);
+    T* t1 = c.take<T>(n);
+    uint16_t* mag = c.take<uint16_t>(n);
+    uint32_t* sign = c.take<uint32_t>((uint64_t(n) + 31) / 32);
+    uint32_t* table = reinterpret_cast<uint32_t*>(mag);  // repair key table, once the magnitudes are dead
+    DecodeRecord* rec = m.result;
+    uint32_t* status = &rec->status;
+    const uint8_t flags = uint8_t((g.dyadic ? 1u : 0u) | (sizeof(T) == 8 ? 4u : 0u) | (3u << 3));
+    {
+        StageClock ck("decode_plan", s);
+        decode_plan(archive, archive_bytes,
+                    PlanExpect{n, uint32_t(sizeof(T)), flags, uint8_t(g.lx), uint8_t(g.lz), uint8_t(g.bz)}, rec, s);
+    }
+    // the directory check runs on the side stream beside the chunk decoders;
+    // brick pairs leave the magnitude scratch unused, so the repair key table
+    // (it reads only the archive) follows it there
+    const SideStream* side = m.cache ? &m.cache->side : nullptr;
+    const SideJoin join{side};
+    const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
+    bool keys_on_side = false;
+    if (pairs) {
+        StageClock ck("decode_dequantize", s);
+        decode_dequantize_pairs_planned<T>(archive + 32, rec, g.d, coeff, status, s, side);
+        if (side) {
+            repair_keys_planned<T>(archive, rec, g.d, table, status, side->stream);
+            hip_check(hipEventRecord(side->mid, side->stream), "event record");
+            keys_on_side = true;
+        }
+    } else {
+        {
+            StageClock ck("wzp_decode", s);
+            wzp_decode_planned(archive + 32, rec, n, mag, sign, status, s, side);
+        }
+        const uint16_t* rank = m.ranks(g.d, g.bz, s);
+        StageClock ck("dequantize", s);
+        dequantize_bricks<T>(mag, sign, brick_geometry(g.d, g.bz), 0.0, rank, coeff, s, rec);
+    }
+    if (!keys_on_side) repair_keys_planned<T>(archive, rec, g.d, table, status, s);
+    {
+        StageClock ck("qoutliers", s);
+        apply_qoutliers_planned<T>(coeff, archive, rec, n, status, s);
+    }
+    Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}, 0.0};
+    screen.np = n;  // an upper bound: the count comes from the record
+    screen.ptable = table;
+    screen.ptable_built = true;
+    screen.plan = rec;
+    screen.archive = archive;
+    if (keys_on_side) hip_check(hipStreamWaitEvent(s, side->mid, 0), "stream wait");
+    if (!transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, true, s, &screen))
+        throw std::logic_error("planned decode without a fused finest level");
+    if (side) hip_check(hipStreamWaitEvent(s, side->done, 0), "stream wait");
+    hip_check(hipMemcpyAsync(m.host, rec, sizeof(DecodeRecord), hipMemcpyDeviceToHost, s), "status copy");
     hip_check(hipStreamSynchronize(s), "decompress");
     hip_check(hipGetLastError(), "decompress launch");
     return m.host->status;
@@ -435,11 +546,20 @@ template <class T>
 void decompress_with(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* output, hipStream_t s,
                      bool canonical_chunk, const DecodeSource& m) {
     const Geometry g = geometry(d, sizeof(T));
-    uint8_t* h = reinterpret_cast<uint8_t*>(m.host);  // 48 host bytes, reused for the header
-    hip_check(hipMemcpyAsync(h, archive, 32, hipMemcpyDeviceToHost, s), "header copy");
-    hip_check(hipStreamSynchronize(s), "header copy");
     uint8_t hdr[32];
-    std::memcpy(hdr, h, 32);
+    uint32_t st = 0;
+    bool decoded = false;  // by the planned decode (its header was valid and canonical)
+    if (planned_decode_fits(g, archive_bytes, sizeof(T))) {
+        check_disjoint(archive, archive_bytes, output, uint64_t(g.n) * sizeof(T));
+        st = decode_planned<T>(archive, archive_bytes, g, output, m, s);
+        std::memcpy(hdr, m.host->header, 32);
+        decoded = m.host->ok != 0;
+    } else {
+        uint8_t* h = m.host->header;
+        hip_check(hipMemcpyAsync(h, archive, 32, hipMemcpyDeviceToHost, s), "header copy");
+        hip_check(hipStreamSynchronize(s), "header copy");
+        std::memcpy(hdr, h, 32);
+    }
     const uint64_t wzp = uint64_t(hdr[16]) | (uint64_t(hdr[17]) << 8) | (uint64_t(hdr[18]) << 16) |
                          (uint64_t(hdr[19]) << 24) | (uint64_t(hdr[20]) << 32) | (uint64_t(hdr[21]) << 40) |
                          (uint64_t(hdr[22]) << 48) | (uint64_t(hdr[23]) << 56);
@@ -457,7 +577,7 @@ void decompress_with(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* 
     check_disjoint(archive, end, output, uint64_t(g.n) * sizeof(T));
     if (wzp < ws.header || wzp > ws.header + uint64_t(g.n) * 2 + (uint64_t(g.n) + 7) / 8)
         throw std::runtime_error("WZP archive size is outside its valid bounds");
-    uint32_t st = decode_pass<T>(archive, hdr, g, output, true, m, s);
+    if (!decoded) st = decode_pass<T>(archive, hdr, g, output, true, m, s);
     // a side list out of order (not written by compress, but valid when its
     // indices are distinct): decode again with the duplicate checks
     if (st & kUnordered) st = decode_pass<T>(archive, hdr, g, output, false, m, s) & ~uint32_t(kUnordered);
@@ -476,8 +596,9 @@ void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* outpu
     DeviceWorkspace& w = workspace();
     std::lock_guard<std::mutex> guard(w.lock);
     w.init();
-    const DecodeSource m{static_cast<uint8_t*>(w.reserve(decode_scratch_bytes(g.n, sizeof(T)))), w.result,
-                         w.host_result, &w, nullptr};
+    const DecodeSource m{static_cast<uint8_t*>(w.reserve(decode_scratch_bytes(g.n, sizeof(T)))),
+                         reinterpret_cast<DecodeRecord*>(w.result), reinterpret_cast<DecodeRecord*>(w.host_result), &w,
+                         nullptr};
     decompress_with<T>(archive, archive_bytes, d, output, s, canonical_chunk, m);
 }
 
@@ -489,7 +610,7 @@ void decompress_into(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* 
     const Geometry g = geometry(d, sizeof(T));
     if (decode_scratch_bytes(g.n, sizeof(T)) > mem.scratch_bytes)
         throw std::invalid_argument("field exceeds the decoder's fixed scratch");
-    EncodeResult host{};
+    DecodeRecord host{};
     const DecodeSource m{mem.scratch, mem.result, &host, nullptr, mem.rank};
     decompress_with<T>(archive, archive_bytes, d, output, s, canonical_chunk, m);
 }

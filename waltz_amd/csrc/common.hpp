@@ -194,6 +194,62 @@ WZ_HD Layout archive_layout(uint64_t wzp, uint32_t np, uint64_t repair_bytes, ui
     return L;
 }
 
+// ------------------------------------------------------------------ decode plan
+// A decode launched before its header reaches the host: what the launch
+// assumes of the archive (the geometry's canonical flags, levels and z band)
+// and the record the device fills from the header (cpu/src/codec.cpp:
+// 279-300, include/utils/Blob.hpp:38-90): the header bytes, the sections'
+// places, and `ok` when the header is valid, canonical for this launch and
+// its sections end exactly at the archive's size. Every planned kernel does
+// nothing unless `ok`; the host then checks the header it reads back.
+struct PlanExpect {
+    uint32_t n, width;
+    uint8_t flags, lx, lz, bz;
+};
+struct DecodeRecord {
+    uint32_t status, ok, np, nq;
+    unsigned long long wzp, pi, pv, qi, qv;
+    double quantum;
+    uint8_t header[32];
+};
+constexpr uint64_t kRecordBytes = 256;  // device / pinned record size (either result record)
+static_assert(sizeof(DecodeRecord) <= kRecordBytes, "decode record");
+
+WZ_D void plan_decode(const uint8_t* a, uint64_t bytes, const PlanExpect& e, DecodeRecord* out) {
+    DecodeRecord r{};
+    if (bytes >= 32)
+        for (int k = 0; k < 32; ++k) r.header[k] = a[k];
+    const uint8_t* h = r.header;
+    auto rd32 = [&](int o) {
+        return uint32_t(h[o]) | (uint32_t(h[o + 1]) << 8) | (uint32_t(h[o + 2]) << 16) | (uint32_t(h[o + 3]) << 24);
+    };
+    const uint64_t wzp = uint64_t(rd32(16)) | (uint64_t(rd32(20)) << 32);
+    const uint32_t pb = rd32(4), np = rd32(24), nq = rd32(28);
+    const double quantum = __longlong_as_double(
+        static_cast<long long>(uint64_t(rd32(8)) | (uint64_t(rd32(12)) << 32)));
+    const double qn = e.width == 4 ? double(float(quantum)) : quantum;
+    const uint64_t header = wzp_shape(e.n).header;
+    bool ok = bytes >= 32 && h[0] == e.flags && h[1] == e.lx && h[2] == e.lz && h[3] == e.bz;
+    ok = ok && quantum > 0 && __builtin_isfinite(quantum) && qn > 0 && __builtin_isfinite(qn);
+    ok = ok && np <= e.n && nq <= e.n && uint64_t(pb) == uint64_t(np) * e.width;
+    ok = ok && wzp >= header && wzp <= header + 2ull * e.n + (e.n + 7ull) / 8 && wzp <= bytes - 32;
+    if (ok) {
+        const Layout L = archive_layout(wzp, np, pb, nq);
+        ok = L.end == bytes;
+        r.np = np;
+        r.nq = nq;
+        r.wzp = wzp;
+        r.pi = L.pi;
+        r.pv = L.pv;
+        r.qi = L.qi;
+        r.qv = L.qv;
+        r.quantum = quantum;
+    }
+    if (!ok) r.np = r.nq = 0;
+    r.ok = ok;
+    *out = r;
+}
+
 // Device status bits shared by all kernels of one call.
 enum Status : uint32_t {
     kNonFiniteInput = 1u << 0,

@@ -369,12 +369,15 @@ __global__ void __launch_bounds__(FT) fwd_level_kernel(FwdLevel<T> a) {
 // and stores go through buffer windows at wave-uniform plane or row bases.
 // For the repair emitter each Y thread's 8 original samples are loaded as
 // the step begins.
-template <class T, int Emit, bool Requant = false, bool RowWords = true>
+template <class T, int Emit, bool Requant = false, bool RowWords = true, bool Patch = false>
 __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     constexpr int H = sizeof(T) == 4 ? 8 : 4;   // canonical Z tile / 2 (pairs per tile)
     __shared__ T Zp[2 * FH * PA];                // two reconstructed z planes (split XY)
     __shared__ T Xp[2 * FH * (TX + 1)];          // after X: rows still split in Y
     __shared__ uint32_t seg_fill;                // repair entries of this workgroup
+    // Patch: each Y wave's slots of its step's 8 rows x 64 lanes (the lane's
+    // repair, NaN for none)
+    __shared__ T pslot[Patch ? IT * 8 : 1];
     using C = Lift<T>;
     const uint32_t t = threadIdx.x;
     Tile tl;
@@ -468,6 +471,44 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
         const bool in = Z < n.nz;
         return window_n(base + uint64_t(in ? Z : 0u) * a.opz, in ? plane_bytes : 0u);
     };
+    // Patch (the decoder's repairs applied as samples are stored): in a Y
+    // wave's step, lanes 8r .. 8r+7 hold list entries first_r - 1 .. first_r
+    // + 6 of row r, first_r being the key table's entry for the row's
+    // 64-sample tile. Lane 8r proves the entry current (the entry before it
+    // lies below the tile: a stale entry fails this or the next test), lanes
+    // 8r+1 .. 8r+7 hold the tile's first repairs. The table entries of the
+    // next step are scalar loads (wave-uniform, outside the vector memory
+    // counter); the window loads as its step begins, ahead of the
+    // coefficient loads, so waiting for it never waits for those.
+    const uint32_t pl = t & 63u;
+    const uint32_t txn = (n.nx + TX - 1) / TX, tw = min(uint32_t(TX), n.nx - x0);
+    uint32_t np = a.np;  // the list (planned decode: from the device record)
+    const uint32_t* plist = a.pidx;
+    const T* perrs = a.perr;
+    if constexpr (Patch) {
+        if (a.plan) {
+            np = a.plan->ok ? a.plan->np : 0u;
+            plist = reinterpret_cast<const uint32_t*>(a.archive + a.plan->pi);
+            perrs = reinterpret_cast<const T*>(a.archive + a.plan->pv);
+        }
+    }
+    const auto pidx = window_n(plist, Patch ? 4u * np : 0u);
+    const auto perr = window_n(perrs, Patch ? uint32_t(sizeof(T)) * np : 0u);
+    using TableWord = __attribute__((address_space(4))) const uint32_t;  // read-only in this kernel
+    const auto* ptab = reinterpret_cast<TableWord*>(reinterpret_cast<uintptr_t>(a.ptable));
+    uint32_t tnext[8] = {};  // table entries of the next step's rows (0xFFFFFFFF: none)
+    auto load_table = [&](int32_t g) {
+        const uint32_t Z = 2 * uint32_t(g - 2) + yp;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t Y = y0 + 8 * yt + r;
+            const bool in = g >= 2 && Z < n.nz && Y < n.ny;
+            const uint32_t v = ptab[in ? (Z * n.ny + Y) * txn + tl.bx : 0u];
+            tnext[r] = in ? v : 0xFFFFFFFFu;
+        }
+    };
+    uint32_t pfv = 0, pci = 0;  // first_r of the lane's row, the lane's entry index
+    T pce = T(0);               // and its repair
     auto lane_off = [&](uint32_t Y) { return xin && Y < n.ny ? (Y * a.opy + X) * uint32_t(sizeof(T)) : kOutside; };
     // repair emitter: this workgroup's record segment (cap entries)
     const auto seg_index = window_n(a.rep.index + uint64_t(wg) * a.rep.cap, 4 * a.rep.cap);
@@ -508,6 +549,19 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     // its own previous stores before it starts the Z pass).
     auto step = [&](int32_t g, auto emit_c, auto y_c) {
         const bool emit = emit_c, yw = y_c;  // compile-time for float, run-time for double
+        if constexpr (Patch) {
+            if (emit) {
+                const uint32_t r = pl >> 3;
+                pfv = tnext[0];
+#pragma unroll
+                for (int k = 1; k < 8; ++k) pfv = r == uint32_t(k) ? tnext[k] : pfv;
+                const uint32_t pos = pfv - 1u + (pl & 7u);
+                const bool held = pos < np;
+                pci = __builtin_amdgcn_raw_buffer_load_b32(pidx, int(held ? 4u * pos : kOutside), 0, 0);
+                pce = buf_load<T>(perr, held ? uint32_t(sizeof(T)) * pos : kOutside);
+            }
+            load_table(g + 1);
+        }
         if constexpr (early) {
             if (emit) fetch_orig(g, ref);
         }
@@ -582,8 +636,55 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
         if constexpr (Emit != kEmitRepair) {
             bool fin = true;
             const auto wout = plane_window(a.out, Z);
+            T pv[8];  // Patch: this lane's repair in each row, NaN for none
+            if constexpr (Patch) {
+                const uint32_t r = pl >> 3, k = pl & 7u, Yr = y0 + 8 * yt + r;
+                const bool rin = Z < n.nz && Yr < n.ny;
+                const uint32_t lo = (Z * n.ny + Yr) * n.nx + x0;  // index of the row tile's first sample
+                const bool held = rin && pfv - 1u + k < np;
+                const bool in_tile = held && k && pci - lo < tw;
+                const unsigned long long cur = __ballot(k == 0 && rin && (pfv == 0 || (held && pci < lo)));
+                const unsigned long long opens = __ballot(k == 1 && in_tile);
+                const bool apply = in_tile && ((cur >> (8 * r)) & 1ull) && ((opens >> (8 * r + 1)) & 1ull);
+                T* slot = pslot + wave * 512u;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) slot[64 * j + pl] = T(__builtin_nan(""));
+                // lanes of one wave exchange through LDS in issue order
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (apply) slot[64 * r + (pci - lo)] = pce;
+                // a window still in its tile at its last lane: the rest of the
+                // tile's repairs (at most 64 per tile), 64 entries at a time
+                unsigned long long more = __ballot(apply && k == 7);
+                while (more) {
+                    const uint32_t L = uint32_t(__builtin_ctzll(more));
+                    more &= more - 1;
+                    const uint32_t lo2 = __builtin_amdgcn_readlane(lo, L);
+                    uint32_t base = __builtin_amdgcn_readlane(pfv, L) + 7u;
+                    for (;;) {
+                        const uint32_t pos = base + pl;
+                        const bool in = pos < np;
+                        const uint32_t ci = __builtin_amdgcn_raw_buffer_load_b32(pidx, int(in ? 4u * pos : kOutside), 0, 0);
+                        const T ce = buf_load<T>(perr, in ? uint32_t(sizeof(T)) * pos : kOutside);
+                        const bool hit = in && ci - lo2 < tw;
+                        if (hit) slot[64 * (L >> 3) + (ci - lo2)] = ce;
+                        if (!((__ballot(hit) >> 63) & 1ull)) break;
+                        base += 64;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int j = 0; j < 8; ++j) pv[j] = slot[64 * j + pl];
+            }
             auto put = [&](uint32_t j, T v) {
                 const uint32_t Y = y0 + 8 * yt + j;
+                if constexpr (Patch) {
+                    const T e = pv[j];
+                    if (e == e) v = v - e;  // cpu/src/codec.cpp:330
+                }
                 buf_store_nt(v, wout, lane_off(Y));
                 fin = fin && (!(xz && Y < n.ny) || __builtin_isfinite(v));
             };
@@ -695,11 +796,14 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
             for (; g < gE; ++g) step(g, std::false_type{}, std::true_type{});
             // as many dropped stores as an emitting step issues, so the
             // emitting loop is entered with the memory-counter state it keeps
-            // (the counter tracks stores too) and its waits stay exact
+            // (the counter tracks stores too) and its waits stay exact; each
+            // at its own offset past the empty window, or the compiler keeps
+            // only the last of identical stores
             constexpr int kStores = Emit == kEmitRepair ? 17 : 8;
             const auto none = window_n(a.out ? static_cast<const void*>(a.out) : a.hi, 0u);
 #pragma unroll
-            for (int k = 0; k < kStores; ++k) __builtin_amdgcn_raw_buffer_store_b32(0u, none, int(kOutside), 0, 0);
+            for (int k = 0; k < kStores; ++k)
+                __builtin_amdgcn_raw_buffer_store_b32(0u, none, int(kOutside - 16u * uint32_t(k)), 0, 0);
             for (; g <= g1; ++g) step(g, std::true_type{}, std::true_type{});
         } else {
             for (; g < gE; ++g) step(g, std::false_type{}, std::false_type{});
@@ -743,8 +847,12 @@ void launch_inv_level(const InvLevel<T>& a, hipStream_t s) {
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair>), grid, dim3(IT), 0, s, a);
     else if (a.emit == kEmitRepair)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitRepair, false, false>), grid, dim3(IT), 0, s, a);
+    else if (a.emit == kEmitStoreScreen && a.np)
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen, false, true, true>), grid, dim3(IT), 0, s, a);
     else if (a.emit == kEmitStoreScreen)
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitStoreScreen>), grid, dim3(IT), 0, s, a);
+    else if (a.q > T(0))  // a coarser level of the split compress inverse
+        hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore, true>), grid, dim3(IT), 0, s, a);
     else
         hipLaunchKernelGGL((inv_level_kernel<T, kEmitStore>), grid, dim3(IT), 0, s, a);
 }

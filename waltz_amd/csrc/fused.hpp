@@ -46,8 +46,16 @@ struct InvLevel {
     const T* orig;      // kEmitRepair: the original field
     double bound;
     float bound_lo;     // float repair emitter: the largest float <= bound
-    T q, inv;           // repair emitter, q > 0: high-band loads are quantized and dequantized
+    T q, inv;           // q > 0: high-band loads are quantized and dequantized (repair emitter, store)
     SegList rep;        // kEmitRepair: one segment per workgroup, cap = 64*32*sz
+    // kEmitStoreScreen, np > 0: repairs applied as the samples are stored
+    // (strictly increasing indices; key table from repair_keys, `keys` words)
+    const uint32_t* pidx;
+    const T* perr;
+    const uint32_t* ptable;
+    uint32_t np, keys;
+    const DecodeRecord* plan;  // non-null: np and the list's places from this record of `archive`
+    const uint8_t* archive;
 };
 
 // Repair segments needed by a level-0 inverse with z segment sz.

@@ -34,7 +34,23 @@ struct Level0Emit {
     SegList rep;       // segments/cap per level0_repair_segments()
     double quantum;    // kEmitRepair: > 0 when the finest level's high bands hold the
                        // unquantized coefficients (quantized and dequantized on load)
+    // kEmitStoreScreen: a repair list (strictly increasing indices, np > 0
+    // entries; np = 0: none) applied to the samples as they are stored, and
+    // scratch for its key table (patch_table_words(full) words)
+    const uint32_t* pidx = nullptr;
+    const T* perr = nullptr;
+    uint32_t np = 0;
+    uint32_t* ptable = nullptr;
+    bool ptable_built = false;  // the caller already ran repair_keys into ptable
+    // a planned decode: the list's count and places come from this device
+    // record (np above is then only an upper bound, for the size checks)
+    const DecodeRecord* plan = nullptr;
+    const uint8_t* archive = nullptr;
 };
+// Words of the folded repair patches' key table (one per row and 64-sample x
+// tile), or 0 when a list of np entries of `width`-byte values cannot be
+// folded into the finest level's stores for this geometry.
+uint64_t patch_table_words(Dims full, uint32_t np, unsigned width);
 // Returns true when `emit` was carried out inside the fused finest level;
 // false means the caller must run the equivalent separate pass on `output`.
 template <class T>
@@ -45,6 +61,19 @@ bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
 template <class T>
 void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
                        bool of_inverse, hipStream_t s);
+// The compress inverse of a brick-pair field whose coefficients stay
+// unquantized (the encoder writes nothing back), in two halves: levels >= 2
+// on a quantized and dequantized compact copy of their box (in t1), level 1
+// as a fused kernel that requantizes the high bands it loads, its
+// reconstruction (compact, the level-1 box) at the start of t0; then the
+// finest level fused with the same requantization and `emit`. Applies when
+// coarse_split_fits (dyadic, >= 2 levels, both leading levels fused).
+bool coarse_split_fits(Dims full, unsigned lxy, unsigned width);
+template <class T>
+void inverse_coarse_requant(const T* coeff, T* t0, T* t1, Dims full, unsigned lxy, double quantum, hipStream_t s);
+template <class T>
+void inverse_finest_requant(const T* coeff, const T* t0, Dims full, unsigned lxy, const Level0Emit<T>& emit,
+                            hipStream_t s);
 // Segments and per-segment capacity the fused finest inverse level uses for
 // repairs (0 segments when that level is not fused for this geometry).
 void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width, uint32_t* segments,
@@ -94,9 +123,10 @@ struct SideList {          // unsorted (index, payload) list + row-major flag bi
 template <class T>
 void quantize_bricks(T* coeff, const BrickGeometry& g, double quantum, const uint16_t* rank, uint16_t* mag,
                      uint32_t* sign, SideList q, uint32_t* status, hipStream_t s);
+// plan: a planned decode's device record (quantum from it; nothing unless ok)
 template <class T>
 void dequantize_bricks(const uint16_t* mag, const uint32_t* sign, const BrickGeometry& g, double quantum,
-                       const uint16_t* rank, T* out, hipStream_t s);
+                       const uint16_t* rank, T* out, hipStream_t s, const DecodeRecord* plan = nullptr);
 
 // wzp.hip
 struct WzpScratch {
@@ -112,6 +142,7 @@ struct WzpScratch {
 struct SideStream {
     hipStream_t stream;
     hipEvent_t ready, done;
+    hipEvent_t mid;  // a join point inside the side stream's work (decode: the repair key table built)
 };
 // Staging of the encoder: kChunkSlot bytes per chunk plus read-ahead padding.
 inline uint64_t wzp_staging_bytes(uint32_t chunks) { return uint64_t(chunks) * kChunkSlot + 64; }
@@ -120,26 +151,33 @@ inline uint64_t wzp_staging_bytes(uint32_t chunks) { return uint64_t(chunks) * k
 void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* out, uint64_t capacity,
                 WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
                 const SideStream* pack = nullptr);
-// Checks the group directory of a WZP stream of `bytes` (>= its header)
-// bytes: spans inside the payload and tiling it exactly, else kCorrupt.
-// wzp_decode and decode_dequantize_pairs run it first, on `side` when given
-// (beside the chunk decoders: both only read the stream; the caller's stream
-// must wait for side->done before it reads the status).
-void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s,
-                         const SideStream* side = nullptr);
-// Decodes a WZP stream of `bytes` bytes into packed mag/sign arrays.
+// Decodes a WZP stream of `bytes` bytes into packed mag/sign arrays. The
+// decoders and the decode_* calls below also check the group directory
+// (spans inside the payload and tiling it exactly, else kCorrupt), on `side`
+// when given, beside the chunk decoders (both only read the stream); the
+// caller's stream must wait for side->done before it reads the status.
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
                 hipStream_t s, const SideStream* side = nullptr);
 // Paired-brick geometry only (pair_bricks_fit): quantization fused into the
-// encoder (coefficients dequantized in place, q-outliers appended to `qo`),
-// and dequantization fused into the decoder (writes the coefficient volume).
+// encoder (q-outliers appended to `qo`; the dequantized coefficients written
+// back in place over the whole volume, over the coarser levels' box only, or
+// nowhere), and dequantization fused into the decoder (writes the
+// coefficient volume).
+enum WriteBack { kWriteAll, kWriteCoarseBox, kWriteNone };
 template <class T>
 void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
                            WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
-                           bool coarse_box_only = false, const SideStream* pack = nullptr);
+                           WriteBack wb = kWriteAll, const SideStream* pack = nullptr);
 template <class T>
 void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double quantum, T* coeff, uint32_t* status,
                              hipStream_t s, const SideStream* side = nullptr);
+// The same two, planned on the device: stream size and quantum from `rec`
+// (nothing happens unless rec->ok); the directory check on `side` when given.
+template <class T>
+void decode_dequantize_pairs_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, T* coeff, uint32_t* status,
+                                     hipStream_t s, const SideStream* side);
+void wzp_decode_planned(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint16_t* mag, uint32_t* sign,
+                        uint32_t* status, hipStream_t s, const SideStream* side);
 
 // sections.hip
 constexpr uint32_t kRepairTile = 4096;  // row-major indices per standalone repair segment
@@ -165,6 +203,23 @@ void apply_qoutliers(T* coeff, const uint8_t* idx, const uint8_t* val, uint32_t 
 template <class T>
 void apply_repairs(T* out, const uint8_t* idx, const uint8_t* err, uint32_t count, uint32_t n, uint32_t* status,
                    hipStream_t s);
+// The key table of a repair list folded into the finest inverse level
+// (entry of row r and x tile t at r * ceil(nx/64) + t: the list position of
+// the tile's first repair; tiles without repairs keep stale entries, which
+// the reader rejects against the list), with rpatch's record checks.
+template <class T>
+void repair_keys(const uint32_t* idx, const T* err, uint32_t count, Dims d, uint32_t* table, uint32_t* status,
+                 hipStream_t s);
+// Planned decode (DecodeRecord): the header check into `rec` (its status
+// cleared), and the list kernels reading counts and places from it.
+void decode_plan(const uint8_t* archive, uint64_t archive_bytes, const PlanExpect& e, DecodeRecord* rec,
+                 hipStream_t s);
+template <class T>
+void apply_qoutliers_planned(T* coeff, const uint8_t* archive, const DecodeRecord* rec, uint32_t n, uint32_t* status,
+                             hipStream_t s);
+template <class T>
+void repair_keys_planned(const uint8_t* archive, const DecodeRecord* rec, Dims d, uint32_t* table, uint32_t* status,
+                         hipStream_t s);
 // Flags kCorrupt when an index of the list repeats (or is >= n); `bitmap`:
 // ceil(n/32) words, cleared here.
 void check_unique(const uint8_t* idx, uint32_t count, uint32_t n, uint32_t* bitmap, uint32_t* status, hipStream_t s);
@@ -184,13 +239,13 @@ void decompress(const uint8_t* archive, uint64_t archive_bytes, Dims d, T* outpu
 // Decode with memory the caller supplies (no device allocation inside):
 // `scratch` holds scratch_bytes >= decode_scratch_bytes(n, width) bytes,
 // 256-byte aligned; `rank` 2 * 8 * 8192 u16 (rebuilt by every call that
-// needs them); `result` one device EncodeResult. Same checks and results as
-// decompress.
+// needs them); `result` kRecordBytes of device memory. Same checks and
+// results as decompress.
 struct DecodeMemory {
     uint8_t* scratch;
     uint64_t scratch_bytes;
     uint16_t* rank;
-    EncodeResult* result;
+    DecodeRecord* result;
 };
 uint64_t decode_scratch_bytes(uint32_t n, unsigned width);
 template <class T>

@@ -32,7 +32,7 @@ constexpr uint64_t kDecodeBytes = 3ull * 8 * kMaxSamples + 2ull * kMaxSamples + 
 constexpr uint64_t kRankBytes = 2ull * 8 * 8192 * sizeof(uint16_t);
 constexpr uint64_t kResultBytes = 256;
 constexpr uint64_t kScratchBytes = kDecodeBytes + kRankBytes + kResultBytes;
-static_assert(kDecodeBytes % 256 == 0 && sizeof(wz::EncodeResult) <= kResultBytes, "scratch layout");
+static_assert(kDecodeBytes % 256 == 0 && wz::kRecordBytes <= kResultBytes, "scratch layout");
 
 constexpr int kMaxDevices = 64;
 
@@ -77,7 +77,7 @@ extern "C" const char* waltz_cuda_decode_chunk_noalloc(uint32_t nx, uint32_t ny,
             throw std::invalid_argument("chunk exceeds the fixed decode scratch (1048576 samples)");
         uint8_t* base = device_scratch();
         const wz::DecodeMemory mem{base, kDecodeBytes, reinterpret_cast<uint16_t*>(base + kDecodeBytes),
-                                   reinterpret_cast<wz::EncodeResult*>(base + kDecodeBytes + kRankBytes)};
+                                   reinterpret_cast<wz::DecodeRecord*>(base + kDecodeBytes + kRankBytes)};
         const wz::Dims d{nx, ny, nz};
         if (width == 4)
             wz::decompress_into<float>(static_cast<const uint8_t*>(archive), archive_bytes, d,

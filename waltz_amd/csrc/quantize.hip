@@ -71,7 +71,12 @@ void quantize_bricks(T* coeff, const BrickGeometry& g, double quantum, const uin
 template <class T>
 __global__ void __launch_bounds__(256) dequantize_kernel(const uint16_t* __restrict__ mag,
                                                          const uint32_t* __restrict__ sign, BrickGeometry g, T q,
+                                                         const DecodeRecord* __restrict__ plan,
                                                          const uint16_t* __restrict__ rank, T* __restrict__ out) {
+    if (plan) {  // a planned decode: the quantum of the header checked on the device
+        if (!plan->ok) return;
+        q = T(plan->quantum);
+    }
     const Brick k = brick_of(g, blockIdx.x);
     const uint32_t vol = k.bx * k.by * k.bz;
     constexpr uint32_t kBurst = 8;  // all loads of a burst issued before its stores
@@ -99,11 +104,11 @@ __global__ void __launch_bounds__(256) dequantize_kernel(const uint16_t* __restr
 
 template <class T>
 void dequantize_bricks(const uint16_t* mag, const uint32_t* sign, const BrickGeometry& g, double quantum,
-                       const uint16_t* rank, T* out, hipStream_t s) {
+                       const uint16_t* rank, T* out, hipStream_t s, const DecodeRecord* plan) {
     const uint64_t bricks = uint64_t(g.nbx) * g.nby * g.nbz;
     check_launch_size(bricks * 256);
     hipLaunchKernelGGL((dequantize_kernel<T>), dim3(uint32_t(bricks)), dim3(256), 0, s, mag, sign, g, T(quantum),
-                       rank, out);
+                       plan, rank, out);
 }
 
 template void quantize_bricks<float>(float*, const BrickGeometry&, double, const uint16_t*, uint16_t*, uint32_t*,
@@ -111,8 +116,8 @@ template void quantize_bricks<float>(float*, const BrickGeometry&, double, const
 template void quantize_bricks<double>(double*, const BrickGeometry&, double, const uint16_t*, uint16_t*, uint32_t*,
                                       SideList, uint32_t*, hipStream_t);
 template void dequantize_bricks<float>(const uint16_t*, const uint32_t*, const BrickGeometry&, double,
-                                       const uint16_t*, float*, hipStream_t);
+                                       const uint16_t*, float*, hipStream_t, const DecodeRecord*);
 template void dequantize_bricks<double>(const uint16_t*, const uint32_t*, const BrickGeometry&, double,
-                                        const uint16_t*, double*, hipStream_t);
+                                        const uint16_t*, double*, hipStream_t, const DecodeRecord*);
 
 }  // namespace wz

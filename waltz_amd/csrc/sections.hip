@@ -96,11 +96,10 @@ void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SegList rep
 }
 
 // ------------------------------------------------------------------ decode patches
+// q-outlier record j: the coefficient's magnitude replaced (sign kept).
 template <class T>
-__global__ void qpatch_kernel(T* __restrict__ c, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ val,
-                              uint32_t count, uint32_t n, T q, uint32_t* __restrict__ status) {
-    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-    if (j >= count) return;
+WZ_D void qpatch_one(T* __restrict__ c, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ val,
+                     uint32_t j, uint32_t n, T q, uint32_t* __restrict__ status) {
     const uint32_t i = idx[j], v = val[j];
     if (i >= n || v <= 65535u || v > 0x7FFFFFFFu) {
         atomicOr(status, uint32_t(kCorrupt));
@@ -109,6 +108,13 @@ __global__ void qpatch_kernel(T* __restrict__ c, const uint32_t* __restrict__ id
     if (j && idx[j - 1] >= i) atomicOr(status, uint32_t(kUnordered));  // duplicates are checked apart
     const T x = T(v) * q;  // cpu/src/codec.cpp:299-302
     c[i] = c[i] < T(0) ? -x : x;
+}
+
+template <class T>
+__global__ void qpatch_kernel(T* __restrict__ c, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ val,
+                              uint32_t count, uint32_t n, T q, uint32_t* __restrict__ status) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j < count) qpatch_one(c, idx, val, j, n, q, status);
 }
 
 template <class T>
@@ -126,6 +132,71 @@ __global__ void rpatch_kernel(T* __restrict__ out, const uint32_t* __restrict__ 
     const T v = out[i] - e;  // cpu/src/codec.cpp:330
     out[i] = v;
     if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
+}
+
+// Repair record j: the checks of rpatch_kernel, and the record that opens
+// its (row, 64-sample x tile) key writes its position to the table.
+template <class T>
+WZ_D void repair_key_one(const uint32_t* __restrict__ idx, const T* __restrict__ err, uint32_t j, uint32_t n,
+                         uint32_t nx, uint32_t txn, uint32_t* __restrict__ table, uint32_t* __restrict__ status) {
+    const uint32_t i = idx[j];
+    if (i >= n || !__builtin_isfinite(err[j])) {
+        atomicOr(status, uint32_t(kCorrupt));
+        return;
+    }
+    auto key = [&](uint32_t v) { return (v / nx) * txn + (v % nx) / 64u; };
+    bool opens = true;
+    if (j) {
+        const uint32_t p = idx[j - 1];
+        if (p >= i) atomicOr(status, uint32_t(kUnordered));
+        opens = p >= n || key(p) != key(i);
+    }
+    if (opens) table[key(i)] = j;
+}
+
+template <class T>
+__global__ void repair_keys_kernel(const uint32_t* __restrict__ idx, const T* __restrict__ err, uint32_t count,
+                                   uint32_t n, uint32_t nx, uint32_t txn, uint32_t* __restrict__ table,
+                                   uint32_t* __restrict__ status) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j < count) repair_key_one(idx, err, j, n, nx, txn, table, status);
+}
+
+// ------------------------------------------------------------------ planned decode
+// The header check of a decode planned on the device (one thread).
+__global__ void decode_plan_kernel(const uint8_t* __restrict__ archive, uint64_t bytes, PlanExpect e,
+                                   DecodeRecord* __restrict__ rec) {
+    if (threadIdx.x == 0) plan_decode(archive, bytes, e, rec);
+}
+
+// The list kernels of a planned decode: a fixed grid walks the record's counts.
+constexpr uint32_t kPlannedBlocks = 512;
+
+template <class T>
+__global__ void __launch_bounds__(256) qpatch_planned_kernel(T* __restrict__ c, const uint8_t* __restrict__ archive,
+                                                             const DecodeRecord* __restrict__ rec, uint32_t n,
+                                                             uint32_t* __restrict__ status) {
+    if (!rec->ok) return;
+    const uint32_t count = rec->nq;
+    const auto* idx = reinterpret_cast<const uint32_t*>(archive + rec->qi);
+    const auto* val = reinterpret_cast<const uint32_t*>(archive + rec->qv);
+    const T q = T(rec->quantum);
+    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < count; j += gridDim.x * 256u)
+        qpatch_one(c, idx, val, j, n, q, status);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) repair_keys_planned_kernel(const uint8_t* __restrict__ archive,
+                                                                  const DecodeRecord* __restrict__ rec, uint32_t n,
+                                                                  uint32_t nx, uint32_t txn,
+                                                                  uint32_t* __restrict__ table,
+                                                                  uint32_t* __restrict__ status) {
+    if (!rec->ok) return;
+    const uint32_t count = rec->np;
+    const auto* idx = reinterpret_cast<const uint32_t*>(archive + rec->pi);
+    const auto* err = reinterpret_cast<const T*>(archive + rec->pv);
+    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < count; j += gridDim.x * 256u)
+        repair_key_one(idx, err, j, n, nx, txn, table, status);
 }
 
 // Duplicate detection of an unordered side list (cpu/src/codec.cpp:260-275).
@@ -166,6 +237,45 @@ void apply_repairs(T* out, const uint8_t* idx, const uint8_t* err, uint32_t coun
                        reinterpret_cast<const uint32_t*>(idx), reinterpret_cast<const T*>(err), count, n, status);
 }
 
+template <class T>
+void repair_keys(const uint32_t* idx, const T* err, uint32_t count, Dims d, uint32_t* table, uint32_t* status,
+                 hipStream_t s) {
+    if (!count) return;
+    const uint32_t n = d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL((repair_keys_kernel<T>), dim3((count + 255) / 256), dim3(256), 0, s, idx, err, count, n, d.nx,
+                       (d.nx + 63) / 64, table, status);
+}
+
+void decode_plan(const uint8_t* archive, uint64_t archive_bytes, const PlanExpect& e, DecodeRecord* rec,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(decode_plan_kernel, dim3(1), dim3(64), 0, s, archive, archive_bytes, e, rec);
+}
+
+template <class T>
+void apply_qoutliers_planned(T* coeff, const uint8_t* archive, const DecodeRecord* rec, uint32_t n, uint32_t* status,
+                             hipStream_t s) {
+    hipLaunchKernelGGL((qpatch_planned_kernel<T>), dim3(kPlannedBlocks), dim3(256), 0, s, coeff, archive, rec, n,
+                       status);
+}
+
+template <class T>
+void repair_keys_planned(const uint8_t* archive, const DecodeRecord* rec, Dims d, uint32_t* table, uint32_t* status,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((repair_keys_planned_kernel<T>), dim3(kPlannedBlocks), dim3(256), 0, s, archive, rec,
+                       d.nx * d.ny * d.nz, d.nx, (d.nx + 63) / 64, table, status);
+}
+
+uint64_t patch_table_words(Dims full, uint32_t np, unsigned width) {
+    const uint64_t n = uint64_t(full.nx) * full.ny * full.nz;
+    const uint64_t words = uint64_t(full.ny) * full.nz * ((full.nx + 63) / 64);
+    // the table sits in the decoder's magnitude scratch (2n bytes); the
+    // kernel's 32-bit buffer windows hold the table and both list sections
+    const uint64_t limit = uint64_t(1) << 31;
+    if (!np || 4 * words > 2 * n || 4 * words > limit || uint64_t(np) * 4 > limit || uint64_t(np) * width > limit)
+        return 0;
+    return words;
+}
+
 void check_unique(const uint8_t* idx, uint32_t count, uint32_t n, uint32_t* bitmap, uint32_t* status, hipStream_t s) {
     if (!count) return;
     zero_words(bitmap, (uint64_t(n) + 31) / 32, s);
@@ -185,6 +295,11 @@ void check_finite(const T* out, uint32_t n, uint32_t* status, hipStream_t s) {
     template void apply_qoutliers<T>(T*, const uint8_t*, const uint8_t*, uint32_t, uint32_t, double, uint32_t*,    \
                                      hipStream_t);                                                                \
     template void apply_repairs<T>(T*, const uint8_t*, const uint8_t*, uint32_t, uint32_t, uint32_t*, hipStream_t); \
+    template void repair_keys<T>(const uint32_t*, const T*, uint32_t, Dims, uint32_t*, uint32_t*, hipStream_t);     \
+    template void apply_qoutliers_planned<T>(T*, const uint8_t*, const DecodeRecord*, uint32_t, uint32_t*,         \
+                                             hipStream_t);                                                        \
+    template void repair_keys_planned<T>(const uint8_t*, const DecodeRecord*, Dims, uint32_t*, uint32_t*,           \
+                                         hipStream_t);                                                            \
     template void check_finite<T>(const T*, uint32_t, uint32_t*, hipStream_t);
 WZ_INST(float)
 WZ_INST(double)

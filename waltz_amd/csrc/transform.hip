@@ -92,6 +92,17 @@ __global__ void __launch_bounds__(256) plane_pair_pass(const T* __restrict__ src
     pass(std::integral_constant<int, A1>{}, P1, 0, act.nx, dst, zb, full.nx);
 }
 
+// Box `act` of a full-pitch volume, quantized and dequantized (the value a
+// coefficient holds after quantization), to a compact array of pitch act.nx.
+template <class T>
+__global__ void __launch_bounds__(256) box_requant(const T* __restrict__ src, T* __restrict__ dst, Dims full, Dims act,
+                                                   uint32_t total, T q, T inv) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t x = i % act.nx, r = i / act.nx, y = r % act.ny, z = r / act.ny;
+    dst[i] = requantize<T>(src[(uint64_t(z) * full.ny + y) * full.nx + x], q, inv);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) box_copy(const T* __restrict__ src, T* __restrict__ dst, Dims full, Dims act,
                                                 uint32_t total) {
@@ -190,13 +201,14 @@ struct Passes {
     hipStream_t s;
     Tuning tune;  // tiles and LDS pair limit of the per-axis passes
     bool first = true;
+    unsigned base = 0;  // level number of the box's level 0 (stage names)
 
     // Passes: src -> tmp0 -> tmp1 -> ... -> final. A single pass whose source
     // is already `home` goes through tmp0 and is copied back.
     void level(Dims full, Dims act, const int* axes, int count, bool last_level, unsigned index) {
         static const char* fwd[] = {"forward_L0", "forward_L1", "forward_L2", "forward_L3", "forward_L4"};
         static const char* inv[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
-        StageClock clock(Inverse ? inv[index] : fwd[index], s);
+        StageClock clock(Inverse ? inv[index + base] : fwd[index + base], s);
         const T* src = first ? first_src : home;
         T* final_dst = last_level && last_dst ? last_dst : home;
         for (int k = 0; k < count; ++k) {
@@ -363,17 +375,78 @@ void fused_forward(const T* input, T* coeff, T* tmp0, Dims full, unsigned levels
     }
 }
 
+// The finest inverse level as a fused kernel: high bands from `coeff`, the
+// coarser reconstruction from `lo` (pitches lpy, lpz), into `output`, with
+// the caller's emitter (store, screen and folded repairs, or the repair
+// emitter, which may requantize the high bands on load).
+template <class T>
+void launch_finest(const T* coeff, const T* lo, uint32_t lpy, uint32_t lpz, T* output, Dims full, uint32_t sz,
+                   const Level0Emit<T>* emit, hipStream_t s, const char* name) {
+    const bool repairs = emit && emit->mode == kEmitRepair;
+    const bool patch = emit && emit->mode == kEmitStoreScreen && emit->np;
+    const uint64_t keys = patch ? patch_table_words(full, emit->np, sizeof(T)) : 0;
+    if (!keys && patch) throw std::logic_error("repair list cannot be folded into the finest level");
+    if (patch && !emit->ptable_built && emit->plan) throw std::logic_error("planned repair keys not built");
+    if (patch && !emit->ptable_built) {  // the key table of the repairs the finest level applies
+        StageClock clock("patch_keys", s);
+        repair_keys<T>(emit->pidx, emit->perr, emit->np, full, emit->ptable, emit->status, s);
+    }
+    StageClock clock(repairs ? "inverse_L0_repairs" : name, s);
+    InvLevel<T> a{};
+    a.hi = coeff;
+    a.hpy = full.nx;
+    a.hpz = uint64_t(full.nx) * full.ny;
+    a.lo = lo;
+    a.lpy = lpy;
+    a.lpz = lpz;
+    a.n = full;
+    a.out = output;
+    a.opy = full.nx;
+    a.opz = uint64_t(full.nx) * full.ny;
+    a.sz = sz;
+    a.emit = kEmitStore;
+    if (emit) {
+        a.emit = emit->mode;
+        a.status = emit->status;
+        a.orig = emit->orig;
+        a.bound = emit->bound;
+        if (repairs && emit->quantum > 0) {
+            a.q = T(emit->quantum);
+            a.inv = T(1.0 / emit->quantum);
+        }
+        a.bound_lo = float(emit->bound);
+        if (double(a.bound_lo) > emit->bound) a.bound_lo = std::nextafter(a.bound_lo, 0.0f);
+        a.rep = emit->rep;
+        if (patch) {
+            a.pidx = emit->pidx;
+            a.perr = emit->perr;
+            a.ptable = emit->ptable;
+            a.np = emit->np;
+            a.keys = uint32_t(keys);
+            a.plan = emit->plan;
+            a.archive = emit->archive;
+        }
+        if (repairs) {  // the segment layout the caller allocated fixes the z segment
+            a.sz = emit->rep.cap / (64u * 32u);
+            if (a.sz % 16 || inv_segments(full, a.sz) != emit->rep.segments)
+                throw std::logic_error("repair segments do not match the finest level's launch");
+        }
+        if (repairs && (full.nx & 31u))  // atomics path needs a clean bitmap
+            zero_words(emit->rep.bitmap, (uint64_t(full.nx) * full.ny * full.nz + 31) / 32, s);
+    }
+    launch_inv_level<T>(a, s);
+}
+
 template <class T>
 void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, const Tuning& tu,
-                   const Level0Emit<T>* emit, hipStream_t s) {
+                   const Level0Emit<T>* emit, hipStream_t s, unsigned base) {
     Dims sizes[5], n = full;
     for (unsigned l = 0; l < levels; ++l) { sizes[l] = n; n = shrink(n, true, true); }
     const T* lo = coeff;
     uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
     static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
-    for (int l = int(levels) - 1; l >= 0; --l) {
-        const bool repairs = l == 0 && emit && emit->mode == kEmitRepair;
-        StageClock clock(repairs ? "inverse_L0_repairs" : names[l], s);
+    for (int l = int(levels) - 1; l >= 1; --l) {
+        StageClock clock(names[l + base], s);
         InvLevel<T> a{};
         a.hi = coeff;
         a.hpy = full.nx;
@@ -382,42 +455,17 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
         a.lpy = lpy;
         a.lpz = lpz;
         a.n = sizes[l];
-        if (l == 0) {
-            a.out = output;
-            a.opy = full.nx;
-            a.opz = uint64_t(full.nx) * full.ny;
-        } else {
-            a.out = low_box(tmp0, full, unsigned(l));
-            a.opy = sizes[l].nx;
-            a.opz = uint64_t(sizes[l].nx) * sizes[l].ny;
-        }
+        a.out = low_box(tmp0, full, unsigned(l));
+        a.opy = sizes[l].nx;
+        a.opz = uint64_t(sizes[l].nx) * sizes[l].ny;
         a.sz = tu.sz[l];
         a.emit = kEmitStore;
-        if (l == 0 && emit) {
-            a.emit = emit->mode;
-            a.status = emit->status;
-            a.orig = emit->orig;
-            a.bound = emit->bound;
-            if (emit->mode == kEmitRepair && emit->quantum > 0) {
-                a.q = T(emit->quantum);
-                a.inv = T(1.0 / emit->quantum);
-            }
-            a.bound_lo = float(emit->bound);
-            if (double(a.bound_lo) > emit->bound) a.bound_lo = std::nextafter(a.bound_lo, 0.0f);
-            a.rep = emit->rep;
-            if (emit->mode == kEmitRepair) {  // the segment layout the caller allocated fixes the z segment
-                a.sz = emit->rep.cap / (64u * 32u);
-                if (a.sz % 16 || inv_segments(full, a.sz) != emit->rep.segments)
-                    throw std::logic_error("repair segments do not match the finest level's launch");
-            }
-            if (emit->mode == kEmitRepair && (full.nx & 31u))  // atomics path needs a clean bitmap
-                zero_words(emit->rep.bitmap, (uint64_t(full.nx) * full.ny * full.nz + 31) / 32, s);
-        }
         launch_inv_level<T>(a, s);
         lo = a.out;
         lpy = a.opy;
         lpz = a.opz;
     }
+    launch_finest<T>(coeff, lo, uint32_t(lpy), uint32_t(lpz), output, full, tu.sz[0], emit, s, names[base]);
 }
 
 void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width, uint32_t* segments,
@@ -478,17 +526,19 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
     }
 }
 
+namespace {
+// The inverse with launch choices `tu`; `base`: the level number of this
+// field's level 0 (a coarse box run as a field of its own), for stage names.
 template <class T>
-bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       hipStream_t s, const Level0Emit<T>* emit) {
-    Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s,
-                      tuning_for(full, lxy, sizeof(T), true)};
+bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                  hipStream_t s, const Level0Emit<T>* emit, const Tuning& tu, unsigned base) {
+    Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s, tu};
+    p.base = base;
     if (lxy == 0 && lz == 0) {
         if (output != coeff) copy_box<T>(coeff, output, full, full, s);
         return false;
     }
     if (dyadic) {
-        const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
         const unsigned fused = tu.fused;
         Dims sizes[5], act = full;
         for (unsigned l = 0; l < lxy; ++l) { sizes[l] = act; act = shrink(act, true, true); }
@@ -498,10 +548,10 @@ bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
             p.level(full, sizes[l], zxy, 3, !fused && l == 0, unsigned(l));
         if (!fused) return false;
         if (output == coeff) {  // the level-0 kernel must not overwrite what it reads
-            fused_inverse<T>(coeff, tmp1, tmp0, full, fused, tu, emit, s);
+            fused_inverse<T>(coeff, tmp1, tmp0, full, fused, tu, emit, s, base);
             if (!emit || emit->mode != kEmitRepair) copy_box<T>(tmp1, coeff, full, full, s);
         } else {
-            fused_inverse<T>(coeff, output, tmp0, full, fused, tu, emit, s);
+            fused_inverse<T>(coeff, output, tmp0, full, fused, tu, emit, s, base);
         }
         return emit != nullptr;
     }
@@ -516,6 +566,73 @@ bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
     return false;
 }
 
+uint64_t pad64(uint64_t v) { return (v + 63) & ~uint64_t(63); }
+uint64_t volume(Dims d) { return uint64_t(d.nx) * d.ny * d.nz; }
+}  // namespace
+
+template <class T>
+bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
+                       hipStream_t s, const Level0Emit<T>* emit) {
+    return inverse_impl<T>(coeff, output, tmp0, tmp1, full, lxy, lz, dyadic, s, emit,
+                           tuning_for(full, lxy, sizeof(T), true), 0);
+}
+
+bool coarse_split_fits(Dims full, unsigned lxy, unsigned width) {
+    if (lxy < 2 || tuning_for(full, lxy, width, true).fused < 2) return false;
+    const Dims h1 = shrink(full, true, true), h2 = shrink(h1, true, true);
+    return 4 * pad64(volume(h2)) <= volume(full) && volume(h1) <= volume(full);
+}
+
+template <class T>
+void inverse_coarse_requant(const T* coeff, T* t0, T* t1, Dims full, unsigned lxy, double quantum, hipStream_t s) {
+    if (!coarse_split_fits(full, lxy, sizeof(T))) throw std::logic_error("coarse inverse split does not apply");
+    const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
+    const Dims h1 = shrink(full, true, true), h2 = shrink(h1, true, true);
+    const uint64_t v2 = pad64(volume(h2));
+    T *c2 = t1, *o2 = t1 + v2, *w0 = t1 + 2 * v2, *w1 = t1 + 3 * v2;
+    const T q = T(quantum), inv = T(1.0 / quantum);
+    {
+        StageClock clock("coarse_requant", s);
+        const uint64_t total = volume(h2);
+        check_launch_size(total);
+        hipLaunchKernelGGL((box_requant<T>), dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, coeff, c2, full, h2,
+                           uint32_t(total), q, inv);
+    }
+    const T* lo = c2;
+    if (lxy > 2) {  // levels >= 2: the box as a field of its own, with this field's launch choices
+        Tuning t2 = tu;
+        t2.fused = tu.fused - 2;
+        for (unsigned l = 0; l < 5; ++l) t2.sz[l] = l + 2 < 5 ? tu.sz[l + 2] : 16u;
+        inverse_impl<T>(c2, o2, w0, w1, h2, lxy - 2, lxy - 2, true, s, nullptr, t2, 2);
+        lo = o2;
+    }
+    StageClock clock("inverse_L1", s);
+    InvLevel<T> a{};
+    a.hi = coeff;
+    a.hpy = full.nx;
+    a.hpz = uint64_t(full.nx) * full.ny;
+    a.lo = lo;
+    a.lpy = h2.nx;
+    a.lpz = uint64_t(h2.nx) * h2.ny;
+    a.n = h1;
+    a.out = t0;
+    a.opy = h1.nx;
+    a.opz = uint64_t(h1.nx) * h1.ny;
+    a.sz = tu.sz[1];
+    a.emit = kEmitStore;
+    a.q = q;
+    a.inv = inv;
+    launch_inv_level<T>(a, s);
+}
+
+template <class T>
+void inverse_finest_requant(const T* coeff, const T* t0, Dims full, unsigned lxy, const Level0Emit<T>& emit,
+                            hipStream_t s) {
+    const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
+    const Dims h1 = shrink(full, true, true);
+    launch_finest<T>(coeff, t0, h1.nx, h1.nx * h1.ny, nullptr, full, tu.sz[0], &emit, s, "inverse_L0");
+}
+
 template void transform_forward<float>(const float*, float*, float*, float*, Dims, unsigned, unsigned, bool,
                                        uint32_t*, hipStream_t);
 template void transform_forward<double>(const double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
@@ -524,5 +641,11 @@ template bool transform_inverse<float>(float*, float*, float*, float*, Dims, uns
                                        const Level0Emit<float>*);
 template bool transform_inverse<double>(double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
                                         hipStream_t, const Level0Emit<double>*);
+template void inverse_coarse_requant<float>(const float*, float*, float*, Dims, unsigned, double, hipStream_t);
+template void inverse_coarse_requant<double>(const double*, double*, double*, Dims, unsigned, double, hipStream_t);
+template void inverse_finest_requant<float>(const float*, const float*, Dims, unsigned, const Level0Emit<float>&,
+                                            hipStream_t);
+template void inverse_finest_requant<double>(const double*, const double*, Dims, unsigned, const Level0Emit<double>&,
+                                             hipStream_t);
 
 }  // namespace wz

@@ -75,7 +75,7 @@ void wzp_encode(const uint16_t* mag, const uint32_t* sign, uint32_t n, uint8_t* 
 template <class T>
 void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_t* out, uint64_t capacity,
                            WzpScratch w, unsigned long long* total, uint32_t* status, hipStream_t s,
-                           bool coarse_box_only, const SideStream* pack) {
+                           WriteBack wb, const SideStream* pack) {
     if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
     const uint32_t n = d.nx * d.ny * d.nz;
     const WzpShape ws = wzp_shape(n);
@@ -85,9 +85,9 @@ void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_
                              T(quantum),
                              T(1.0 / quantum),
                              qo,
-                             coarse_box_only ? half_len(d.nx) : d.nx,
-                             coarse_box_only ? half_len(d.ny) : d.ny,
-                             coarse_box_only ? half_len(d.nz) : d.nz};
+                             wb == kWriteNone ? 0u : wb == kWriteCoarseBox ? half_len(d.nx) : d.nx,
+                             wb == kWriteNone ? 0u : wb == kWriteCoarseBox ? half_len(d.ny) : d.ny,
+                             wb == kWriteNone ? 0u : wb == kWriteCoarseBox ? half_len(d.nz) : d.nz};
     hipLaunchKernelGGL(wzp_encode_kernel<BrickSource<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, src, n, ws.groups,
                        out, capacity, w, status);
     launch_pack(out, capacity, ws, w, total, status, s, pack);
@@ -108,34 +108,94 @@ __global__ void __launch_bounds__(kDirThreads) wzp_directory_kernel(const uint8_
     check_directory(in, bytes, n, chunks, groups, len, sh, status);
 }
 
-// The stream's directory check runs beside its chunk decoders (both only
-// read the stream and report into `status`). bytes >= the stream header.
-void wzp_check_directory(const uint8_t* in, uint64_t bytes, uint32_t n, uint32_t* status, hipStream_t s,
-                         const SideStream* side) {
+// Planned decode: the stream size and quantum come from the device record.
+__global__ void __launch_bounds__(kDirThreads) wzp_directory_planned_kernel(const uint8_t* __restrict__ in,
+                                                                           const DecodeRecord* __restrict__ rec,
+                                                                           uint32_t n, uint32_t chunks,
+                                                                           uint32_t groups,
+                                                                           uint32_t* __restrict__ status) {
+    __shared__ uint32_t len[kMaxGroups];
+    __shared__ uint32_t sh[kDirThreads / 64];
+    if (!rec->ok) return;
+    check_directory(in, rec->wzp, n, chunks, groups, len, sh, status);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) wzp_decode_pairs_planned_kernel(const uint8_t* __restrict__ in,
+                                                                      const DecodeRecord* __restrict__ rec,
+                                                                      uint32_t n, uint32_t chunks, uint32_t groups,
+                                                                      T* __restrict__ coeff, uint32_t nx, uint32_t ny,
+                                                                      uint32_t* __restrict__ status) {
+    if (!rec->ok) return;
+    decode_chunk(in, rec->wzp, n, chunks, groups, blockIdx.x, BrickSink<T>{coeff, nx, ny, T(rec->quantum)}, status);
+}
+
+__global__ void __launch_bounds__(256) wzp_decode_mag_planned_kernel(const uint8_t* __restrict__ in,
+                                                                    const DecodeRecord* __restrict__ rec, uint32_t n,
+                                                                    uint32_t chunks, uint32_t groups, MagSink sink,
+                                                                    uint32_t* __restrict__ status) {
+    if (!rec->ok) return;
+    decode_chunk(in, rec->wzp, n, chunks, groups, blockIdx.x, sink, status);
+}
+
+// A stream's chunk decoders, enqueued on s first (the device starts them as
+// soon as the work before them ends), and its directory check, on the side
+// stream when given (both only read the stream and report into the status).
+template <class Decoders, class Directory>
+void stream_decode(uint32_t n, hipStream_t s, const SideStream* side, Decoders decoders, Directory directory) {
     const WzpShape ws = wzp_shape(n);
-    if (ws.chunks == 0) {
-        if (side) hip_check(hipEventRecord(side->done, s), "event record");
-        return;
-    }
-    if (bytes < ws.header) throw std::runtime_error("WZP archive size is outside its valid bounds");
+    if (side) hip_check(hipEventRecord(side->ready, s), "event record");
+    if (ws.chunks) decoders(ws);
     hipStream_t ds = s;
     if (side) {
-        hip_check(hipEventRecord(side->ready, s), "event record");
         hip_check(hipStreamWaitEvent(side->stream, side->ready, 0), "stream wait");
         ds = side->stream;
     }
-    hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, bytes, n, ws.chunks, ws.groups,
-                       status);
+    if (ws.chunks) directory(ws, ds);
     if (side) hip_check(hipEventRecord(side->done, ds), "event record");
+}
+
+template <class Launch>
+void planned_stream_decode(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint32_t* status, hipStream_t s,
+                           const SideStream* side, Launch decoders) {
+    stream_decode(n, s, side, decoders, [&](const WzpShape& ws, hipStream_t ds) {
+        hipLaunchKernelGGL(wzp_directory_planned_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, rec, n, ws.chunks,
+                           ws.groups, status);
+    });
+}
+
+template <class T>
+void decode_dequantize_pairs_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, T* coeff, uint32_t* status,
+                                     hipStream_t s, const SideStream* side) {
+    if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
+    const uint32_t n = d.nx * d.ny * d.nz;
+    planned_stream_decode(in, rec, n, status, s, side, [&](const WzpShape& ws) {
+        hipLaunchKernelGGL(wzp_decode_pairs_planned_kernel<T>, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n,
+                           ws.chunks, ws.groups, coeff, d.nx, d.ny, status);
+    });
+}
+
+void wzp_decode_planned(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint16_t* mag, uint32_t* sign,
+                        uint32_t* status, hipStream_t s, const SideStream* side) {
+    planned_stream_decode(in, rec, n, status, s, side, [&](const WzpShape& ws) {
+        hipLaunchKernelGGL(wzp_decode_mag_planned_kernel, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n,
+                           ws.chunks, ws.groups, MagSink{mag, sign}, status);
+    });
 }
 
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
                 hipStream_t s, const SideStream* side) {
-    const WzpShape ws = wzp_shape(n);
-    wzp_check_directory(in, bytes, n, status, s, side);
-    if (ws.chunks == 0) return;
-    hipLaunchKernelGGL(wzp_decode_kernel<MagSink>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n, ws.chunks,
-                       ws.groups, MagSink{mag, sign}, status);
+    if (n && bytes < wzp_shape(n).header) throw std::runtime_error("WZP archive size is outside its valid bounds");
+    stream_decode(
+        n, s, side,
+        [&](const WzpShape& ws) {
+            hipLaunchKernelGGL(wzp_decode_kernel<MagSink>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n,
+                               ws.chunks, ws.groups, MagSink{mag, sign}, status);
+        },
+        [&](const WzpShape& ws, hipStream_t ds) {
+            hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, bytes, n, ws.chunks,
+                               ws.groups, status);
+        });
 }
 
 template <class T>
@@ -143,19 +203,31 @@ void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double q
                              hipStream_t s, const SideStream* side) {
     if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
     const uint32_t n = d.nx * d.ny * d.nz;
-    const WzpShape ws = wzp_shape(n);
-    wzp_check_directory(in, bytes, n, status, s, side);
-    hipLaunchKernelGGL(wzp_decode_kernel<BrickSink<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n,
-                       ws.chunks, ws.groups, BrickSink<T>{coeff, d.nx, d.ny, T(quantum)}, status);
+    if (bytes < wzp_shape(n).header) throw std::runtime_error("WZP archive size is outside its valid bounds");
+    stream_decode(
+        n, s, side,
+        [&](const WzpShape& ws) {
+            hipLaunchKernelGGL(wzp_decode_kernel<BrickSink<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n,
+                               ws.chunks, ws.groups, BrickSink<T>{coeff, d.nx, d.ny, T(quantum)}, status);
+        },
+        [&](const WzpShape& ws, hipStream_t ds) {
+            hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, bytes, n, ws.chunks,
+                               ws.groups, status);
+        });
 }
 
 template void quantize_encode_pairs<float>(float*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
-                                           unsigned long long*, uint32_t*, hipStream_t, bool, const SideStream*);
+                                           unsigned long long*, uint32_t*, hipStream_t, WriteBack, const SideStream*);
 template void quantize_encode_pairs<double>(double*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
-                                            unsigned long long*, uint32_t*, hipStream_t, bool, const SideStream*);
+                                            unsigned long long*, uint32_t*, hipStream_t, WriteBack,
+                                            const SideStream*);
 template void decode_dequantize_pairs<float>(const uint8_t*, uint64_t, Dims, double, float*, uint32_t*,
                                              hipStream_t, const SideStream*);
 template void decode_dequantize_pairs<double>(const uint8_t*, uint64_t, Dims, double, double*, uint32_t*,
                                               hipStream_t, const SideStream*);
+template void decode_dequantize_pairs_planned<float>(const uint8_t*, const DecodeRecord*, Dims, float*, uint32_t*,
+                                                     hipStream_t, const SideStream*);
+template void decode_dequantize_pairs_planned<double>(const uint8_t*, const DecodeRecord*, Dims, double*, uint32_t*,
+                                                      hipStream_t, const SideStream*);
 
 }  // namespace wz

@@ -197,23 +197,31 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
     const uint32_t mbytes = code == 0 ? 0u : raw ? raw_bytes : code;
     const uint32_t sbytes = code == 0 ? 0u : sc ? sc : kSignSlot;
     if (t < 16) esel[t] = expansion_selector(t);
-    if (t == 0) {
-        const uint32_t g = c / kGroup;
-        uint64_t off = load32(in + uint64_t(g) * 4);
-        uint32_t ok = 1;
-        for (uint32_t j = g * kGroup; j < c; ++j) {
+    if (t < 64) {
+        // the chunk's offset: its group's offset plus the sizes of the
+        // group's earlier chunks, one meta per lane of the first wave (all
+        // loads in flight at once), summed across the 16 lanes
+        const uint32_t g = c / kGroup, j = g * kGroup + t;
+        const uint32_t goff = t == 0 ? load32(in + uint64_t(g) * 4) : 0u;
+        uint32_t size = 0;
+        if (t < kGroup && j < c) {
             const uint32_t mj = load24(meta + uint64_t(j) * 3);
             const uint32_t cj = mj & kRawCode, sj = mj >> 14;
-            if (!cj) continue;
-            off += (cj == kRawCode ? 2 * chunk_elements(n, j) : cj) + (sj ? sj : kSignSlot);
+            if (cj) size = (cj == kRawCode ? 2 * chunk_elements(n, j) : cj) + (sj ? sj : kSignSlot);
         }
-        if (code == 0 && sc != 0) ok = 0;
-        if (code && !raw && (code < 4 || code + 2 >= raw_bytes)) ok = 0;
-        if (code && sbytes > (E + 7) / 8) ok = 0;
-        if (header + off + mbytes + sbytes > bytes) ok = 0;
-        info[0] = uint32_t(off);
-        info[1] = ok;
-        if (!ok) atomicOr(status, uint32_t(kCorrupt));
+#pragma unroll
+        for (int o = 1; o < int(kGroup); o <<= 1) size += __shfl_xor(size, o);
+        if (t == 0) {
+            const uint64_t off = uint64_t(goff) + size;
+            uint32_t ok = 1;
+            if (code == 0 && sc != 0) ok = 0;
+            if (code && !raw && (code < 4 || code + 2 >= raw_bytes)) ok = 0;
+            if (code && sbytes > (E + 7) / 8) ok = 0;
+            if (header + off + mbytes + sbytes > bytes) ok = 0;
+            info[0] = uint32_t(off);
+            info[1] = ok;
+            if (!ok) atomicOr(status, uint32_t(kCorrupt));
+        }
     }
     __syncthreads();
     if (!info[1]) return;
