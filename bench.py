@@ -263,10 +263,10 @@ TRAFFIC_FILE = ROOT / "profiles" / "codec_512cube_f32_rel1e-3_traffic.md"
 KERNELS_FILE = ROOT / "profiles" / "codec_512cube_f32_rel1e-3_kernels.md"
 # the kernel each timed stage launches (as rocprofv3 names it)
 STAGE_KERNEL = {
-    "inverse_L0_repairs": "wz::inv_level_kernel<float, 2, true, true>",
+    "inverse_L0_repairs": "wz::inv_level_kernel<float, 2, true, true, false>",
     "inverse_L0": "wz::inv_level_kernel<float, 1, false, true, true>",
     "forward_L0": "wz::fwd_level_kernel<float>",
-    "decode_dequantize": "wz::wzp_decode_kernel<wz::BrickSink<float> >",
+    "decode_dequantize": "wz::wzp_decode_pairs_planned_kernel<float>",
     "quantize_encode": "wz::wzp_encode_kernel<wz::BrickSource<float> >",
 }
 

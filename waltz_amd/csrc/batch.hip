@@ -237,6 +237,10 @@ __global__ void __launch_bounds__(256) batch_wzp_pairs_kernel(const uint8_t* __r
 // Rank table sets of a plan: per distinct (geometry, z band) the packed rank
 // of every local index of the eight brick edge shapes (quantize.hip's tables).
 constexpr uint64_t kRankSet = 2ull * 8 * 8192;  // u16 entries per set (rank + inverse)
+// Bricks per workgroup of the separate (de)quantization passes: the bricks
+// of small z bands hold a few hundred samples, and a workgroup per brick
+// spent most of its life finding its job and brick
+constexpr uint32_t kBricksPerBlock = 8;
 
 struct SetRank {
     const uint16_t* set;
@@ -246,6 +250,9 @@ struct SetRank {
 };
 
 // Dequantization of one brick (quantize.hip's brick order, packed base and ranks).
+// Dequantization of up to kBricksPerBlock consecutive bricks of one job
+// (quantize.hip's brick order, packed base and ranks), their samples taken
+// as one range so that every burst is full.
 template <class T>
 __global__ void __launch_bounds__(256) batch_dequant_kernel(const JobShape* __restrict__ shape,
                                                             const JobRun* __restrict__ run,
@@ -253,33 +260,57 @@ __global__ void __launch_bounds__(256) batch_dequant_kernel(const JobShape* __re
                                                             const uint16_t* __restrict__ ranks,
                                                             const uint16_t* __restrict__ mag,
                                                             const uint32_t* __restrict__ sign, T* __restrict__ home) {
+    __shared__ Brick kb[kBricksPerBlock];
+    __shared__ uint32_t kstart[kBricksPerBlock + 1];  // sample offsets of the bricks in the range
     const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
     const JobRun& r = run[j];
     if (r.state != kJobOk) return;
     const JobShape& sh = shape[j];
     const BrickGeometry g{sh.d, sh.bz, sh.nbx, sh.nby, sh.nbz};
-    const Brick k = brick_of(g, blockIdx.x - brick_pfx[j]);
+    const uint32_t b0 = (blockIdx.x - brick_pfx[j]) * kBricksPerBlock;
+    const uint32_t cnt = min(kBricksPerBlock, sh.nbx * sh.nby * sh.nbz - b0);
+    if (threadIdx.x < cnt) kb[threadIdx.x] = brick_of(g, b0 + threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t at = 0;
+        for (uint32_t b = 0; b < cnt; ++b) {
+            kstart[b] = at;
+            at += kb[b].bx * kb[b].by * kb[b].bz;
+        }
+        for (uint32_t b = cnt; b <= kBricksPerBlock; ++b) kstart[b] = at;
+    }
+    __syncthreads();
+    uint32_t ks[kBricksPerBlock + 1];
+#pragma unroll
+    for (uint32_t b = 0; b <= kBricksPerBlock; ++b) ks[b] = kstart[b];
+    const uint32_t total = ks[kBricksPerBlock];
     const T q = T(r.quantum);
     const uint16_t* m = mag + sh.mag;
     const uint32_t* sg = sign + sh.sign;
-    const uint16_t* rank = ranks + uint64_t(sh.rank) * kRankSet + k.shape * 8192;
+    const uint16_t* rset = ranks + uint64_t(sh.rank) * kRankSet;
     T* c = home + sh.bank;
-    const uint32_t vol = k.bx * k.by * k.bz;
     constexpr uint32_t kBurst = 8;  // all loads of a burst issued before its stores
-    for (uint32_t i0 = 0; i0 < vol; i0 += kBurst * 256u) {
-        uint32_t mv[kBurst], sv[kBurst], pv[kBurst];
+    for (uint32_t i0 = 0; i0 < total; i0 += kBurst * 256u) {
+        uint32_t mv[kBurst], sv[kBurst], pv[kBurst], bi[kBurst], li[kBurst];
 #pragma unroll
         for (uint32_t u = 0; u < kBurst; ++u) {
-            const uint32_t i = i0 + u * 256u + threadIdx.x;
-            const uint64_t p = k.base + rank[i < vol ? i : 0u];
+            const uint32_t s = min(i0 + u * 256u + threadIdx.x, total - 1);
+            uint32_t b = 0;  // the brick holding sample s of the range
+#pragma unroll
+            for (uint32_t e = 1; e < kBricksPerBlock; ++e) b += s >= ks[e] ? 1u : 0u;
+            bi[u] = b;
+            li[u] = s - ks[b];
+            const Brick& k = kb[b];
+            const uint64_t p = k.base + rset[k.shape * 8192 + li[u]];
             pv[u] = uint32_t(p & 31u);
             mv[u] = m[p];
             sv[u] = sg[p >> 5];
         }
 #pragma unroll
         for (uint32_t u = 0; u < kBurst; ++u) {
-            const uint32_t i = i0 + u * 256u + threadIdx.x;
-            if (i >= vol) continue;
+            if (i0 + u * 256u + threadIdx.x >= total) continue;
+            const Brick& k = kb[bi[u]];
+            const uint32_t i = li[u];
             const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
             const T v = T(mv[u]) * q;  // cpu/src/codec.cpp:146-147
             c[(uint64_t(k.z0 + z) * sh.d.ny + (k.y0 + y)) * sh.d.nx + k.x0 + x] = ((sv[u] >> pv[u]) & 1u) ? -v : v;
@@ -584,10 +615,16 @@ __global__ void __launch_bounds__(256) batch_quantize_kernel(const JobShape* __r
     const JobComp& jc = comp[j];
     const BrickGeometry g{sh.d, sh.bz, sh.nbx, sh.nby, sh.nbz};
     const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
-    quantize_brick(static_cast<T*>(b.home) + sh.bank, g, T(quantum), T(1.0 / quantum),
-                   SetRank{ranks + uint64_t(sh.rank) * kRankSet},
-                   static_cast<uint16_t*>(b.tmp0) + sh.mag, static_cast<uint32_t*>(b.tmp1) + sh.sign, qo,
-                   &b.res[j].status, blockIdx.x - brick_pfx[j]);
+    // up to kBricksPerBlock consecutive bricks of the job, one after another
+    const uint32_t b0 = (blockIdx.x - brick_pfx[j]) * kBricksPerBlock;
+    const uint32_t b1 = min(b0 + kBricksPerBlock, sh.nbx * sh.nby * sh.nbz);
+    for (uint32_t k = b0; k < b1; ++k) {
+        if (k != b0) __syncthreads();  // the previous brick's sign words are out of LDS
+        quantize_brick(static_cast<T*>(b.home) + sh.bank, g, T(quantum), T(1.0 / quantum),
+                       SetRank{ranks + uint64_t(sh.rank) * kRankSet},
+                       static_cast<uint16_t*>(b.tmp0) + sh.mag, static_cast<uint32_t*>(b.tmp1) + sh.sign, qo,
+                       &b.res[j].status, k);
+    }
 }
 
 WZ_D WzpScratch job_wzp(const CompBufs& b, const JobComp& jc) {
@@ -1331,7 +1368,7 @@ void finish_plan(HostPlan& p) {
             pchunks += s.chunks;
         } else {
             mchunks += s.chunks;
-            bricks += uint64_t(s.nbx) * s.nby * s.nbz;
+            bricks += (uint64_t(s.nbx) * s.nby * s.nbz + kBricksPerBlock - 1) / kBricksPerBlock;  // brick blocks
         }
         sched[j] = inverse_schedule(s, p.width);
         steps = std::max(steps, sched[j].size());

@@ -505,6 +505,7 @@ uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Ge
         StageClock ck("decode_dequantize", s);
         decode_dequantize_pairs_planned<T>(archive + 32, rec, g.d, coeff, status, s, side);
         if (side) {
+            StageClock ck("patch_keys", side->stream);
             repair_keys_planned<T>(archive, rec, g.d, table, status, side->stream);
             hip_check(hipEventRecord(side->mid, side->stream), "event record");
             keys_on_side = true;
@@ -518,7 +519,10 @@ uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Ge
         StageClock ck("dequantize", s);
         dequantize_bricks<T>(mag, sign, brick_geometry(g.d, g.bz), 0.0, rank, coeff, s, rec);
     }
-    if (!keys_on_side) repair_keys_planned<T>(archive, rec, g.d, table, status, s);
+    if (!keys_on_side) {
+        StageClock ck("patch_keys", s);
+        repair_keys_planned<T>(archive, rec, g.d, table, status, s);
+    }
     {
         StageClock ck("qoutliers", s);
         apply_qoutliers_planned<T>(coeff, archive, rec, n, status, s);

@@ -477,9 +477,11 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     // 64-sample tile. Lane 8r proves the entry current (the entry before it
     // lies below the tile: a stale entry fails this or the next test), lanes
     // 8r+1 .. 8r+7 hold the tile's first repairs. The table entries of the
-    // next step are scalar loads (wave-uniform, outside the vector memory
-    // counter); the window loads as its step begins, ahead of the
-    // coefficient loads, so waiting for it never waits for those.
+    // next step load one step ahead (lane r holds row r's, spread to its
+    // lanes by a cross-lane permute; measured faster than scalar loads, whose
+    // waits share a counter with the LDS traffic), and the window loads as its
+    // step begins, ahead of the coefficient loads, so waiting for it never
+    // waits for those.
     const uint32_t pl = t & 63u;
     const uint32_t txn = (n.nx + TX - 1) / TX, tw = min(uint32_t(TX), n.nx - x0);
     uint32_t np = a.np;  // the list (planned decode: from the device record)
@@ -494,19 +496,12 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     }
     const auto pidx = window_n(plist, Patch ? 4u * np : 0u);
     const auto perr = window_n(perrs, Patch ? uint32_t(sizeof(T)) * np : 0u);
-    using TableWord = __attribute__((address_space(4))) const uint32_t;  // read-only in this kernel
-    const auto* ptab = reinterpret_cast<TableWord*>(reinterpret_cast<uintptr_t>(a.ptable));
-    uint32_t tnext[8] = {};  // table entries of the next step's rows (0xFFFFFFFF: none)
-    auto load_table = [&](int32_t g) {
-        const uint32_t Z = 2 * uint32_t(g - 2) + yp;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const uint32_t Y = y0 + 8 * yt + r;
-            const bool in = g >= 2 && Z < n.nz && Y < n.ny;
-            const uint32_t v = ptab[in ? (Z * n.ny + Y) * txn + tl.bx : 0u];
-            tnext[r] = in ? v : 0xFFFFFFFFu;
-        }
+    const auto ptab = window_n(a.ptable, Patch ? 4u * a.keys : 0u);
+    auto tab_off = [&](int32_t g) {
+        const uint32_t Z = 2 * uint32_t(g - 2) + yp, Y = y0 + 8 * yt + pl;
+        return pl < 8 && g >= 2 && Z < n.nz && Y < n.ny ? 4u * ((Z * n.ny + Y) * txn + tl.bx) : kOutside;
     };
+    uint32_t ptv = 0;           // lanes 0..7: row r's table entry of the next step
     uint32_t pfv = 0, pci = 0;  // first_r of the lane's row, the lane's entry index
     T pce = T(0);               // and its repair
     auto lane_off = [&](uint32_t Y) { return xin && Y < n.ny ? (Y * a.opy + X) * uint32_t(sizeof(T)) : kOutside; };
@@ -551,16 +546,13 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
         const bool emit = emit_c, yw = y_c;  // compile-time for float, run-time for double
         if constexpr (Patch) {
             if (emit) {
-                const uint32_t r = pl >> 3;
-                pfv = tnext[0];
-#pragma unroll
-                for (int k = 1; k < 8; ++k) pfv = r == uint32_t(k) ? tnext[k] : pfv;
+                pfv = uint32_t(__builtin_amdgcn_ds_bpermute(int((pl >> 3) << 2), int(ptv)));
                 const uint32_t pos = pfv - 1u + (pl & 7u);
                 const bool held = pos < np;
                 pci = __builtin_amdgcn_raw_buffer_load_b32(pidx, int(held ? 4u * pos : kOutside), 0, 0);
                 pce = buf_load<T>(perr, held ? uint32_t(sizeof(T)) * pos : kOutside);
             }
-            load_table(g + 1);
+            ptv = __builtin_amdgcn_raw_buffer_load_b32(ptab, int(tab_off(g + 1)), 0, 0);
         }
         if constexpr (early) {
             if (emit) fetch_orig(g, ref);
