@@ -99,6 +99,22 @@ WZ_D uint32_t job_of(const uint32_t* __restrict__ pfx, uint32_t jobs, uint32_t b
     return lo;
 }
 
+// The same for a block index the whole workgroup shares (every lane of each
+// wave calls it): two rounds of 64 parallel reads and a ballot instead of a
+// chain of dependent reads, for up to 4096 jobs.
+WZ_D uint32_t job_of_wave(const uint32_t* __restrict__ pfx, uint32_t jobs, uint32_t b) {
+    if (jobs <= 1) return 0;
+    if (jobs > 4096) return job_of(pfx, jobs, b);
+    const uint32_t lane = __lane_id();
+    const uint32_t step = (jobs + 63) / 64;
+    const uint32_t p1 = lane * step;  // the segment [p1, p1 + step) of lane l
+    const unsigned long long m1 = __ballot(p1 < jobs && pfx[p1 < jobs ? p1 : 0u] <= b);
+    const uint32_t seg = 63u - uint32_t(__builtin_clzll(m1));  // lane 0 always holds (pfx[0] = 0)
+    const uint32_t p2 = seg * step + lane;
+    const unsigned long long m2 = __ballot(lane < step && p2 < jobs && pfx[p2 < jobs ? p2 : 0u] <= b);
+    return seg * step + (63u - uint32_t(__builtin_clzll(m2)));
+}
+
 WZ_D uint32_t rd32(const uint8_t* p) {
     return uint32_t(p[0]) | (uint32_t(p[1]) << 8) | (uint32_t(p[2]) << 16) | (uint32_t(p[3]) << 24);
 }
@@ -210,7 +226,7 @@ __global__ void __launch_bounds__(256) batch_wzp_kernel(const uint8_t* __restric
                                                         const JobRun* __restrict__ run,
                                                         const uint32_t* __restrict__ mag_pfx, uint32_t jobs,
                                                         uint16_t* mag, uint32_t* sign, uint32_t* status) {
-    const uint32_t j = job_of(mag_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(mag_pfx, jobs, blockIdx.x);
     const JobRun& r = run[j];
     if (r.state != kJobOk) return;
     const JobShape& sh = shape[j];
@@ -226,7 +242,7 @@ __global__ void __launch_bounds__(256) batch_wzp_pairs_kernel(const uint8_t* __r
                                                               const JobRun* __restrict__ run,
                                                               const uint32_t* __restrict__ pair_pfx, uint32_t jobs,
                                                               T* __restrict__ home, uint32_t* status) {
-    const uint32_t j = job_of(pair_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(pair_pfx, jobs, blockIdx.x);
     const JobRun& r = run[j];
     if (r.state != kJobOk) return;
     const JobShape& sh = shape[j];
@@ -262,7 +278,7 @@ __global__ void __launch_bounds__(256) batch_dequant_kernel(const JobShape* __re
                                                             const uint32_t* __restrict__ sign, T* __restrict__ home) {
     __shared__ Brick kb[kBricksPerBlock];
     __shared__ uint32_t kstart[kBricksPerBlock + 1];  // sample offsets of the bricks in the range
-    const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(brick_pfx, jobs, blockIdx.x);
     const JobRun& r = run[j];
     if (r.state != kJobOk) return;
     const JobShape& sh = shape[j];
@@ -311,7 +327,8 @@ __global__ void __launch_bounds__(256) batch_dequant_kernel(const JobShape* __re
             if (i0 + u * 256u + threadIdx.x >= total) continue;
             const Brick& k = kb[bi[u]];
             const uint32_t i = li[u];
-            const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
+            uint32_t x, y, z;
+            brick_xyz(k, i, x, y, z);
             const T v = T(mv[u]) * q;  // cpu/src/codec.cpp:146-147
             c[(uint64_t(k.z0 + z) * sh.d.ny + (k.y0 + y)) * sh.d.nx + k.x0 + x] = ((sv[u] >> pv[u]) & 1u) ? -v : v;
         }
@@ -421,7 +438,7 @@ __global__ void __launch_bounds__(256) batch_pass_kernel(const JobShape* __restr
                                                          const uint32_t* __restrict__ pfx, uint32_t jobs, Banks bk,
                                                          T fill, uint32_t* status) {
     constexpr int ZTile = sizeof(T) == 4 ? 16 : 8;  // canonical inverse Z tile
-    const uint32_t j = job_of(pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(pfx, jobs, blockIdx.x);
     const Pass ps = pass[j];
     const uint32_t i = (blockIdx.x - pfx[j]) * 256u + threadIdx.x;
     const uint32_t state = run[j].state;
@@ -450,30 +467,40 @@ __global__ void __launch_bounds__(256) batch_pass_kernel(const JobShape* __restr
     if (__ballot(!fin) && (threadIdx.x & 63u) == 0) atomicOr(status, uint32_t(kNonFiniteOutput));
 }
 
+constexpr uint32_t kRecords = 4;         // consecutive repair records per thread and round
+constexpr uint32_t kRpatchBlocks = 2048;  // fixed grid (the record count is known on the device only)
+
 template <class T>
 __global__ void __launch_bounds__(256) batch_rpatch_kernel(const uint8_t* __restrict__ packed,
                                                            const JobShape* __restrict__ shape,
                                                            const JobRun* __restrict__ run,
                                                            const uint32_t* __restrict__ np_pfx, uint32_t jobs,
                                                            T* __restrict__ out, uint32_t* status) {
+    // each thread takes kRecords consecutive records (of one job or a few)
+    // per round: one job search, then a walk to the next job
     const uint32_t total = np_pfx[jobs];
-    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < total; k += gridDim.x * 256u) {
-        const uint32_t j = job_of(np_pfx, jobs, k), e = k - np_pfx[j];
-        const JobRun& r = run[j];
-        const JobShape& sh = shape[j];
-        const uint8_t* a = packed + r.arc;
-        const uint32_t i = reinterpret_cast<const uint32_t*>(a + r.pi)[e];
-        const T err = reinterpret_cast<const T*>(a + r.pv)[e];
-        if (i >= sh.n || !__builtin_isfinite(err)) {
-            atomicOr(status, uint32_t(kCorrupt));
-            continue;
+    for (uint32_t k0 = (blockIdx.x * 256u + threadIdx.x) * kRecords; k0 < total;
+         k0 += gridDim.x * 256u * kRecords) {
+        uint32_t j = job_of(np_pfx, jobs, k0);
+        for (uint32_t k = k0; k < min(k0 + kRecords, total); ++k) {
+            while (k >= np_pfx[j + 1]) ++j;  // jobs without repairs are skipped
+            const uint32_t e = k - np_pfx[j];
+            const JobRun& r = run[j];
+            const JobShape& sh = shape[j];
+            const uint8_t* a = packed + r.arc;
+            const uint32_t i = reinterpret_cast<const uint32_t*>(a + r.pi)[e];
+            const T err = reinterpret_cast<const T*>(a + r.pv)[e];
+            if (i >= sh.n || !__builtin_isfinite(err)) {
+                atomicOr(status, uint32_t(kCorrupt));
+                continue;
+            }
+            const uint32_t x = i % sh.d.nx, y = (i / sh.d.nx) % sh.d.ny, z = i / (sh.d.nx * sh.d.ny);
+            if (x >= sh.clip.nx || y >= sh.clip.ny || z >= sh.clip.nz) continue;
+            T* o = out + sh.out + uint64_t(z) * sh.opz + uint64_t(y) * sh.opy + x;
+            const T v = *o - err;  // cpu/src/codec.cpp:330
+            *o = v;
+            if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
         }
-        const uint32_t x = i % sh.d.nx, y = (i / sh.d.nx) % sh.d.ny, z = i / (sh.d.nx * sh.d.ny);
-        if (x >= sh.clip.nx || y >= sh.clip.ny || z >= sh.clip.nz) continue;
-        T* o = out + sh.out + uint64_t(z) * sh.opz + uint64_t(y) * sh.opy + x;
-        const T v = *o - err;  // cpu/src/codec.cpp:330
-        *o = v;
-        if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
     }
 }
 
@@ -574,7 +601,7 @@ __global__ void __launch_bounds__(256) batch_fwd_pass_kernel(const JobShape* __r
                                                              const uint32_t* __restrict__ pfx, uint32_t jobs,
                                                              CompBufs b) {
     constexpr int ZTile = sizeof(T) == 4 ? 16 : 8;
-    const uint32_t j = job_of(pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(pfx, jobs, blockIdx.x);
     const Pass ps = pass[j];
     const uint32_t i = (blockIdx.x - pfx[j]) * 256u + threadIdx.x;
     if (ps.kind == kPassNone) return;
@@ -610,7 +637,7 @@ __global__ void __launch_bounds__(256) batch_quantize_kernel(const JobShape* __r
                                                              const uint32_t* __restrict__ brick_pfx, uint32_t jobs,
                                                              const uint16_t* __restrict__ ranks, double quantum,
                                                              CompBufs b) {
-    const uint32_t j = job_of(brick_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(brick_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     const BrickGeometry g{sh.d, sh.bz, sh.nbx, sh.nby, sh.nbz};
@@ -637,7 +664,7 @@ __global__ void __launch_bounds__(256) batch_encode_kernel(const JobShape* __res
                                                            const JobComp* __restrict__ comp,
                                                            const uint32_t* __restrict__ mag_pfx, uint32_t jobs,
                                                            CompBufs b) {
-    const uint32_t j = job_of(mag_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(mag_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     const MagSource src{static_cast<const uint16_t*>(b.tmp0) + sh.mag, static_cast<const uint32_t*>(b.tmp1) + sh.sign};
@@ -653,7 +680,7 @@ __global__ void __launch_bounds__(256) batch_encode_pairs_kernel(const JobShape*
                                                                  const JobComp* __restrict__ comp,
                                                                  const uint32_t* __restrict__ pair_pfx,
                                                                  uint32_t jobs, double quantum, CompBufs b) {
-    const uint32_t j = job_of(pair_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(pair_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
@@ -672,7 +699,7 @@ __global__ void __launch_bounds__(kPackThreads) batch_pack_kernel(const JobShape
     __shared__ uint32_t g_sh;
     if (threadIdx.x == 0) g_sh = atomicAdd(b.ticket, 1u);
     __syncthreads();
-    const uint32_t j = job_of(group_pfx, jobs, g_sh);
+    const uint32_t j = job_of_wave(group_pfx, jobs, g_sh);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     pack_group(g_sh - group_pfx[j], b.out + jc.arc + 32, jc.cap - 32, sh.chunks, sh.groups, job_wzp(b, jc),
@@ -690,7 +717,7 @@ __global__ void __launch_bounds__(256) batch_repair_kernel(const JobShape* __res
                                                            const JobComp* __restrict__ comp,
                                                            const uint32_t* __restrict__ tile_pfx, uint32_t jobs,
                                                            double bound, CompBufs b) {
-    const uint32_t j = job_of(tile_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(tile_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     repair_tile(static_cast<const T*>(b.input) + sh.out, static_cast<const T*>(b.recon) + sh.out, sh.n, bound,
@@ -725,7 +752,7 @@ __global__ void __launch_bounds__(256) batch_prefix_kernel(const JobShape* __res
                                                            const uint32_t* __restrict__ pblock_pfx, uint32_t jobs,
                                                            CompBufs b) {
     const uint32_t k = blockIdx.x >> 1, which = blockIdx.x & 1u;
-    const uint32_t j = job_of(pblock_pfx, jobs, k);
+    const uint32_t j = job_of_wave(pblock_pfx, jobs, k);
     const JobShape& sh = shape[j];
     const JobRanks r = job_ranks(b, sh, comp[j]);
     const uint32_t blk = k - pblock_pfx[j];
@@ -759,7 +786,7 @@ __global__ void __launch_bounds__(256) batch_place_kernel(const JobShape* __rest
                                                           const JobComp* __restrict__ comp,
                                                           const uint32_t* __restrict__ place_pfx, uint32_t jobs,
                                                           CompBufs b) {
-    const uint32_t j = job_of(place_pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(place_pfx, jobs, blockIdx.x);
     const JobShape& sh = shape[j];
     const JobComp& jc = comp[j];
     const EncodeResult* res = &b.res[j];
@@ -838,12 +865,13 @@ WZ_D void lds_pass(T* A, uint32_t pitch, uint32_t n, uint32_t lines) {
     const uint32_t h = half_len(n);
     for (uint32_t l0 = 0; l0 < lines; l0 += per) {
         const uint32_t gl = min(per, lines - l0), items = gl * nt;
+        const FastDiv by_gl(gl);  // w < K * threads, gl <= that: exact
         T e[K][P], o[K][P];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t w = t + uint32_t(k) * kLdsThreads;
             if (w >= items) continue;
-            const uint32_t line = l0 + w % gl, tile = w / gl;
+            const uint32_t tile = by_gl.div(w), line = l0 + (w - tile * gl);
             const uint32_t base = Ax == 0 ? line * pitch : line, stride = Ax == 0 ? 1u : pitch;
             const int32_t start = int32_t(tile) * Tl;
 #pragma unroll
@@ -858,7 +886,7 @@ WZ_D void lds_pass(T* A, uint32_t pitch, uint32_t n, uint32_t lines) {
         for (int k = 0; k < K; ++k) {
             const uint32_t w = t + uint32_t(k) * kLdsThreads;
             if (w >= items) continue;
-            const uint32_t line = l0 + w % gl, tile = w / gl;
+            const uint32_t tile = by_gl.div(w), line = l0 + (w - tile * gl);
             const uint32_t base = Ax == 0 ? line * pitch : line, stride = Ax == 0 ? 1u : pitch;
             const uint32_t start = tile * Tl;
             if constexpr (Inv) lift_inverse<T, P, Ax != 0>(e[k], o[k]);
@@ -883,7 +911,7 @@ __global__ void __launch_bounds__(kLdsThreads) batch_lds_kernel(const JobShape* 
     constexpr uint32_t Cap = PlaneCap<T, Large>::value;
     constexpr int ZTile = Inverse ? (sizeof(T) == 4 ? 16 : 8) : 8;
     __shared__ T A[Cap];
-    const uint32_t j = job_of(pfx, jobs, blockIdx.x);
+    const uint32_t j = job_of_wave(pfx, jobs, blockIdx.x);
     const Pass ps = pass[j];
     if (ps.kind != kPassXYPlane && ps.kind != kPassZCol) return;
     const uint32_t state = run ? run[j].state : uint32_t(kJobOk);
@@ -902,6 +930,7 @@ __global__ void __launch_bounds__(kLdsThreads) batch_lds_kernel(const JobShape* 
     const uint32_t rows = xy ? act.ny : act.nz;
     const uint32_t pitch = xy ? (act.nx | 1u) : kColumns;
     const uint32_t cells = cols * rows;
+    const FastDiv by_cols(cols);  // cells fit LDS: exact
     auto gat = [&](uint32_t c, uint32_t r) -> uint64_t {  // global index of staged cell (c, r)
         return xy ? uint64_t(z) * plane + uint64_t(r) * full.nx + c
                   : uint64_t(r) * plane + uint64_t(y) * full.nx + x0 + c;
@@ -914,14 +943,16 @@ __global__ void __launch_bounds__(kLdsThreads) batch_lds_kernel(const JobShape* 
 #pragma unroll
         for (uint32_t k = 0; k < kBurst; ++k) {
             const uint32_t i = i0 + k * kLdsThreads + t;
-            v[k] = i < cells ? src[gat(i % cols, i / cols)] : T(0);
+            const uint32_t r = by_cols.div(i);
+            v[k] = i < cells ? src[gat(i - r * cols, r)] : T(0);
         }
 #pragma unroll
         for (uint32_t k = 0; k < kBurst; ++k) {
             const uint32_t i = i0 + k * kLdsThreads + t;
             if (i < cells) {
                 ok = ok && (!screen || __builtin_isfinite(v[k]));
-                A[(i / cols) * pitch + i % cols] = v[k];
+                const uint32_t r = by_cols.div(i);
+                A[r * pitch + (i - r * cols)] = v[k];
             }
         }
     }
@@ -952,7 +983,7 @@ __global__ void __launch_bounds__(kLdsThreads) batch_lds_kernel(const JobShape* 
     if (ps.dst == kOutput) {
         const Put<T> put{nullptr, static_cast<T*>(b.out), &sh, state == kJobHole, fill};
         for (uint32_t i = t; i < cells; i += kLdsThreads) {
-            const uint32_t c = i % cols, r = i / cols;
+            const uint32_t r = by_cols.div(i), c = i - r * cols;
             const uint32_t X = xy ? c : x0 + c, Y = xy ? r : y, Zc = xy ? z : r;
             fin = put(gat(c, r), X, Y, Zc, A[r * pitch + c]) && fin;
         }
@@ -960,7 +991,7 @@ __global__ void __launch_bounds__(kLdsThreads) batch_lds_kernel(const JobShape* 
     } else {
         T* dst = stage_bank<T>(b, sh, ps.dst);
         for (uint32_t i = t; i < cells; i += kLdsThreads) {
-            const uint32_t c = i % cols, r = i / cols;
+            const uint32_t r = by_cols.div(i), c = i - r * cols;
             dst[gat(c, r)] = A[r * pitch + c];
         }
     }
@@ -1632,7 +1663,7 @@ struct DevicePlan {
             launch_lds<T, true>(p.lpfx[st][J], p.large[st], run, pass + st * J, lds_pfx + st * (J + 1), sb, T(fill),
                                 status, s);
         }
-        hipLaunchKernelGGL((batch_rpatch_kernel<T>), dim3(256), dim3(256), 0, s, packed, shape, run, np_pfx, J,
+        hipLaunchKernelGGL((batch_rpatch_kernel<T>), dim3(kRpatchBlocks), dim3(256), 0, s, packed, shape, run, np_pfx, J,
                            static_cast<T*>(bk.out), status);
     }
 

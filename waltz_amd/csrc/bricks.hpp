@@ -13,6 +13,21 @@ struct Brick {
     uint64_t base;
 };
 
+// Position (x, y, z) in brick k of its local row-major index i: shifts for
+// bricks full in x and y (all but the edge ones), integer division otherwise.
+WZ_D void brick_xyz(const Brick& k, uint32_t i, uint32_t& x, uint32_t& y, uint32_t& z) {
+    if (k.bx == 16 && k.by == 16) {
+        x = i & 15u;
+        y = (i >> 4) & 15u;
+        z = i >> 8;
+    } else {
+        const uint32_t r = i / k.bx;
+        x = i - r * k.bx;
+        y = r % k.by;
+        z = r / k.by;
+    }
+}
+
 WZ_D Brick brick_of(const BrickGeometry& g, uint32_t b) {
     Brick k;
     const uint32_t bxi = b % g.nbx, r = b / g.nbx, byi = r % g.nby, bzi = r / g.nby;
@@ -54,7 +69,8 @@ WZ_D void quantize_brick(T* __restrict__ coeff, const BrickGeometry& g, T q, T i
       for (uint32_t u = 0; u < kBurst; ++u) {
         const uint32_t i = i0 + u * 256u + threadIdx.x;
         const uint32_t ic = i < vol ? i : 0u;
-        const uint32_t x = ic % k.bx, y = (ic / k.bx) % k.by, z = ic / (k.bx * k.by);
+        uint32_t x, y, z;
+        brick_xyz(k, ic, x, y, z);
         av[u] = (uint64_t(k.z0 + z) * g.d.ny + (k.y0 + y)) * g.d.nx + k.x0 + x;
         rv[u] = rank(k, ic, x, y, z);
         cv[u] = coeff[av[u]];

@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(256) dequantize_kernel(const uint16_t* __restr
         for (uint32_t u = 0; u < kBurst; ++u) {
             const uint32_t i = i0 + u * 256u + threadIdx.x;
             if (i >= vol) continue;
-            const uint32_t x = i % k.bx, y = (i / k.bx) % k.by, z = i / (k.bx * k.by);
+            uint32_t x, y, z;
+            brick_xyz(k, i, x, y, z);
             const uint64_t at = (uint64_t(k.z0 + z) * g.d.ny + (k.y0 + y)) * g.d.nx + k.x0 + x;
             const T v = T(mv[u]) * q;  // cpu/src/codec.cpp:146-147
             out[at] = ((sv[u] >> pv[u]) & 1u) ? -v : v;
