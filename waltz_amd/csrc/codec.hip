@@ -205,6 +205,9 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     // layout of the per-call scratch
     // brick-pair fields quantize inside the encoder: no sign words to clear
     const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
+    // other fields of full bricks quantize inside the encoder too
+    const bool ranks = rank_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
+    const bool fusedq = pairs || ranks;
     auto plan = [&](Carve& c) {
         struct P {
             T *coeff, *t0, *t1, *rec;
@@ -232,7 +235,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.size = c.take<uint32_t>(ws.chunks);
         // zero-initialised region: sign, q bitmap, lookback descriptors, counters
         const size_t z0 = c.used;
-        p.sign = c.take<uint32_t>(pairs ? 0 : words);
+        p.sign = c.take<uint32_t>(fusedq ? 0 : words);
         p.qbm = c.take<uint32_t>(words);
         p.desc = c.take<unsigned long long>(ws.chunks);
         p.counters = c.take<uint32_t>(4);  // ticket, q count, repair total
@@ -252,9 +255,10 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     transform_forward<T>(input, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, status, s);
     SideList qo{p.qbm, p.qidx, p.qval, p.counters + 1, nq};
     WzpScratch wsc{p.desc, reinterpret_cast<uint8_t*>(p.rec), p.size, p.counters};
-    // with the fused repair emitter on brick pairs the finest level's high
-    // bands stay unquantized in coeff; that kernel requantizes them on load
-    const bool requant = fsegs && pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
+    // with the fused repair emitter and a quantizing encoder the finest
+    // level's high bands stay unquantized in coeff; that kernel requantizes
+    // them on load
+    const bool requant = fsegs && fusedq;
     // With a fused finest level the reconstruction buffer (which shares the
     // staging slots) is never written, so the group pass runs on the side
     // stream beside the inverse levels, whose coarse passes leave most of the
@@ -267,10 +271,16 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     // it as much as they hid, and the group pass then collided with the
     // finest level, which needs every CU's two workgroup slots.
     const bool split = requant && coarse_split_fits(g.d, g.lx, sizeof(T));
-    if (pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz)) {
+    const WriteBack wb = split ? kWriteNone : requant ? kWriteCoarseBox : kWriteAll;
+    if (pairs) {
         StageClock c("quantize_encode", s);
         quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,
-                                 status, s, split ? kWriteNone : requant ? kWriteCoarseBox : kWriteAll, side);
+                                 status, s, wb, side);
+    } else if (ranks) {
+        const uint16_t* rank = w.ranks(g.d, g.bz, s);
+        StageClock c("quantize_encode", s);
+        quantize_encode_ranks<T>(p.coeff, g.d, g.bz, rank, quantum, qo, archive + 32, capacity - 32, wsc,
+                                 &res->wzp_bytes, status, s, wb, side);
     } else {
         const BrickGeometry bg = brick_geometry(g.d, g.bz);
         const uint16_t* rank = w.ranks(g.d, g.bz, s);
@@ -422,13 +432,16 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
     const SideStream* side = m.cache ? &m.cache->side : nullptr;
     const SideJoin join{side};
     const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz);
-    if (pairs) {
+    const uint16_t* rank = pairs ? nullptr : m.ranks(g.d, bz, s);
+    const bool ranks = rank_bricks_fit(g.d.nx, g.d.ny, g.d.nz, bz) && reinterpret_cast<uintptr_t>(rank) % 8 == 0;
+    if (pairs || ranks) {
         StageClock ck("decode_dequantize", s);
-        decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s, side);
+        if (pairs) decode_dequantize_pairs<T>(archive + 32, wzp, g.d, quantum, coeff, status, s, side);
+        else decode_dequantize_ranks<T>(archive + 32, wzp, g.d, bz, rank, quantum, coeff, status, s, side);
         if (keys && side) {
-            // the key table reads only the archive, and the brick-pair decoder
-            // leaves the magnitude scratch unused: built on the side stream
-            // after the directory check, beside the chunk decoders
+            // the key table reads only the archive, and the dequantizing
+            // decoders leave the magnitude scratch unused: built on the side
+            // stream after the directory check, beside the chunk decoders
             repair_keys<T>(screen.pidx, screen.perr, np, g.d, screen.ptable, status, side->stream);
             hip_check(hipEventRecord(side->mid, side->stream), "event record");
             screen.ptable_built = true;
@@ -439,7 +452,6 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
             wzp_decode(archive + 32, wzp, n, mag, sign, status, s, side);
         }
         const BrickGeometry bg = brick_geometry(g.d, bz);
-        const uint16_t* rank = m.ranks(g.d, bz, s);
         StageClock ck("dequantize", s);
         dequantize_bricks<T>(mag, sign, bg, quantum, rank, coeff, s);
     }
@@ -494,30 +506,42 @@ uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Ge
         decode_plan(archive, archive_bytes,
                     PlanExpect{n, uint32_t(sizeof(T)), flags, uint8_t(g.lx), uint8_t(g.lz), uint8_t(g.bz)}, rec, s);
     }
-    // the directory check runs on the side stream beside the chunk decoders;
-    // brick pairs leave the magnitude scratch unused, so the repair key table
-    // (it reads only the archive) follows it there
+    // Beside the chunk decoders, the side stream builds the repair key table
+    // (the dequantizing decoders leave the magnitude scratch unused; it reads
+    // only the archive), then checks the WZP directory; the caller's stream
+    // waits for the table only before the finest level, after the coarse
+    // levels.
     const SideStream* side = m.cache ? &m.cache->side : nullptr;
     const SideJoin join{side};
     const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
-    bool keys_on_side = false;
-    if (pairs) {
+    const uint16_t* rank = pairs ? nullptr : m.ranks(g.d, g.bz, s);
+    const bool ranks = rank_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz) && reinterpret_cast<uintptr_t>(rank) % 8 == 0;
+    const bool keys_on_side = (pairs || ranks) && side;
+    if (side) hip_check(hipEventRecord(side->ready, s), "event record");
+    if (pairs || ranks) {
         StageClock ck("decode_dequantize", s);
-        decode_dequantize_pairs_planned<T>(archive + 32, rec, g.d, coeff, status, s, side);
-        if (side) {
-            StageClock ck("patch_keys", side->stream);
-            repair_keys_planned<T>(archive, rec, g.d, table, status, side->stream);
-            hip_check(hipEventRecord(side->mid, side->stream), "event record");
-            keys_on_side = true;
-        }
+        if (pairs) decode_dequantize_pairs_planned<T>(archive + 32, rec, g.d, coeff, status, s);
+        else decode_dequantize_ranks_planned<T>(archive + 32, rec, g.d, g.bz, rank, coeff, status, s);
     } else {
         {
             StageClock ck("wzp_decode", s);
-            wzp_decode_planned(archive + 32, rec, n, mag, sign, status, s, side);
+            wzp_decode_planned(archive + 32, rec, n, mag, sign, status, s);
         }
-        const uint16_t* rank = m.ranks(g.d, g.bz, s);
         StageClock ck("dequantize", s);
         dequantize_bricks<T>(mag, sign, brick_geometry(g.d, g.bz), 0.0, rank, coeff, s, rec);
+    }
+    if (side) {
+        hipStream_t ss = side->stream;
+        hip_check(hipStreamWaitEvent(ss, side->ready, 0), "stream wait");
+        if (keys_on_side) {
+            StageClock ck("patch_keys", ss);
+            repair_keys_planned<T>(archive, rec, g.d, table, status, ss);
+        }
+        hip_check(hipEventRecord(side->mid, ss), "event record");
+        wzp_directory_planned(archive + 32, rec, n, status, ss);
+        hip_check(hipEventRecord(side->done, ss), "event record");
+    } else {
+        wzp_directory_planned(archive + 32, rec, n, status, s);
     }
     if (!keys_on_side) {
         StageClock ck("patch_keys", s);
@@ -533,9 +557,9 @@ uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Ge
     screen.ptable_built = true;
     screen.plan = rec;
     screen.archive = archive;
+    const LowBox<T> low = inverse_coarse_levels<T>(coeff, t0, t1, g.d, g.lx, s);
     if (keys_on_side) hip_check(hipStreamWaitEvent(s, side->mid, 0), "stream wait");
-    if (!transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, true, s, &screen))
-        throw std::logic_error("planned decode without a fused finest level");
+    inverse_finest_level<T>(coeff, low, output, g.d, g.lx, screen, s);
     if (side) hip_check(hipStreamWaitEvent(s, side->done, 0), "stream wait");
     hip_check(hipMemcpyAsync(m.host, rec, sizeof(DecodeRecord), hipMemcpyDeviceToHost, s), "status copy");
     hip_check(hipStreamSynchronize(s), "decompress");

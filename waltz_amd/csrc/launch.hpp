@@ -56,6 +56,22 @@ uint64_t patch_table_words(Dims full, uint32_t np, unsigned width);
 template <class T>
 bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
                        hipStream_t s, const Level0Emit<T>* emit = nullptr);
+// transform_inverse of a dyadic field with a fused finest level, in two
+// parts, so that a caller can order other work between them: the levels
+// above the finest one (they read and write coeff inside the level-1 box
+// [0, ceil(nx/2)) x [0, ceil(ny/2)) x [0, ceil(nz/2)) only, and tmp0/tmp1),
+// which return where their reconstruction of that box lies; then the finest
+// level into `output` (not coeff) with `emit`.
+template <class T>
+struct LowBox {
+    const T* p;
+    uint32_t py, pz;  // row and plane pitch (elements)
+};
+template <class T>
+LowBox<T> inverse_coarse_levels(T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, hipStream_t s);
+template <class T>
+void inverse_finest_level(const T* coeff, const LowBox<T>& low, T* output, Dims full, unsigned lxy,
+                          const Level0Emit<T>& emit, hipStream_t s);
 // Transpose of transform_forward (of_inverse = false) or transform_inverse
 // (true) with the same geometry, in place on `data`; tmp0/tmp1 as above.
 template <class T>
@@ -172,12 +188,29 @@ template <class T>
 void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double quantum, T* coeff, uint32_t* status,
                              hipStream_t s, const SideStream* side = nullptr);
 // The same two, planned on the device: stream size and quantum from `rec`
-// (nothing happens unless rec->ok); the directory check on `side` when given.
+// (nothing happens unless rec->ok); the directory check is a launch of its
+// own (wzp_directory_planned), on whichever stream the caller picks.
 template <class T>
 void decode_dequantize_pairs_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, T* coeff, uint32_t* status,
-                                     hipStream_t s, const SideStream* side);
+                                     hipStream_t s);
 void wzp_decode_planned(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint16_t* mag, uint32_t* sign,
-                        uint32_t* status, hipStream_t s, const SideStream* side);
+                        uint32_t* status, hipStream_t s);
+void wzp_directory_planned(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint32_t* status, hipStream_t s);
+// The same fusions for the other fields of full bricks (rank_bricks_fit, z
+// band bz): `rank` is the full brick's rank table (build_rank_tables, shape
+// 0), 8-byte aligned.
+template <class T>
+void quantize_encode_ranks(T* coeff, Dims d, uint32_t bz, const uint16_t* rank, double quantum, SideList qo,
+                           uint8_t* out, uint64_t capacity, WzpScratch w, unsigned long long* total,
+                           uint32_t* status, hipStream_t s, WriteBack wb = kWriteAll,
+                           const SideStream* pack = nullptr);
+template <class T>
+void decode_dequantize_ranks(const uint8_t* in, uint64_t bytes, Dims d, uint32_t bz, const uint16_t* rank,
+                             double quantum, T* coeff, uint32_t* status, hipStream_t s,
+                             const SideStream* side = nullptr);
+template <class T>
+void decode_dequantize_ranks_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, uint32_t bz,
+                                     const uint16_t* rank, T* coeff, uint32_t* status, hipStream_t s);
 
 // sections.hip
 constexpr uint32_t kRepairTile = 4096;  // row-major indices per standalone repair segment

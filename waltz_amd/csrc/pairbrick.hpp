@@ -41,4 +41,20 @@ WZP_HD constexpr uint32_t subblock_rank(uint32_t dx, uint32_t dy, uint32_t dz) {
     return (dx & 1u) | ((dy & 1u) << 1) | (dz << 2) | ((dx >> 1) << 3) | ((dy >> 1) << 4);
 }
 
+// Chunk <-> brick map of the other fields of full bricks (nx % 16 == 0,
+// ny % 16 == 0, nz % bz == 0, z band bz <= 16): brick b of the row-major
+// brick grid holds packed positions [bV, bV + V), V = 256 bz, in recursive
+// octant rank order (one table of V ranks, the full brick's), and chunk c
+// holds packed positions [8192c, 8192c + E), so the bricks of a chunk are
+// consecutive (a chunk starts or ends inside a brick when V does not divide
+// 8192). The (de)quantizing coders visit a chunk's bricks in row-major
+// order, four x-consecutive samples at a time, and place each sample at its
+// rank in the chunk.
+constexpr bool rank_bricks_fit(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bz) {
+    return bz >= 1 && bz <= 16 && nx % 16 == 0 && ny % 16 == 0 && nz % bz == 0 && nx && ny && nz &&
+           !pair_bricks_fit(nx, ny, nz, bz);
+}
+// Bricks that can overlap one chunk (V = 256 at least, starting anywhere).
+constexpr uint32_t kRankBricksMax = 8192 / 256 + 2;
+
 }  // namespace wz

@@ -437,13 +437,15 @@ void launch_finest(const T* coeff, const T* lo, uint32_t lpy, uint32_t lpz, T* o
     launch_inv_level<T>(a, s);
 }
 
+// Fused levels levels-1 .. 1 (the coarse LLL read from coeff): where their
+// reconstruction of the level-1 box lies (coeff itself when only the finest
+// level is fused).
 template <class T>
-void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, const Tuning& tu,
-                   const Level0Emit<T>* emit, hipStream_t s, unsigned base) {
+LowBox<T> fused_coarse(T* coeff, T* tmp0, Dims full, unsigned levels, const Tuning& tu, hipStream_t s,
+                       unsigned base) {
     Dims sizes[5], n = full;
     for (unsigned l = 0; l < levels; ++l) { sizes[l] = n; n = shrink(n, true, true); }
-    const T* lo = coeff;
-    uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
+    LowBox<T> low{coeff, full.nx, full.nx * full.ny};
     static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
     for (int l = int(levels) - 1; l >= 1; --l) {
         StageClock clock(names[l + base], s);
@@ -451,9 +453,9 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
         a.hi = coeff;
         a.hpy = full.nx;
         a.hpz = uint64_t(full.nx) * full.ny;
-        a.lo = lo;
-        a.lpy = lpy;
-        a.lpz = lpz;
+        a.lo = low.p;
+        a.lpy = low.py;
+        a.lpz = low.pz;
         a.n = sizes[l];
         a.out = low_box(tmp0, full, unsigned(l));
         a.opy = sizes[l].nx;
@@ -461,11 +463,17 @@ void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, con
         a.sz = tu.sz[l];
         a.emit = kEmitStore;
         launch_inv_level<T>(a, s);
-        lo = a.out;
-        lpy = a.opy;
-        lpz = a.opz;
+        low = LowBox<T>{a.out, a.opy, a.opz};
     }
-    launch_finest<T>(coeff, lo, uint32_t(lpy), uint32_t(lpz), output, full, tu.sz[0], emit, s, names[base]);
+    return low;
+}
+
+template <class T>
+void fused_inverse(T* coeff, T* output, T* tmp0, Dims full, unsigned levels, const Tuning& tu,
+                   const Level0Emit<T>* emit, hipStream_t s, unsigned base) {
+    static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
+    const LowBox<T> low = fused_coarse<T>(coeff, tmp0, full, levels, tu, s, base);
+    launch_finest<T>(coeff, low.p, low.py, low.pz, output, full, tu.sz[0], emit, s, names[base]);
 }
 
 void level0_repair_segments(Dims full, unsigned lxy, bool dyadic, unsigned width, uint32_t* segments,
@@ -577,6 +585,26 @@ bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigne
                            tuning_for(full, lxy, sizeof(T), true), 0);
 }
 
+template <class T>
+LowBox<T> inverse_coarse_levels(T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, hipStream_t s) {
+    const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
+    if (!lxy || !tu.fused) throw std::logic_error("inverse split without a fused finest level");
+    Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, nullptr, nullptr, s, tu};
+    Dims sizes[5], act = full;
+    for (unsigned l = 0; l < lxy; ++l) { sizes[l] = act; act = shrink(act, true, true); }
+    static const int zxy[3] = {2, 0, 1};
+    for (int l = int(lxy) - 1; l >= int(tu.fused); --l) p.level(full, sizes[l], zxy, 3, false, unsigned(l));
+    return fused_coarse<T>(coeff, tmp0, full, tu.fused, tu, s, 0);
+}
+
+template <class T>
+void inverse_finest_level(const T* coeff, const LowBox<T>& low, T* output, Dims full, unsigned lxy,
+                          const Level0Emit<T>& emit, hipStream_t s) {
+    if (output == coeff) throw std::logic_error("the finest level must not overwrite its coefficients");
+    const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
+    launch_finest<T>(coeff, low.p, low.py, low.pz, output, full, tu.sz[0], &emit, s, "inverse_L0");
+}
+
 bool coarse_split_fits(Dims full, unsigned lxy, unsigned width) {
     if (lxy < 2 || tuning_for(full, lxy, width, true).fused < 2) return false;
     const Dims h1 = shrink(full, true, true), h2 = shrink(h1, true, true);
@@ -633,6 +661,12 @@ void inverse_finest_requant(const T* coeff, const T* t0, Dims full, unsigned lxy
     launch_finest<T>(coeff, t0, h1.nx, h1.nx * h1.ny, nullptr, full, tu.sz[0], &emit, s, "inverse_L0");
 }
 
+template LowBox<float> inverse_coarse_levels<float>(float*, float*, float*, Dims, unsigned, hipStream_t);
+template LowBox<double> inverse_coarse_levels<double>(double*, double*, double*, Dims, unsigned, hipStream_t);
+template void inverse_finest_level<float>(const float*, const LowBox<float>&, float*, Dims, unsigned,
+                                          const Level0Emit<float>&, hipStream_t);
+template void inverse_finest_level<double>(const double*, const LowBox<double>&, double*, Dims, unsigned,
+                                           const Level0Emit<double>&, hipStream_t);
 template void transform_forward<float>(const float*, float*, float*, float*, Dims, unsigned, unsigned, bool,
                                        uint32_t*, hipStream_t);
 template void transform_forward<double>(const double*, double*, double*, double*, Dims, unsigned, unsigned, bool,

@@ -93,6 +93,33 @@ void quantize_encode_pairs(T* coeff, Dims d, double quantum, SideList qo, uint8_
     launch_pack(out, capacity, ws, w, total, status, s, pack);
 }
 
+// The write-back box of a fused quantizing encoder.
+Dims writeback_box(Dims d, WriteBack wb) {
+    if (wb == kWriteNone) return Dims{0, 0, 0};
+    if (wb == kWriteCoarseBox) return Dims{half_len(d.nx), half_len(d.ny), half_len(d.nz)};
+    return d;
+}
+
+void check_rank_bricks(Dims d, uint32_t bz, const uint16_t* rank) {
+    if (!rank_bricks_fit(d.nx, d.ny, d.nz, bz) || reinterpret_cast<uintptr_t>(rank) % 8)
+        throw std::invalid_argument("field does not tile into full bricks");
+}
+
+template <class T>
+void quantize_encode_ranks(T* coeff, Dims d, uint32_t bz, const uint16_t* rank, double quantum, SideList qo,
+                           uint8_t* out, uint64_t capacity, WzpScratch w, unsigned long long* total,
+                           uint32_t* status, hipStream_t s, WriteBack wb, const SideStream* pack) {
+    check_rank_bricks(d, bz, rank);
+    const uint32_t n = d.nx * d.ny * d.nz;
+    const WzpShape ws = wzp_shape(n);
+    const Dims h = writeback_box(d, wb);
+    const RankSource<T> src{coeff, RankBricks{rank, d.nx, d.ny, bz}, T(quantum), T(1.0 / quantum), qo, h.nx, h.ny,
+                            h.nz};
+    hipLaunchKernelGGL(wzp_encode_kernel<RankSource<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, src, n, ws.groups,
+                       out, capacity, w, status);
+    launch_pack(out, capacity, ws, w, total, status, s, pack);
+}
+
 template <class Sink>
 __global__ void __launch_bounds__(256) wzp_decode_kernel(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t n,
                                                          uint32_t chunks, uint32_t groups, Sink sink,
@@ -130,6 +157,16 @@ __global__ void __launch_bounds__(256) wzp_decode_pairs_planned_kernel(const uin
     decode_chunk(in, rec->wzp, n, chunks, groups, blockIdx.x, BrickSink<T>{coeff, nx, ny, T(rec->quantum)}, status);
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) wzp_decode_ranks_planned_kernel(const uint8_t* __restrict__ in,
+                                                                      const DecodeRecord* __restrict__ rec,
+                                                                      uint32_t n, uint32_t chunks, uint32_t groups,
+                                                                      T* __restrict__ coeff, RankBricks g,
+                                                                      uint32_t* __restrict__ status) {
+    if (!rec->ok) return;
+    decode_chunk(in, rec->wzp, n, chunks, groups, blockIdx.x, RankSink<T>{coeff, g, T(rec->quantum)}, status);
+}
+
 __global__ void __launch_bounds__(256) wzp_decode_mag_planned_kernel(const uint8_t* __restrict__ in,
                                                                     const DecodeRecord* __restrict__ rec, uint32_t n,
                                                                     uint32_t chunks, uint32_t groups, MagSink sink,
@@ -155,32 +192,41 @@ void stream_decode(uint32_t n, hipStream_t s, const SideStream* side, Decoders d
     if (side) hip_check(hipEventRecord(side->done, ds), "event record");
 }
 
-template <class Launch>
-void planned_stream_decode(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint32_t* status, hipStream_t s,
-                           const SideStream* side, Launch decoders) {
-    stream_decode(n, s, side, decoders, [&](const WzpShape& ws, hipStream_t ds) {
-        hipLaunchKernelGGL(wzp_directory_planned_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, rec, n, ws.chunks,
-                           ws.groups, status);
-    });
+template <class T>
+void decode_dequantize_pairs_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, T* coeff, uint32_t* status,
+                                     hipStream_t s) {
+    if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
+    const uint32_t n = d.nx * d.ny * d.nz;
+    const WzpShape ws = wzp_shape(n);
+    if (!ws.chunks) return;
+    hipLaunchKernelGGL(wzp_decode_pairs_planned_kernel<T>, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n,
+                       ws.chunks, ws.groups, coeff, d.nx, d.ny, status);
 }
 
 template <class T>
-void decode_dequantize_pairs_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, T* coeff, uint32_t* status,
-                                     hipStream_t s, const SideStream* side) {
-    if (!pair_bricks_fit(d.nx, d.ny, d.nz, 16)) throw std::invalid_argument("field does not tile into brick pairs");
+void decode_dequantize_ranks_planned(const uint8_t* in, const DecodeRecord* rec, Dims d, uint32_t bz,
+                                     const uint16_t* rank, T* coeff, uint32_t* status, hipStream_t s) {
+    check_rank_bricks(d, bz, rank);
     const uint32_t n = d.nx * d.ny * d.nz;
-    planned_stream_decode(in, rec, n, status, s, side, [&](const WzpShape& ws) {
-        hipLaunchKernelGGL(wzp_decode_pairs_planned_kernel<T>, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n,
-                           ws.chunks, ws.groups, coeff, d.nx, d.ny, status);
-    });
+    const WzpShape ws = wzp_shape(n);
+    if (!ws.chunks) return;
+    hipLaunchKernelGGL(wzp_decode_ranks_planned_kernel<T>, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n,
+                       ws.chunks, ws.groups, coeff, RankBricks{rank, d.nx, d.ny, bz}, status);
 }
 
 void wzp_decode_planned(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint16_t* mag, uint32_t* sign,
-                        uint32_t* status, hipStream_t s, const SideStream* side) {
-    planned_stream_decode(in, rec, n, status, s, side, [&](const WzpShape& ws) {
-        hipLaunchKernelGGL(wzp_decode_mag_planned_kernel, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n,
-                           ws.chunks, ws.groups, MagSink{mag, sign}, status);
-    });
+                        uint32_t* status, hipStream_t s) {
+    const WzpShape ws = wzp_shape(n);
+    if (!ws.chunks) return;
+    hipLaunchKernelGGL(wzp_decode_mag_planned_kernel, dim3(ws.chunks), dim3(kThreads), 0, s, in, rec, n, ws.chunks,
+                       ws.groups, MagSink{mag, sign}, status);
+}
+
+void wzp_directory_planned(const uint8_t* in, const DecodeRecord* rec, uint32_t n, uint32_t* status, hipStream_t s) {
+    const WzpShape ws = wzp_shape(n);
+    if (!ws.chunks) return;
+    hipLaunchKernelGGL(wzp_directory_planned_kernel, dim3(1), dim3(kDirThreads), 0, s, in, rec, n, ws.chunks,
+                       ws.groups, status);
 }
 
 void wzp_decode(const uint8_t* in, uint64_t bytes, uint32_t n, uint16_t* mag, uint32_t* sign, uint32_t* status,
@@ -216,6 +262,37 @@ void decode_dequantize_pairs(const uint8_t* in, uint64_t bytes, Dims d, double q
         });
 }
 
+template <class T>
+void decode_dequantize_ranks(const uint8_t* in, uint64_t bytes, Dims d, uint32_t bz, const uint16_t* rank,
+                             double quantum, T* coeff, uint32_t* status, hipStream_t s, const SideStream* side) {
+    check_rank_bricks(d, bz, rank);
+    const uint32_t n = d.nx * d.ny * d.nz;
+    if (bytes < wzp_shape(n).header) throw std::runtime_error("WZP archive size is outside its valid bounds");
+    const RankSink<T> sink{coeff, RankBricks{rank, d.nx, d.ny, bz}, T(quantum)};
+    stream_decode(
+        n, s, side,
+        [&](const WzpShape& ws) {
+            hipLaunchKernelGGL(wzp_decode_kernel<RankSink<T>>, dim3(ws.chunks), dim3(kThreads), 0, s, in, bytes, n,
+                               ws.chunks, ws.groups, sink, status);
+        },
+        [&](const WzpShape& ws, hipStream_t ds) {
+            hipLaunchKernelGGL(wzp_directory_kernel, dim3(1), dim3(kDirThreads), 0, ds, in, bytes, n, ws.chunks,
+                               ws.groups, status);
+        });
+}
+
+#define WZ_RANKS(T)                                                                                               \
+    template void quantize_encode_ranks<T>(T*, Dims, uint32_t, const uint16_t*, double, SideList, uint8_t*,      \
+                                           uint64_t, WzpScratch, unsigned long long*, uint32_t*, hipStream_t,    \
+                                           WriteBack, const SideStream*);                                          \
+    template void decode_dequantize_ranks<T>(const uint8_t*, uint64_t, Dims, uint32_t, const uint16_t*, double,  \
+                                             T*, uint32_t*, hipStream_t, const SideStream*);                       \
+    template void decode_dequantize_ranks_planned<T>(const uint8_t*, const DecodeRecord*, Dims, uint32_t,        \
+                                                     const uint16_t*, T*, uint32_t*, hipStream_t);
+WZ_RANKS(float)
+WZ_RANKS(double)
+#undef WZ_RANKS
+
 template void quantize_encode_pairs<float>(float*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
                                            unsigned long long*, uint32_t*, hipStream_t, WriteBack, const SideStream*);
 template void quantize_encode_pairs<double>(double*, Dims, double, SideList, uint8_t*, uint64_t, WzpScratch,
@@ -226,8 +303,8 @@ template void decode_dequantize_pairs<float>(const uint8_t*, uint64_t, Dims, dou
 template void decode_dequantize_pairs<double>(const uint8_t*, uint64_t, Dims, double, double*, uint32_t*,
                                               hipStream_t, const SideStream*);
 template void decode_dequantize_pairs_planned<float>(const uint8_t*, const DecodeRecord*, Dims, float*, uint32_t*,
-                                                     hipStream_t, const SideStream*);
+                                                     hipStream_t);
 template void decode_dequantize_pairs_planned<double>(const uint8_t*, const DecodeRecord*, Dims, double*, uint32_t*,
-                                                      hipStream_t, const SideStream*);
+                                                      hipStream_t);
 
 }  // namespace wz

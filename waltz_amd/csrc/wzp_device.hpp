@@ -78,11 +78,14 @@ WZ_D void store4(double* p, const double (&v)[4]) {
 }
 
 // Decoder outputs. store() takes thread t's magnitudes [32t, 32t + 32) of
-// chunk c and their sign word.
+// chunk c and their sign word; a collective sink (kCollective) is called by
+// every thread of the workgroup at once and may use the decoder's staging
+// buffer `lds` (kChunk / 2 + kThreads words), whose contents are dead by then.
 struct MagSink {  // packed magnitude / sign arrays
+    static constexpr bool kCollective = false;
     uint16_t* mag;
     uint32_t* sign;
-    WZ_D void store(uint32_t c, uint32_t t, uint32_t E, const uint32_t (&m)[32], uint32_t sw) const {
+    WZ_D void store(uint32_t c, uint32_t t, uint32_t E, const uint32_t (&m)[32], uint32_t sw, uint32_t*) const {
         const uint32_t first = t * 32;
         uint16_t* p = mag + uint64_t(c) * kChunk + first;
         if (first < E) sign[uint64_t(c) * (kChunk / 32) + t] = sw;
@@ -104,10 +107,11 @@ struct MagSink {  // packed magnitude / sign arrays
 // as dequantize_kernel does per sample (cpu/src/codec.cpp:146-147).
 template <class T>
 struct BrickSink {
+    static constexpr bool kCollective = false;
     T* coeff;
     uint32_t nx, ny;
     T q;
-    WZ_D void store(uint32_t c, uint32_t t, uint32_t, const uint32_t (&m)[32], uint32_t sw) const {
+    WZ_D void store(uint32_t c, uint32_t t, uint32_t, const uint32_t (&m)[32], uint32_t sw, uint32_t*) const {
         uint32_t X, Y, Z;
         pair_subblock(nx, ny, c, t, X, Y, Z);
 #pragma unroll
@@ -121,6 +125,108 @@ struct BrickSink {
                 d[dx] = ((sw >> i) & 1u) ? -v : v;
             }
             store4(coeff + (uint64_t(Z + dz) * ny + (Y + dy)) * nx + X, d);
+        }
+    }
+};
+
+// The bricks of chunk c in a field of full bricks (rank_bricks_fit): the
+// chunk's first packed position, its first brick and their count; thread i <
+// count writes the row-major index of brick first + i's origin to kb[i] and,
+// when ko is given, the origin's x, y, z to ko[3i .. 3i + 2].
+struct RankBricks {
+    const uint16_t* rank;  // local row-major index -> rank of a full 16 x 16 x bz brick
+    uint32_t nx, ny, bz;
+};
+struct RankChunk {
+    uint32_t p0, first, count;
+};
+WZ_D RankChunk rank_chunk(const RankBricks& g, uint32_t c, uint32_t E, uint32_t* kb, uint32_t* ko = nullptr) {
+    const uint32_t p0 = c * kChunk;
+    const uint32_t first = (p0 >> 8) / g.bz, last = ((p0 + E - 1u) >> 8) / g.bz;
+    const uint32_t count = last - first + 1u;
+    if (threadIdx.x < count) {
+        const uint32_t nbx = g.nx >> 4, nby = g.ny >> 4, b = first + threadIdx.x;
+        const uint32_t r = b / nbx, bx = b - r * nbx, bzi = r / nby, by = r - bzi * nby;
+        kb[threadIdx.x] = (bzi * g.bz * g.ny + 16u * by) * g.nx + 16u * bx;
+        if (ko) {
+            ko[3 * threadIdx.x] = 16u * bx;
+            ko[3 * threadIdx.x + 1] = 16u * by;
+            ko[3 * threadIdx.x + 2] = bzi * g.bz;
+        }
+    }
+    return RankChunk{p0, first, count};
+}
+
+// Quad f (4 x-consecutive samples) of a chunk's bricks: the row-major index
+// of its first sample, its brick (of the chunk's) and index inside the brick,
+// and its four packed positions less the chunk's first (those outside
+// [0, E) belong to a neighbouring chunk).
+struct RankQuad {
+    uint32_t at, lb, li;
+    uint32_t j[4];
+};
+WZ_D RankQuad rank_quad(const RankBricks& g, const RankChunk& rc, const FastDiv& by_q, const uint32_t* kb,
+                        uint32_t f) {
+    const uint32_t q = 64u * g.bz;  // quads per brick
+    const uint32_t lb = by_q.div(f), li = 4u * (f - lb * q);
+    const uint2 rr = *reinterpret_cast<const uint2*>(g.rank + li);  // ranks of li .. li + 3
+    const uint32_t jb = (rc.first + lb) * (256u * g.bz) - rc.p0;    // (wraps below the chunk)
+    RankQuad out;
+    out.at = kb[lb] + ((li >> 8) * g.ny + ((li >> 4) & 15u)) * g.nx + (li & 15u);
+    out.lb = lb;
+    out.li = li;
+    out.j[0] = jb + (rr.x & 0xFFFFu);
+    out.j[1] = jb + (rr.x >> 16);
+    out.j[2] = jb + (rr.y & 0xFFFFu);
+    out.j[3] = jb + (rr.y >> 16);
+    return out;
+}
+
+// Dequantizes straight into the coefficient volume of a field of full
+// bricks (rank_bricks_fit), as dequantize_kernel does per sample
+// (cpu/src/codec.cpp:146-147): the chunk's magnitudes and sign words go to
+// LDS in packed order, then the workgroup writes the chunk's bricks row by
+// row, each sample read at its rank.
+template <class T>
+struct RankSink {
+    static constexpr bool kCollective = true;
+    T* coeff;
+    RankBricks g;
+    T q;
+    WZ_D void store(uint32_t c, uint32_t t, uint32_t E, const uint32_t (&m)[32], uint32_t sw, uint32_t* lds) const {
+        __shared__ uint32_t kb[kRankBricksMax];
+        const uint16_t* mg = reinterpret_cast<const uint16_t*>(lds);
+        uint32_t* const sg = lds + kChunk / 2;
+        __syncthreads();  // every thread is done with the staging buffer
+        uint4* v = reinterpret_cast<uint4*>(lds) + 4u * t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            v[k] = make_uint4(m[8 * k] | (m[8 * k + 1] << 16), m[8 * k + 2] | (m[8 * k + 3] << 16),
+                              m[8 * k + 4] | (m[8 * k + 5] << 16), m[8 * k + 6] | (m[8 * k + 7] << 16));
+        sg[t] = sw;
+        const RankChunk rc = rank_chunk(g, c, E, kb);
+        __syncthreads();
+        const FastDiv by_q(64u * g.bz);  // f * q < 2^32: exact
+        const uint32_t nq = rc.count * 64u * g.bz;
+        for (uint32_t f = t; f < nq; f += kThreads) {
+            const RankQuad rq = rank_quad(g, rc, by_q, kb, f);
+            T d[4];
+            bool all = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool in = rq.j[k] < E;
+                all = all && in;
+                const uint32_t j = in ? rq.j[k] : 0u;
+                const T x = T(mg[j]) * q;
+                d[k] = ((sg[j >> 5] >> (j & 31u)) & 1u) ? -x : x;
+            }
+            if (all) {
+                store4(coeff + rq.at, d);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (rq.j[k] < E) coeff[rq.at + k] = d[k];
+            }
         }
     }
 };
@@ -229,7 +335,7 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
         uint32_t zero[32];
 #pragma unroll
         for (int i = 0; i < 32; ++i) zero[i] = 0;
-        sink.store(c, t, E, zero, 0u);
+        sink.store(c, t, E, zero, 0u, ib32);
         return;
     }
     // stage the chunk payload in LDS with aligned word loads that never leave
@@ -363,6 +469,9 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
     uint32_t NNZ;
     const uint32_t es = block_exclusive(popc32(nzmask), sh, NNZ);
     if (NNZ == 0 || (NNZ + 7) / 8 != sbytes) bad = true;
+    // (a lane's zero bitmap past the chunk end is its own check: one verdict
+    // for the workgroup before a collective sink's barriers)
+    if constexpr (Sink::kCollective) bad = __syncthreads_or(bad) != 0;
     if (bad) {
         if (t == 0 || (threadIdx.x & 63u) == 0) atomicOr(status, uint32_t(kCorrupt));
         return;
@@ -377,7 +486,7 @@ WZ_D void decode_chunk(const uint8_t* __restrict__ in, uint64_t bytes, uint32_t 
         bits = uint32_t(v >> sh8) & (cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u));
     }
     if (t == 0 && (NNZ & 7u) && (ps[sbytes - 1] >> (NNZ & 7u))) atomicOr(status, uint32_t(kCorrupt));
-    sink.store(c, t, E, m, expand_bits(bits, nzmask));
+    sink.store(c, t, E, m, expand_bits(bits, nzmask), ib32);
 }
 
 // Groups of the largest stream (n < 2^31 samples, 8192 per chunk, 16 chunks per group).

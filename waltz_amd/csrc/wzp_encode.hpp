@@ -22,11 +22,15 @@ WZ_HD uint32_t compaction_selector(uint32_t m) {
 }
 
 // Encoder inputs. load() gives thread t of chunk c the chunk's magnitudes
-// [32t, 32t + 32) (zero past the chunk's E elements) and returns their sign word.
+// [32t, 32t + 32) (zero past the chunk's E elements) and returns their sign
+// word; a collective source (kCollective) is called by every thread of the
+// workgroup at once and may use the encoder's staging buffer `lds` (kChunk / 2
+// + kThreads words), free at that point, which it leaves free again.
 struct MagSource {  // packed magnitude / sign arrays
+    static constexpr bool kCollective = false;
     const uint16_t* mag;
     const uint32_t* sign;
-    WZ_D uint32_t load(uint32_t c, uint32_t t, uint32_t E, uint32_t (&m)[32], uint32_t*) const {
+    WZ_D uint32_t load(uint32_t c, uint32_t t, uint32_t E, uint32_t (&m)[32], uint32_t*, uint32_t*) const {
         const uint32_t first = t * 32;
         const uint16_t* p = mag + uint64_t(c) * kChunk + first;
         if (first + 32 <= E) {
@@ -54,6 +58,7 @@ struct MagSource {  // packed magnitude / sign arrays
 // place and appends q-outliers, exactly as quantize_kernel does per sample.
 template <class T>
 struct BrickSource {
+    static constexpr bool kCollective = false;
     T* coeff;
     uint32_t nx, ny;
     T q, inv;
@@ -62,7 +67,7 @@ struct BrickSource {
     // [0,hz) only: the whole volume, or the coarser levels' box when the
     // finest inverse level requantizes its high bands on load
     uint32_t hx, hy, hz;
-    WZ_D uint32_t load(uint32_t c, uint32_t t, uint32_t, uint32_t (&m)[32], uint32_t* status) const {
+    WZ_D uint32_t load(uint32_t c, uint32_t t, uint32_t, uint32_t (&m)[32], uint32_t* status, uint32_t*) const {
         uint32_t X, Y, Z;
         pair_subblock(nx, ny, c, t, X, Y, Z);
         uint32_t sw = 0;
@@ -119,6 +124,109 @@ struct BrickSource {
     }
 };
 
+// Quantizes the coefficient volume of a field of full bricks on the fly
+// (rank_bricks_fit): the workgroup reads the chunk's bricks row by row (four
+// x-consecutive samples per thread and step), quantizes them as
+// quantize_kernel does (write-back inside [0,hx) x [0,hy) x [0,hz) as in
+// BrickSource, q-outliers appended) and places magnitudes and sign bits at
+// their ranks in LDS; each thread then takes its 32 magnitudes from there.
+template <class T>
+struct RankSource {
+    static constexpr bool kCollective = true;
+    T* coeff;
+    RankBricks g;
+    T q, inv;
+    SideList qo;
+    uint32_t hx, hy, hz;
+    WZ_D uint32_t load(uint32_t c, uint32_t t, uint32_t E, uint32_t (&m)[32], uint32_t* status, uint32_t* lds) const {
+        __shared__ uint32_t kb[kRankBricksMax], ko[3 * kRankBricksMax];
+        uint16_t* const mg = reinterpret_cast<uint16_t*>(lds);
+        uint32_t* const sg = lds + kChunk / 2;
+        sg[t] = 0;
+        const RankChunk rc = rank_chunk(g, c, E, kb, ko);
+        __syncthreads();
+        const FastDiv by_q(64u * g.bz);  // f * q < 2^32: exact
+        const uint32_t nq = rc.count * 64u * g.bz;
+        bool bad = false;
+        // a burst of quads per thread: their rank and coefficient loads are
+        // all issued before the first LDS and write-back stores
+        constexpr uint32_t kBurst = 4;
+        for (uint32_t f0 = 0; f0 < nq; f0 += kBurst * kThreads) {
+            RankQuad rq[kBurst];
+            T v[kBurst][4];
+#pragma unroll
+            for (uint32_t u = 0; u < kBurst; ++u) {
+                const uint32_t f = f0 + u * kThreads + t;
+                rq[u] = rank_quad(g, rc, by_q, kb, f < nq ? f : 0u);
+                load4(coeff + rq[u].at, v[u]);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBurst; ++u) {
+                if (f0 + u * kThreads + t >= nq) continue;
+                const RankQuad& r = rq[u];
+                T d[4];
+                bool all = true;
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k) {
+                    const bool in = r.j[k] < E;
+                    all = all && in;
+                    // cpu/src/quantize_avx2.cpp:45-62: finite and |scaled| < 2^63
+                    // (one compare: false for NaN and infinities), llrint,
+                    // INT32 saturation
+                    const T scaled = v[u][k] * inv;
+                    const T rounded = rint_t(scaled);
+                    const T am = abs_t(rounded);
+                    const uint32_t mv = am >= T(2147483648.0) ? uint32_t(INT32_MAX) : uint32_t(int32_t(am));
+                    d[k] = requantize<T>(v[u][k], q, inv);
+                    if (!in) continue;
+                    bad = bad || !(abs_t(scaled) < T(9223372036854775808.0));
+                    mg[r.j[k]] = uint16_t(mv > 65535u ? 65535u : mv);
+                    if (rounded < T(0)) atomicOr(&sg[r.j[k] >> 5], 1u << (r.j[k] & 31u));
+                    if (mv > 65535u) {  // q-outlier (rare)
+                        const uint32_t gi = r.at + k;
+                        atomicOr(&qo.bitmap[gi >> 5], 1u << (gi & 31u));
+                        const uint32_t slot = atomicAdd(qo.count, 1u);
+                        if (slot < qo.capacity) {
+                            qo.index[slot] = gi;
+                            static_cast<uint32_t*>(qo.payload)[slot] = mv;
+                        }
+                    }
+                }
+                // dequantized write-back inside the box (a quad never
+                // straddles hx: hx is 0, nx or nx / 2, multiples of 8)
+                const uint32_t X = ko[3 * r.lb] + (r.li & 15u), Y = ko[3 * r.lb + 1] + ((r.li >> 4) & 15u),
+                               Z = ko[3 * r.lb + 2] + (r.li >> 8);
+                if (X < hx && Y < hy && Z < hz) {
+                    if (all) {
+                        store4(coeff + r.at, d);
+                    } else {
+#pragma unroll
+                        for (uint32_t k = 0; k < 4; ++k)
+                            if (r.j[k] < E) coeff[r.at + k] = d[k];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t first = 32u * t;
+        const uint4* w = reinterpret_cast<const uint4*>(lds) + 4u * t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 x = w[k];
+            const uint32_t wds[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                m[8 * k + 2 * h] = first + 8 * k + 2 * h < E ? wds[h] & 0xFFFFu : 0u;
+                m[8 * k + 2 * h + 1] = first + 8 * k + 2 * h + 1 < E ? wds[h] >> 16 : 0u;
+            }
+        }
+        const uint32_t sw = sg[t];
+        __syncthreads();  // the buffer is free again for the caller
+        if (bad) atomicOr(status, uint32_t(kQuantDomain));
+        return sw;
+    }
+};
+
 // Encodes chunk c of an n-sample stream with `groups` groups (one workgroup of
 // kThreads threads): meta at out + 4 groups + 3 c (when inside `capacity`),
 // payload bytes to the chunk's staging slot, its length to w.size[c].
@@ -149,7 +257,7 @@ WZ_D void encode_chunk(const Source& src, uint32_t c, uint32_t n, uint32_t group
 
     // 1. this thread's 32 magnitudes and their sign word
     uint32_t m[32];
-    const uint32_t sw = src.load(c, t, E, m, status);
+    const uint32_t sw = src.load(c, t, E, m, status, stage);
     uint32_t nzmask = 0;
 #pragma unroll
     for (int i = 0; i < 32; ++i) nzmask |= uint32_t(m[i] != 0) << i;
