@@ -50,13 +50,17 @@ namespace {
 enum PassKind : uint8_t { kPassX = 0, kPassY = 1, kPassZ = 2, kPassCopy = 3, kPassNone = 255 };
 enum BankId : uint8_t { kHome = 0, kTmp0 = 1, kTmp1 = 2, kOutput = 3 };
 enum JobState : uint32_t { kJobOk = 0, kJobHole = 1, kJobBad = 2 };
+// (De)quantization of a job: separate brick passes around the WZP coders, or
+// inside them for full 16^3 brick pairs (pair_bricks_fit) and for the other
+// fields of full bricks (rank_bricks_fit).
+enum JobCoder : uint32_t { kCoderSeparate = 0, kCoderPairs = 1, kCoderRanks = 2 };
 
 struct JobShape {
     Dims d;
     uint32_t lx, lz, bz, dyadic;
     uint32_t n, chunks, groups, nbx, nby, nbz;
     uint32_t rank;      // the job's Morton rank table set (one per distinct geometry)
-    uint32_t pairs;     // full 16^3 brick pairs: (de)quantization runs inside the WZP coder
+    uint32_t coder;     // JobCoder: where (de)quantization runs
     uint64_t bank;      // element offset in each work bank
     uint64_t mag;       // u16 offset of the magnitudes (in the tmp0 bank)
     uint64_t sign;      // word offset of the sign words (in the tmp1 bank)
@@ -251,8 +255,27 @@ __global__ void __launch_bounds__(256) batch_wzp_pairs_kernel(const uint8_t* __r
 }
 
 // Rank table sets of a plan: per distinct (geometry, z band) the packed rank
-// of every local index of the eight brick edge shapes (quantize.hip's tables).
+// of every local index of the eight brick edge shapes (quantize.hip's tables;
+// the full brick's first).
 constexpr uint64_t kRankSet = 2ull * 8 * 8192;  // u16 entries per set (rank + inverse)
+
+// Chunks of the other full-brick jobs (rank_pfx): decoded straight into
+// dequantized coefficients, as the single-field decoder's RankSink does.
+template <class T>
+__global__ void __launch_bounds__(256) batch_wzp_ranks_kernel(const uint8_t* __restrict__ packed,
+                                                              const JobShape* __restrict__ shape,
+                                                              const JobRun* __restrict__ run,
+                                                              const uint32_t* __restrict__ rank_pfx, uint32_t jobs,
+                                                              const uint16_t* __restrict__ ranks,
+                                                              T* __restrict__ home, uint32_t* status) {
+    const uint32_t j = job_of_wave(rank_pfx, jobs, blockIdx.x);
+    const JobRun& r = run[j];
+    if (r.state != kJobOk) return;
+    const JobShape& sh = shape[j];
+    const RankBricks g{ranks + uint64_t(sh.rank) * kRankSet, sh.d.nx, sh.d.ny, sh.bz};
+    decode_chunk(packed + r.arc + 32, r.wzp, sh.n, sh.chunks, sh.groups, blockIdx.x - rank_pfx[j],
+                 RankSink<T>{home + sh.bank, g, T(r.quantum)}, status);
+}
 // Bricks per workgroup of the separate (de)quantization passes: the bricks
 // of small z bands hold a few hundred samples, and a workgroup per brick
 // spent most of its life finding its job and brick
@@ -687,6 +710,26 @@ __global__ void __launch_bounds__(256) batch_encode_pairs_kernel(const JobShape*
     const BrickSource<T> src{static_cast<T*>(b.home) + sh.bank, sh.d.nx, sh.d.ny, T(quantum), T(1.0 / quantum), qo,
                              sh.d.nx, sh.d.ny, sh.d.nz};
     encode_chunk(src, blockIdx.x - pair_pfx[j], sh.n, sh.groups, b.out + jc.arc + 32, jc.cap - 32, job_wzp(b, jc),
+                 &b.res[j].status);
+}
+
+// Chunks of the other full-brick jobs (rank_pfx): quantized inside the
+// encoder, as the single-field RankSource does, with the dequantized
+// coefficients written back over the whole job for the inverse passes.
+template <class T>
+__global__ void __launch_bounds__(256) batch_encode_ranks_kernel(const JobShape* __restrict__ shape,
+                                                                 const JobComp* __restrict__ comp,
+                                                                 const uint32_t* __restrict__ rank_pfx,
+                                                                 uint32_t jobs, const uint16_t* __restrict__ ranks,
+                                                                 double quantum, CompBufs b) {
+    const uint32_t j = job_of_wave(rank_pfx, jobs, blockIdx.x);
+    const JobShape& sh = shape[j];
+    const JobComp& jc = comp[j];
+    const SideList qo{b.qbm + jc.words, b.qidx + jc.qlist, b.qval + jc.qlist, b.qcount + j, jc.qcap};
+    const RankBricks g{ranks + uint64_t(sh.rank) * kRankSet, sh.d.nx, sh.d.ny, sh.bz};
+    const RankSource<T> src{static_cast<T*>(b.home) + sh.bank, g, T(quantum), T(1.0 / quantum), qo,
+                            sh.d.nx, sh.d.ny, sh.d.nz};
+    encode_chunk(src, blockIdx.x - rank_pfx[j], sh.n, sh.groups, b.out + jc.arc + 32, jc.cap - 32, job_wzp(b, jc),
                  &b.res[j].status);
 }
 
@@ -1234,7 +1277,7 @@ struct HostPlan {
     std::vector<JobShape> shape;
     std::vector<uint32_t> chunk_pfx, brick_pfx;
     // chunks of the brick-pair jobs and of the others (WZP coder launches)
-    std::vector<uint32_t> pair_pfx, mag_pfx;
+    std::vector<uint32_t> pair_pfx, rank_pfx, mag_pfx;
     std::vector<std::vector<Pass>> step;      // step[s][job]
     std::vector<std::vector<uint32_t>> pfx;   // block prefixes per step (per-axis kernel)
     std::vector<std::vector<uint32_t>> lpfx;  // block prefixes per step (LDS kernel)
@@ -1367,10 +1410,11 @@ void finish_plan(HostPlan& p) {
     p.chunk_pfx.assign(J + 1, 0);
     p.brick_pfx.assign(J + 1, 0);
     p.pair_pfx.assign(J + 1, 0);
+    p.rank_pfx.assign(J + 1, 0);
     p.mag_pfx.assign(J + 1, 0);
     std::vector<std::vector<Pass>> sched(J);
     size_t steps = 0;
-    uint64_t bank = 0, mag = 0, sign = 0, chunks = 0, bricks = 0, pchunks = 0, mchunks = 0;
+    uint64_t bank = 0, mag = 0, sign = 0, chunks = 0, bricks = 0, pchunks = 0, rchunks = 0, mchunks = 0;
     p.rank_geo.clear();
     for (uint32_t j = 0; j < J; ++j) {
         JobShape& s = p.shape[j];
@@ -1390,13 +1434,18 @@ void finish_plan(HostPlan& p) {
         p.chunk_pfx[j] = uint32_t(chunks);
         p.brick_pfx[j] = uint32_t(bricks);
         p.pair_pfx[j] = uint32_t(pchunks);
+        p.rank_pfx[j] = uint32_t(rchunks);
         p.mag_pfx[j] = uint32_t(mchunks);
         chunks += s.chunks;
-        // brick-pair jobs (de)quantize inside the WZP coder: no bricks in the
+        // full-brick jobs (de)quantize inside the WZP coder: no bricks in the
         // separate passes
-        s.pairs = pair_bricks_fit(s.d.nx, s.d.ny, s.d.nz, s.bz) ? 1u : 0u;
-        if (s.pairs) {
+        s.coder = pair_bricks_fit(s.d.nx, s.d.ny, s.d.nz, s.bz)   ? kCoderPairs
+                  : rank_bricks_fit(s.d.nx, s.d.ny, s.d.nz, s.bz) ? kCoderRanks
+                                                                  : kCoderSeparate;
+        if (s.coder == kCoderPairs) {
             pchunks += s.chunks;
+        } else if (s.coder == kCoderRanks) {
+            rchunks += s.chunks;
         } else {
             mchunks += s.chunks;
             bricks += (uint64_t(s.nbx) * s.nby * s.nbz + kBricksPerBlock - 1) / kBricksPerBlock;  // brick blocks
@@ -1408,6 +1457,7 @@ void finish_plan(HostPlan& p) {
     p.chunk_pfx[J] = uint32_t(chunks);
     p.brick_pfx[J] = uint32_t(bricks);
     p.pair_pfx[J] = uint32_t(pchunks);
+    p.rank_pfx[J] = uint32_t(rchunks);
     p.mag_pfx[J] = uint32_t(mchunks);
     p.bank_elems = bank;
     p.mag_elems = mag;
@@ -1424,7 +1474,7 @@ struct DevicePlan {
     JobShape* shape = nullptr;
     JobRun* run = nullptr;
     uint32_t *chunk_pfx = nullptr, *brick_pfx = nullptr, *np_pfx = nullptr, *nq_pfx = nullptr, *status = nullptr;
-    uint32_t *pair_pfx = nullptr, *mag_pfx = nullptr;
+    uint32_t *pair_pfx = nullptr, *rank_pfx = nullptr, *mag_pfx = nullptr;
     Pass* pass = nullptr;        // steps x jobs
     uint32_t* pass_pfx = nullptr;  // steps x (jobs + 1)
     uint32_t* host_status = nullptr;  // pinned
@@ -1456,7 +1506,7 @@ struct DevicePlan {
             return at;
         };
         const size_t o_shape = carve(sizeof(JobShape) * J), o_run = carve(sizeof(JobRun) * J);
-        const size_t o_pair = carve(4 * (J + 1)), o_magp = carve(4 * (J + 1));
+        const size_t o_pair = carve(4 * (J + 1)), o_rankp = carve(4 * (J + 1)), o_magp = carve(4 * (J + 1));
         const size_t o_cp = carve(4 * (J + 1)), o_bp = carve(4 * (J + 1)), o_np = carve(4 * (J + 1)),
                      o_nq = carve(4 * (J + 1)), o_st = carve(4), o_pass = carve(sizeof(Pass) * J * S),
                      o_pp = carve(4 * (J + 1) * S);
@@ -1471,6 +1521,7 @@ struct DevicePlan {
         chunk_pfx = reinterpret_cast<uint32_t*>(mem + o_cp);
         brick_pfx = reinterpret_cast<uint32_t*>(mem + o_bp);
         pair_pfx = reinterpret_cast<uint32_t*>(mem + o_pair);
+        rank_pfx = reinterpret_cast<uint32_t*>(mem + o_rankp);
         mag_pfx = reinterpret_cast<uint32_t*>(mem + o_magp);
         np_pfx = reinterpret_cast<uint32_t*>(mem + o_np);
         nq_pfx = reinterpret_cast<uint32_t*>(mem + o_nq);
@@ -1497,6 +1548,7 @@ struct DevicePlan {
         hip_check(hipMemcpyAsync(chunk_pfx, p.chunk_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(brick_pfx, p.brick_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(pair_pfx, p.pair_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
+        hip_check(hipMemcpyAsync(rank_pfx, p.rank_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
         hip_check(hipMemcpyAsync(mag_pfx, p.mag_pfx.data(), 4 * (J + 1), hipMemcpyHostToDevice, s), "plan");
         if (S) {
             hip_check(hipMemcpyAsync(pass, flat.data(), sizeof(Pass) * flat.size(), hipMemcpyHostToDevice, s), "plan");
@@ -1598,6 +1650,9 @@ struct DevicePlan {
         if (p.pair_pfx[J])
             hipLaunchKernelGGL((batch_encode_pairs_kernel<T>), dim3(p.pair_pfx[J]), dim3(256), 0, s, shape, comp,
                                pair_pfx, J, quantum, b);
+        if (p.rank_pfx[J])
+            hipLaunchKernelGGL((batch_encode_ranks_kernel<T>), dim3(p.rank_pfx[J]), dim3(256), 0, s, shape, comp,
+                               rank_pfx, J, ranks, quantum, b);
         if (p.group_pfx[J])
             hipLaunchKernelGGL(batch_pack_kernel, dim3(p.group_pfx[J]), dim3(kPackThreads), 0, s, shape, comp,
                                group_pfx, J, b);
@@ -1648,6 +1703,9 @@ struct DevicePlan {
         if (p.pair_pfx[J])
             hipLaunchKernelGGL((batch_wzp_pairs_kernel<T>), dim3(p.pair_pfx[J]), dim3(256), 0, s, packed, shape, run,
                                pair_pfx, J, static_cast<T*>(bk.home), status);
+        if (p.rank_pfx[J])
+            hipLaunchKernelGGL((batch_wzp_ranks_kernel<T>), dim3(p.rank_pfx[J]), dim3(256), 0, s, packed, shape, run,
+                               rank_pfx, J, ranks, static_cast<T*>(bk.home), status);
         if (p.brick_pfx[J])
             hipLaunchKernelGGL((batch_dequant_kernel<T>), dim3(p.brick_pfx[J]), dim3(256), 0, s, shape, run,
                                brick_pfx, J, ranks, static_cast<const uint16_t*>(bk.tmp0),
