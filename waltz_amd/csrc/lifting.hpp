@@ -91,4 +91,60 @@ WZ_D void lift_tile(const T* src, uint64_t base, uint64_t stride, T* dst, uint64
     }
 }
 
+// Threads of a workgroup that runs lds_pass.
+constexpr uint32_t kLdsThreads = 512;
+
+// One lifting pass in place over lines of n samples in LDS (line l starts at
+// l * pitch with unit stride for Ax = 0, at l with stride `pitch` otherwise).
+// Lines go in groups small enough that every thread holds the (even, odd)
+// pairs of its K tiles: all of a group's loads happen before its stores
+// (the band split / interleave writes inside the same lines); groups touch
+// disjoint lines. Consecutive lanes take consecutive lines of one tile
+// column, so with an odd row pitch they hit distinct LDS banks.
+template <class T, bool Inv, int Ax, int Tl>
+WZ_D void lds_pass(T* A, uint32_t pitch, uint32_t n, uint32_t lines) {
+    constexpr int P = Tl / 2 + 4;
+    constexpr int K = (sizeof(T) == 4 ? 4 : 2) / (Tl > 8 ? 2 : 1);  // <= 128 VGPRs: two workgroups per CU
+    const uint32_t t = threadIdx.x, nt = (n + Tl - 1) / Tl;
+    const uint32_t per = (K * kLdsThreads) / nt;  // lines of a group (>= 1: nt <= K * threads)
+    const uint32_t h = half_len(n);
+    for (uint32_t l0 = 0; l0 < lines; l0 += per) {
+        const uint32_t gl = min(per, lines - l0), items = gl * nt;
+        const FastDiv by_gl(gl);  // w < K * threads, gl <= that: exact
+        T e[K][P], o[K][P];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t w = t + uint32_t(k) * kLdsThreads;
+            if (w >= items) continue;
+            const uint32_t tile = by_gl.div(w), line = l0 + (w - tile * gl);
+            const uint32_t base = Ax == 0 ? line * pitch : line, stride = Ax == 0 ? 1u : pitch;
+            const int32_t start = int32_t(tile) * Tl;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int32_t q = start - 4 + 2 * j;
+                e[k][j] = A[base + (Inv ? split_pos(q, n) : reflect_pos(q, n)) * stride];
+                o[k][j] = A[base + (Inv ? split_pos(q + 1, n) : reflect_pos(q + 1, n)) * stride];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t w = t + uint32_t(k) * kLdsThreads;
+            if (w >= items) continue;
+            const uint32_t tile = by_gl.div(w), line = l0 + (w - tile * gl);
+            const uint32_t base = Ax == 0 ? line * pitch : line, stride = Ax == 0 ? 1u : pitch;
+            const uint32_t start = tile * Tl;
+            if constexpr (Inv) lift_inverse<T, P, Ax != 0>(e[k], o[k]);
+            else lift_forward<T, P>(e[k], o[k]);
+#pragma unroll
+            for (int j = 2; j < P - 2; ++j) {
+                const uint32_t q = start + 2u * uint32_t(j - 2);
+                if (q < n) A[base + (Inv ? q : q >> 1) * stride] = e[k][j];
+                if (q + 1 < n) A[base + (Inv ? q + 1 : h + (q >> 1)) * stride] = o[k][j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace wz

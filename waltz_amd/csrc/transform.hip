@@ -179,6 +179,69 @@ bool plane_pair(const T* src, T* dst, Dims full, Dims act, int a0, int a1, uint3
     return true;
 }
 
+// Every Z level of the plane schedule in one pass (its Z phase): one
+// workgroup per row of 64 columns stages the columns, all z, in LDS, runs the
+// levels' Z lifting there (lds_pass: the tiles and arithmetic of the Z axis
+// passes, so the same bits) and writes them once. src may equal dst; status:
+// screen the input for non-finite samples (the forward's first pass).
+constexpr uint32_t kZColBytes = 66560;
+constexpr uint32_t kZColumns = 64;
+bool zcol_fits(uint32_t nz, unsigned width) { return nz <= kZColBytes / kZColumns / width; }
+
+template <class T, bool Inverse>
+__global__ void __launch_bounds__(kLdsThreads) zcol_levels_kernel(const T* src, T* dst, Dims full, uint32_t levels,
+                                                                  uint32_t* __restrict__ status) {
+    constexpr int ZTile = Inverse ? (sizeof(T) == 4 ? 16 : 8) : 8;
+    __shared__ T A[kZColBytes / sizeof(T)];
+    const uint32_t t = threadIdx.x;
+    const uint32_t xb = (full.nx + kZColumns - 1) / kZColumns;
+    const uint32_t y = blockIdx.x / xb, x0 = (blockIdx.x - y * xb) * kZColumns;
+    const uint32_t cols = min(kZColumns, full.nx - x0), cells = cols * full.nz;
+    const uint64_t plane = uint64_t(full.nx) * full.ny, base = uint64_t(y) * full.nx + x0;
+    const FastDiv by_cols(cols);  // cells fit LDS: exact
+    // every thread issues kBurst loads before it writes any of them to LDS
+    constexpr uint32_t kBurst = 8;
+    bool ok = true;
+    for (uint32_t i0 = 0; i0 < cells; i0 += kBurst * kLdsThreads) {
+        T v[kBurst];
+#pragma unroll
+        for (uint32_t k = 0; k < kBurst; ++k) {
+            const uint32_t i = i0 + k * kLdsThreads + t, r = by_cols.div(i);
+            v[k] = i < cells ? src[uint64_t(r) * plane + base + (i - r * cols)] : T(0);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBurst; ++k) {
+            const uint32_t i = i0 + k * kLdsThreads + t, r = by_cols.div(i);
+            if (i < cells) {
+                ok = ok && (!status || __builtin_isfinite(v[k]));
+                A[r * kZColumns + (i - r * cols)] = v[k];
+            }
+        }
+    }
+    if (__ballot(!ok) && (t & 63u) == 0) atomicOr(status, uint32_t(kNonFiniteInput));
+    __syncthreads();
+    uint32_t ex[5];
+    ex[0] = full.nz;
+    for (uint32_t l = 1; l < 5; ++l) ex[l] = half_len(ex[l - 1]);
+    for (uint32_t k = 0; k < levels; ++k) {
+        const uint32_t l = Inverse ? levels - 1 - k : k;
+        lds_pass<T, Inverse, 2, ZTile>(A, kZColumns, ex[l], cols);  // ends with a barrier
+    }
+    for (uint32_t i = t; i < cells; i += kLdsThreads) {
+        const uint32_t r = by_cols.div(i), c = i - r * cols;
+        dst[uint64_t(r) * plane + base + c] = A[r * kZColumns + c];
+    }
+}
+
+template <class T, bool Inverse>
+void zcol_levels(const T* src, T* dst, Dims full, unsigned levels, uint32_t* status, hipStream_t s) {
+    if (!zcol_fits(full.nz, sizeof(T)) || levels > 5) throw std::logic_error("Z phase does not fit LDS");
+    const uint64_t blocks = uint64_t(full.ny) * ((full.nx + kZColumns - 1) / kZColumns);
+    check_launch_size(blocks * kLdsThreads);
+    hipLaunchKernelGGL((zcol_levels_kernel<T, Inverse>), dim3(uint32_t(blocks)), dim3(kLdsThreads), 0, s, src, dst,
+                       full, uint32_t(levels), status);
+}
+
 template <class T>
 void copy_box(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
     const uint64_t total = uint64_t(act.nx) * act.ny * act.nz;
@@ -523,9 +586,15 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
     static const int zonly[1] = {2};
     static const int yx[2] = {1, 0};
     Dims act = full;
-    for (unsigned l = 0; l < lz; ++l) {
-        p.level(full, act, zonly, 1, false, l);
-        act = shrink(act, false, true);
+    if (lz && zcol_fits(full.nz, sizeof(T))) {  // the Z phase in one LDS-staged pass
+        StageClock clock("forward_Z", s);
+        zcol_levels<T, false>(p.first_src, coeff, full, lz, status, s);
+        p.first = false;
+    } else {
+        for (unsigned l = 0; l < lz; ++l) {
+            p.level(full, act, zonly, 1, false, l);
+            act = shrink(act, false, true);
+        }
     }
     act = full;
     for (unsigned l = 0; l < lxy; ++l) {
@@ -570,7 +639,12 @@ bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy
     static const int xyo[2] = {0, 1};
     static const int zonly[1] = {2};
     for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, xy[l], xyo, 2, lz == 0 && l == 0, unsigned(l));
-    for (int l = int(lz) - 1; l >= 0; --l) p.level(full, zs[l], zonly, 1, l == 0, unsigned(l));
+    if (lz && zcol_fits(full.nz, sizeof(T))) {  // the Z phase in one LDS-staged pass
+        StageClock clock("inverse_Z", s);
+        zcol_levels<T, true>(p.first ? p.first_src : p.home, output, full, lz, nullptr, s);
+    } else {
+        for (int l = int(lz) - 1; l >= 0; --l) p.level(full, zs[l], zonly, 1, l == 0, unsigned(l));
+    }
     return false;
 }
 
