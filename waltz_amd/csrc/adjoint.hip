@@ -169,10 +169,12 @@ __global__ void __launch_bounds__(256) adjoint_copy(const T* __restrict__ src, T
     dst[at] = src[at];
 }
 
-// One group of passes over one box, out of place through tmp0/tmp1, ending in `home`.
+// One group of passes over one box, out of place through tmp0/tmp1, ending in
+// `home`; the first pass reads `first` when given (the whole volume's data).
 template <class T, bool OfInverse>
-void box_passes(T* home, T* tmp0, T* tmp1, Dims full, Dims act, const int* axes, int count, hipStream_t s) {
-    const T* src = home;
+void box_passes(T* home, T* tmp0, T* tmp1, Dims full, Dims act, const int* axes, int count, hipStream_t s,
+                const T* first = nullptr) {
+    const T* src = first ? first : home;
     for (int k = 0; k < count; ++k) {
         T* dst = k == count - 1 ? home : (k & 1 ? tmp1 : tmp0);
         if (dst == src) dst = tmp0;
@@ -188,11 +190,19 @@ void box_passes(T* home, T* tmp0, T* tmp1, Dims full, Dims act, const int* axes,
 }
 }  // namespace
 
-// In place on `data` (full volume); tmp0/tmp1: two full-volume scratch arrays.
-// Schedules are those of transform_forward / transform_inverse, reversed.
+// In place on `data` (full volume), or from `src` into `data` when src is
+// given; tmp0/tmp1: two full-volume scratch arrays. Schedules are those of
+// transform_forward / transform_inverse, reversed.
 template <class T>
 void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       bool of_inverse, hipStream_t s) {
+                       bool of_inverse, hipStream_t s, const T* src) {
+    // the adjoint of the inverse starts with a pass over the whole volume,
+    // which reads src itself; the other schedules start inside a box
+    const T* first = src != data && of_inverse && (lxy || lz) ? src : nullptr;
+    if (src && src != data && !first)
+        hip_check(hipMemcpyAsync(data, src, uint64_t(full.nx) * full.ny * full.nz * sizeof(T),
+                                 hipMemcpyDeviceToDevice, s),
+                  "copy");
     if (dyadic) {
         Dims sizes[5], act = full;
         for (unsigned l = 0; l < lxy; ++l) {
@@ -204,7 +214,8 @@ void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsig
             for (int l = int(lxy) - 1; l >= 0; --l) box_passes<T, false>(data, tmp0, tmp1, full, sizes[l], zxy, 3, s);
         } else {  // inverse: levels L-1..0, axes Z, X, Y -> reversed
             static const int yxz[3] = {1, 0, 2};
-            for (unsigned l = 0; l < lxy; ++l) box_passes<T, true>(data, tmp0, tmp1, full, sizes[l], yxz, 3, s);
+            for (unsigned l = 0; l < lxy; ++l)
+                box_passes<T, true>(data, tmp0, tmp1, full, sizes[l], yxz, 3, s, l == 0 ? first : nullptr);
         }
         return;
     }
@@ -225,13 +236,16 @@ void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsig
         for (int l = int(lz) - 1; l >= 0; --l) box_passes<T, false>(data, tmp0, tmp1, full, zs[l], zonly, 1, s);
     } else {  // inverse: XY levels (X, Y) lxy-1..0, then Z levels lz-1..0 -> reversed
         static const int yx[2] = {1, 0};
-        for (unsigned l = 0; l < lz; ++l) box_passes<T, true>(data, tmp0, tmp1, full, zs[l], zonly, 1, s);
-        for (unsigned l = 0; l < lxy; ++l) box_passes<T, true>(data, tmp0, tmp1, full, xy[l], yx, 2, s);
+        for (unsigned l = 0; l < lz; ++l)
+            box_passes<T, true>(data, tmp0, tmp1, full, zs[l], zonly, 1, s, l == 0 ? first : nullptr);
+        for (unsigned l = 0; l < lxy; ++l)
+            box_passes<T, true>(data, tmp0, tmp1, full, xy[l], yx, 2, s, l == 0 && !lz ? first : nullptr);
     }
 }
 
-template void transform_adjoint<float>(float*, float*, float*, Dims, unsigned, unsigned, bool, bool, hipStream_t);
+template void transform_adjoint<float>(float*, float*, float*, Dims, unsigned, unsigned, bool, bool, hipStream_t,
+                                       const float*);
 template void transform_adjoint<double>(double*, double*, double*, Dims, unsigned, unsigned, bool, bool,
-                                        hipStream_t);
+                                        hipStream_t, const double*);
 
 }  // namespace wz

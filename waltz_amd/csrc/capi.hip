@@ -323,6 +323,11 @@ int waltz_amd_transform_async(int width, int inverse, const void* input, void* o
             T* t0 = static_cast<T*>(tmp0);
             T* t1 = static_cast<T*>(tmp1);
             if (inverse) {
+                // a fused finest level reads the input's high bands in place
+                // and writes the output once
+                if (dyadic && input != output &&
+                    wz::transform_inverse_preserving<T>(static_cast<const T*>(input), out, t0, t1, d, lxy, s))
+                    return;
                 if (input != output)
                     wz::hip_check(hipMemcpyAsync(out, input, n * sizeof(T), hipMemcpyDeviceToDevice, s), "copy");
                 wz::transform_inverse<T>(out, out, t0, t1, d, lxy, lz, dyadic != 0, s);
@@ -345,16 +350,15 @@ int waltz_amd_transform_adjoint_async(int width, int of_inverse, const void* inp
         if (width != 4 && width != 8) throw std::invalid_argument("type must be FP32 or FP64");
         if (!input || !output || !tmp0 || !tmp1) throw std::invalid_argument("null argument");
         const wz::Dims d = dims_of(nx, ny, nz);
-        const uint64_t n = uint64_t(nx) * ny * nz;
         hipStream_t s = as_stream(stream);
-        if (input != output)
-            wz::hip_check(hipMemcpyAsync(output, input, n * size_t(width), hipMemcpyDeviceToDevice, s), "copy");
         if (width == 4)
             wz::transform_adjoint<float>(static_cast<float*>(output), static_cast<float*>(tmp0),
-                                         static_cast<float*>(tmp1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
+                                         static_cast<float*>(tmp1), d, lxy, lz, dyadic != 0, of_inverse != 0, s,
+                                         static_cast<const float*>(input));
         else
             wz::transform_adjoint<double>(static_cast<double*>(output), static_cast<double*>(tmp0),
-                                          static_cast<double*>(tmp1), d, lxy, lz, dyadic != 0, of_inverse != 0, s);
+                                          static_cast<double*>(tmp1), d, lxy, lz, dyadic != 0, of_inverse != 0, s,
+                                          static_cast<const double*>(input));
         wz::hip_check(hipGetLastError(), "adjoint launch");
     });
 }

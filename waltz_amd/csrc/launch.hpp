@@ -69,14 +69,22 @@ struct LowBox {
 };
 template <class T>
 LowBox<T> inverse_coarse_levels(T* coeff, T* tmp0, T* tmp1, Dims full, unsigned lxy, hipStream_t s);
+// transform_inverse of a dyadic field whose finest level runs fused, leaving
+// `coeff` intact: the level-1 box is copied into `output` and the coarse
+// levels run there, the finest level reads the high bands from coeff. False
+// (nothing done) when the geometry has no fused finest level.
+template <class T>
+bool transform_inverse_preserving(const T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy,
+                                  hipStream_t s);
 template <class T>
 void inverse_finest_level(const T* coeff, const LowBox<T>& low, T* output, Dims full, unsigned lxy,
                           const Level0Emit<T>& emit, hipStream_t s);
 // Transpose of transform_forward (of_inverse = false) or transform_inverse
-// (true) with the same geometry, in place on `data`; tmp0/tmp1 as above.
+// (true) with the same geometry, in place on `data`, or from `src` (left
+// intact) into `data`; tmp0/tmp1 as above.
 template <class T>
 void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       bool of_inverse, hipStream_t s);
+                       bool of_inverse, hipStream_t s, const T* src = nullptr);
 // The compress inverse of a brick-pair field whose coefficients stay
 // unquantized (the encoder writes nothing back), in two halves: levels >= 2
 // on a quantized and dequantized compact copy of their box (in t1), level 1

@@ -679,6 +679,23 @@ void inverse_finest_level(const T* coeff, const LowBox<T>& low, T* output, Dims 
     launch_finest<T>(coeff, low.p, low.py, low.pz, output, full, tu.sz[0], &emit, s, "inverse_L0");
 }
 
+template <class T>
+bool transform_inverse_preserving(const T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy,
+                                  hipStream_t s) {
+    const Tuning tu = tuning_for(full, lxy, sizeof(T), true);
+    if (!lxy || !tu.fused || output == coeff) return false;
+    const Dims h1 = shrink(full, true, true);
+    copy_box<T>(coeff, output, full, h1, s);
+    LowBox<T> low = inverse_coarse_levels<T>(output, tmp0, tmp1, full, lxy, s);
+    if (low.p == output) {  // the finest level's coarse input would be overwritten as it stores
+        copy_box<T>(output, tmp1, full, h1, s);
+        low.p = tmp1;
+    }
+    const Level0Emit<T> store{kEmitStore, nullptr, nullptr, 0.0, SegList{}, 0.0};
+    inverse_finest_level<T>(coeff, low, output, full, lxy, store, s);
+    return true;
+}
+
 bool coarse_split_fits(Dims full, unsigned lxy, unsigned width) {
     if (lxy < 2 || tuning_for(full, lxy, width, true).fused < 2) return false;
     const Dims h1 = shrink(full, true, true), h2 = shrink(h1, true, true);
@@ -735,6 +752,9 @@ void inverse_finest_requant(const T* coeff, const T* t0, Dims full, unsigned lxy
     launch_finest<T>(coeff, t0, h1.nx, h1.nx * h1.ny, nullptr, full, tu.sz[0], &emit, s, "inverse_L0");
 }
 
+template bool transform_inverse_preserving<float>(const float*, float*, float*, float*, Dims, unsigned, hipStream_t);
+template bool transform_inverse_preserving<double>(const double*, double*, double*, double*, Dims, unsigned,
+                                                   hipStream_t);
 template LowBox<float> inverse_coarse_levels<float>(float*, float*, float*, Dims, unsigned, hipStream_t);
 template LowBox<double> inverse_coarse_levels<double>(double*, double*, double*, Dims, unsigned, hipStream_t);
 template void inverse_finest_level<float>(const float*, const LowBox<float>&, float*, Dims, unsigned,
