@@ -34,7 +34,7 @@ WZ_HD uint32_t split_index(uint32_t p, uint32_t n) { return (p >> 1) + ((p & 1u)
 
 // Lines per workgroup: X lines are rows (window along x); Y and Z lines are
 // 32 consecutive x, so every global access is a 128-byte (f32) row segment.
-template <int Axis> constexpr int lines_per_group() { return Axis == 0 ? 8 : 32; }
+constexpr int kLines = 32;
 
 // One update step on the LDS window: for every pair j of every line,
 // OddFromEven: o_j += c (e_j + e_{j+1}), else e_j += c (o_{j-1} + o_j).
@@ -56,8 +56,9 @@ WZ_D void step(T (*v)[kWin + 1], T c) {
 template <class T, bool OfInverse, int Axis>
 __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T* __restrict__ dst, Dims full,
                                                     Dims act, uint32_t chunks, uint32_t groups) {
-    constexpr int L = lines_per_group<Axis>();
+    constexpr int L = kLines;
     __shared__ T v[L][kWin + 1];
+    __shared__ uint64_t lbase[L];
     using C = Lift<T>;
     const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
     const uint32_t chunk = blockIdx.x % chunks, rest = blockIdx.x / chunks;
@@ -80,11 +81,14 @@ __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T
         }
     };
 
+    if (threadIdx.x < uint32_t(L)) lbase[threadIdx.x] = line_base(threadIdx.x);
+    __syncthreads();
+
     // zero-extended window [c0 - margin, c0 + 64 + margin) of the interleaved line
     for (uint32_t w = threadIdx.x; w < uint32_t(L * kWin); w += 256) {
         const uint32_t l = Axis == 0 ? w / kWin : w % L, i = Axis == 0 ? w % kWin : w / L;
         const int32_t q = int32_t(c0) - kMargin + int32_t(i);
-        const uint64_t base = line_base(l);
+        const uint64_t base = lbase[l];
         T x = T(0);
         if (base != ~uint64_t(0) && q >= 0 && uint32_t(q) < n) {
             const uint32_t pos = OfInverse ? uint32_t(q) : split_index(uint32_t(q), n);
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T
     for (uint32_t w = threadIdx.x; w < uint32_t(L * kChunkOut); w += 256) {
         const uint32_t l = Axis == 0 ? w / kChunkOut : w % L, i = Axis == 0 ? w % kChunkOut : w / L;
         const uint32_t p = c0 + i;
-        const uint64_t base = line_base(l);
+        const uint64_t base = lbase[l];
         if (base == ~uint64_t(0) || p >= n) continue;
         auto u = [&](int32_t q) { return v[l][q - int32_t(c0) + kMargin]; };
         T s = u(int32_t(p));
@@ -139,7 +143,7 @@ Dims shrink_box(Dims d, bool xy, bool z) {
 
 template <class T, bool OfInverse, int Axis>
 void launch_adjoint(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
-    constexpr int L = lines_per_group<Axis>();
+    constexpr int L = kLines;
     const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
     if (!n || !act.nx || !act.ny || !act.nz) return;
     const uint32_t chunks = (n + kChunkOut - 1) / kChunkOut;
