@@ -242,6 +242,196 @@ void zcol_levels(const T* src, T* dst, Dims full, unsigned levels, uint32_t* sta
                        full, uint32_t(levels), status);
 }
 
+// One XY level of the plane schedule per launch: a workgroup of 256 threads
+// owns a 64 x 32 tile of one plane, runs the level's first pass (inverse: X,
+// forward: Y) over the tile and the halo the second pass reads, keeps the
+// results in LDS and runs the second pass from there, so the level reads its
+// input once (halos aside) and writes once. Tiles and arithmetic are those of
+// lift_tile (inverse Y tile 8 folded, every other tile 8 and unfolded), so
+// every value equals the per-axis passes'.
+constexpr uint32_t kXYW = 64, kXYH = 32, kXYThreads = 256;
+
+struct XYTile {
+    uint32_t x0, y0, z;
+};
+// Tile of this workgroup: consecutive tiles go to one XCD (workgroups are
+// dealt to the eight XCDs in turn), so neighbours' halos meet in its L2.
+WZ_D bool xy_tile(Dims n, XYTile& t) {
+    const uint32_t tx = (n.nx + kXYW - 1) / kXYW, ty = (n.ny + kXYH - 1) / kXYH;
+    const uint32_t total = tx * ty * n.nz, per = (total + 7) / 8;
+    const uint32_t id = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (id >= total) return false;
+    t.x0 = (id % tx) * kXYW;
+    t.y0 = ((id / tx) % ty) * kXYH;
+    t.z = id / (tx * ty);
+    return true;
+}
+uint32_t xy_blocks(Dims n) {
+    const uint64_t total = uint64_t((n.nx + kXYW - 1) / kXYW) * ((n.ny + kXYH - 1) / kXYH) * n.nz;
+    check_launch_size(8 * ((total + 7) / 8) * kXYThreads);
+    return uint32_t(8 * ((total + 7) / 8));
+}
+
+template <class T>
+struct XYInverse {
+    const T* hi;  // the level's coefficients outside its low box
+    uint64_t hpy, hpz;
+    const T* lo;  // its low box [0, ceil(nx/2)) x [0, ceil(ny/2)): the coarser reconstruction
+    uint64_t lpy, lpz;
+    T* out;  // the level's reconstruction (natural layout)
+    uint64_t opy, opz;
+    Dims n;  // level box: nx x ny, every plane
+};
+
+template <class T>
+__global__ void __launch_bounds__(kXYThreads) xy_inverse_kernel(XYInverse<T> a) {
+    constexpr int TX = 8, P = TX / 2 + 4;  // X tiles
+    constexpr uint32_t R = kXYH + 8;       // y positions y0 - 4 .. y0 + 35 (interleaved)
+    constexpr uint32_t kTiles = kXYW / TX;
+    __shared__ T S[R][kXYW + 1];
+    XYTile tl;
+    if (!xy_tile(a.n, tl)) return;
+    const uint32_t t = threadIdx.x, nx = a.n.nx, ny = a.n.ny, hx = half_len(nx), hy = half_len(ny);
+    const T* hiz = a.hi + uint64_t(tl.z) * a.hpz;
+    const T* loz = a.lo + uint64_t(tl.z) * a.lpz;
+    // X: row r of S is y position y0 - 4 + r, read from its split row
+    for (uint32_t item = t; item < R * kTiles; item += kXYThreads) {
+        const uint32_t r = item / kTiles, k = item % kTiles, start = tl.x0 + k * TX;
+        if (start >= nx) continue;
+        const uint32_t sr = split_pos(int32_t(tl.y0) - 4 + int32_t(r), ny);
+        const T* hrow = hiz + uint64_t(sr) * a.hpy;
+        const T* lrow = loz + uint64_t(sr) * a.lpy;
+        const bool lowrow = sr < hy;
+        T e[P], o[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int32_t q = int32_t(start) - 4 + 2 * j;
+            const uint32_t ce = split_pos(q, nx), co = split_pos(q + 1, nx);
+            e[j] = (lowrow && ce < hx) ? lrow[ce] : hrow[ce];
+            o[j] = (lowrow && co < hx) ? lrow[co] : hrow[co];
+        }
+        lift_inverse<T, P, false>(e, o);
+#pragma unroll
+        for (int j = 2; j < P - 2; ++j) {
+            const uint32_t q = start + 2u * uint32_t(j - 2);
+            if (q < nx) S[r][q - tl.x0] = e[j];
+            if (q + 1 < nx) S[r][q + 1 - tl.x0] = o[j];
+        }
+    }
+    __syncthreads();
+    // Y (tile 8, folded): one (column, tile) per thread, lanes 64 consecutive x
+    constexpr int PY = 8;
+    const uint32_t c = t & 63u, k = t >> 6, x = tl.x0 + c, start = tl.y0 + 8 * k;
+    if (x >= nx || start >= ny) return;
+    T e[PY], o[PY];
+#pragma unroll
+    for (int j = 0; j < PY; ++j) {
+        e[j] = S[8 * k + 2 * j][c];
+        o[j] = S[8 * k + 2 * j + 1][c];
+    }
+    lift_inverse<T, PY, true>(e, o);
+    T* col = a.out + uint64_t(tl.z) * a.opz + x;
+#pragma unroll
+    for (int j = 2; j < PY - 2; ++j) {
+        const uint32_t q = start + 2u * uint32_t(j - 2);
+        if (q < ny) col[uint64_t(q) * a.opy] = e[j];
+        if (q + 1 < ny) col[uint64_t(q + 1) * a.opy] = o[j];
+    }
+}
+
+template <class T>
+struct XYForward {
+    const T* src;  // the level's input (natural layout)
+    uint64_t spy, spz;
+    T* hi;  // where the three high bands go
+    uint64_t hpy, hpz;
+    T* lo;  // where the low band [0, ceil(nx/2)) x [0, ceil(ny/2)) goes
+    uint64_t lpy, lpz;
+    Dims n;
+    uint32_t* status;  // non-null: screen the input for non-finite samples
+};
+
+template <class T>
+__global__ void __launch_bounds__(kXYThreads) xy_forward_kernel(XYForward<T> a) {
+    constexpr int TL = 8, P = TL / 2 + 4;  // Y and X tiles
+    constexpr uint32_t C = kXYW + 8;       // x positions x0 - 4 .. x0 + 67 (reflected)
+    __shared__ T S[kXYH][C + 1];
+    XYTile tl;
+    if (!xy_tile(a.n, tl)) return;
+    const uint32_t t = threadIdx.x, nx = a.n.nx, ny = a.n.ny, hx = half_len(nx), hy = half_len(ny);
+    const T* srcz = a.src + uint64_t(tl.z) * a.spz;
+    // Y: column c of S is x position x0 - 4 + c (its reflected column); row r
+    // is the Y output of y position y0 + r (its split row)
+    bool ok = true;
+    for (uint32_t item = t; item < C * (kXYH / TL); item += kXYThreads) {
+        const uint32_t c = item % C, k = item / C, start = tl.y0 + k * TL;
+        if (start >= ny) continue;
+        const T* col = srcz + reflect_pos(int32_t(tl.x0) - 4 + int32_t(c), nx);
+        T e[P], o[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int32_t q = int32_t(start) - 4 + 2 * j;
+            e[j] = col[uint64_t(reflect_pos(q, ny)) * a.spy];
+            o[j] = col[uint64_t(reflect_pos(q + 1, ny)) * a.spy];
+        }
+        if (a.status) {  // input screen (cpu/src/codec.cpp:183)
+#pragma unroll
+            for (int j = 0; j < P; ++j) ok = ok && finite_value(e[j]) && finite_value(o[j]);
+        }
+        lift_forward<T, P>(e, o);
+#pragma unroll
+        for (int j = 2; j < P - 2; ++j) {
+            const uint32_t q = start + 2u * uint32_t(j - 2);
+            if (q < ny) S[q - tl.y0][c] = e[j];
+            if (q + 1 < ny) S[q + 1 - tl.y0][c] = o[j];
+        }
+    }
+    if (a.status && !ok) atomicOr(a.status, uint32_t(kNonFiniteInput));
+    __syncthreads();
+    // X: one (row, tile) per thread; row r is split row sy of y = y0 + r
+    constexpr uint32_t kTiles = kXYW / TL;
+    for (uint32_t item = t; item < kXYH * kTiles; item += kXYThreads) {
+        const uint32_t r = item / kTiles, k = item % kTiles, y = tl.y0 + r, start = tl.x0 + k * TL;
+        if (y >= ny || start >= nx) continue;
+        const uint32_t sy = (y & 1u) ? hy + (y >> 1) : (y >> 1);
+        T e[P], o[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const uint32_t q = start + 2u * uint32_t(j) - tl.x0;  // S column of x position start - 4 + 2j
+            e[j] = S[r][q];
+            o[j] = S[r][q + 1];
+        }
+        lift_forward<T, P>(e, o);
+        const bool lowrow = sy < hy;
+        T* hrow = a.hi + uint64_t(tl.z) * a.hpz + uint64_t(sy) * a.hpy;
+        T* lrow = a.lo + uint64_t(tl.z) * a.lpz + uint64_t(sy) * a.lpy;
+#pragma unroll
+        for (int j = 2; j < P - 2; ++j) {
+            const uint32_t q = start + 2u * uint32_t(j - 2), ce = q >> 1, co = hx + (q >> 1);
+            if (q < nx) (lowrow && ce < hx ? lrow : hrow)[ce] = e[j];
+            if (q + 1 < nx) hrow[co] = o[j];
+        }
+    }
+}
+
+template <class T>
+void xy_inverse(const XYInverse<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL((xy_inverse_kernel<T>), dim3(xy_blocks(a.n)), dim3(kXYThreads), 0, s, a);
+}
+template <class T>
+void xy_forward(const XYForward<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL((xy_forward_kernel<T>), dim3(xy_blocks(a.n)), dim3(kXYThreads), 0, s, a);
+}
+
+// Compact XY boxes of the plane schedule's levels >= 1 (every plane) in
+// `scratch`: level l's box at offset 0 for odd l and after the largest
+// level-1 box for even l (the two never alias).
+template <class T>
+T* xy_box(T* scratch, Dims full, unsigned l) {
+    const uint64_t first = uint64_t(half_len(full.nx)) * half_len(full.ny) * full.nz;
+    return (l & 1u) ? scratch : scratch + ((first + 63) & ~uint64_t(63));
+}
+
 template <class T>
 void copy_box(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
     const uint64_t total = uint64_t(act.nx) * act.ny * act.nz;
@@ -585,6 +775,37 @@ void transform_forward(const T* input, T* coeff, T* tmp0, T* tmp1, Dims full, un
     }
     static const int zonly[1] = {2};
     static const int yx[2] = {1, 0};
+    // every XY level as one fused launch: the Z phase (one LDS-staged pass)
+    // into tmp1, then level l reads its box (tmp1, then the compact boxes in
+    // tmp0) and writes its high bands into coeff and its low band into the
+    // next box (coeff for the last level)
+    if (lxy && (lz ? zcol_fits(full.nz, sizeof(T)) : input != coeff)) {
+        const T* src = input;
+        uint64_t spy = full.nx, spz = uint64_t(full.nx) * full.ny;
+        if (lz) {
+            StageClock clock("forward_Z", s);
+            zcol_levels<T, false>(input, tmp1, full, lz, status, s);
+            src = tmp1;
+        }
+        Dims n = full;
+        static const char* names[] = {"forward_L0", "forward_L1", "forward_L2", "forward_L3", "forward_L4"};
+        for (unsigned l = 0; l < lxy; ++l) {
+            StageClock clock(names[l], s);
+            const Dims h{half_len(n.nx), half_len(n.ny), n.nz};
+            const bool last = l + 1 == lxy;
+            XYForward<T> a{src, spy, spz, coeff, full.nx, uint64_t(full.nx) * full.ny, nullptr, 0, 0, n,
+                           l == 0 && !lz ? status : nullptr};
+            a.lo = last ? coeff : xy_box(tmp0, full, l + 1);
+            a.lpy = last ? full.nx : h.nx;
+            a.lpz = last ? uint64_t(full.nx) * full.ny : uint64_t(h.nx) * h.ny;
+            xy_forward<T>(a, s);
+            src = a.lo;
+            spy = a.lpy;
+            spz = a.lpz;
+            n = h;
+        }
+        return;
+    }
     Dims act = full;
     if (lz && zcol_fits(full.nz, sizeof(T))) {  // the Z phase in one LDS-staged pass
         StageClock clock("forward_Z", s);
@@ -638,6 +859,37 @@ bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy
     for (unsigned l = 0; l < lz; ++l) { zs[l] = act; act = shrink(act, false, true); }
     static const int xyo[2] = {0, 1};
     static const int zonly[1] = {2};
+    // every XY level as one fused launch: level l reads coeff's bands and the
+    // coarser reconstruction (coeff's low quadrant for the coarsest level,
+    // then the compact boxes in tmp0) and writes its box to tmp0, the finest
+    // into tmp1, from which the Z phase (one LDS-staged pass) writes output
+    if (lxy && (lz ? zcol_fits(full.nz, sizeof(T)) : output != coeff)) {
+        static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
+        const T* lo = coeff;
+        uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
+        for (int l = int(lxy) - 1; l >= 0; --l) {
+            StageClock clock(names[l + base], s);
+            XYInverse<T> a{coeff, full.nx, uint64_t(full.nx) * full.ny, lo, lpy, lpz, nullptr, 0, 0, xy[l]};
+            if (l == 0) {
+                a.out = lz ? tmp1 : output;
+                a.opy = full.nx;
+                a.opz = uint64_t(full.nx) * full.ny;
+            } else {
+                a.out = xy_box(tmp0, full, unsigned(l));
+                a.opy = xy[l].nx;
+                a.opz = uint64_t(xy[l].nx) * xy[l].ny;
+            }
+            xy_inverse<T>(a, s);
+            lo = a.out;
+            lpy = a.opy;
+            lpz = a.opz;
+        }
+        if (lz) {
+            StageClock clock("inverse_Z", s);
+            zcol_levels<T, true>(tmp1, output, full, lz, nullptr, s);
+        }
+        return false;
+    }
     for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, xy[l], xyo, 2, lz == 0 && l == 0, unsigned(l));
     if (lz && zcol_fits(full.nz, sizeof(T))) {  // the Z phase in one LDS-staged pass
         StageClock clock("inverse_Z", s);
