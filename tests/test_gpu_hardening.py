@@ -219,3 +219,36 @@ def test_route_taken(wz, oracle, planned):
         finally:
             _lib.profile(False)
     assert ("decode_plan" in stages) == planned, sorted(stages)
+
+
+# A plane-schedule field (lz = 1, lxy = 2): the planned decode with the
+# inverse as fused XY levels and one Z phase, the repairs a pass of their own.
+PLANE = (12, 32, 32)
+
+
+def test_plane_route_taken(wz, oracle):
+    from waltz_amd import _lib
+    a = oracle.compress(field(PLANE, "outliers", np.float32), 1e-2)
+    _lib.profile(True)
+    try:
+        _lib.profile_report()
+        assert _agree(wz, oracle, a, PLANE, np.float32, "whole")
+        stages = _lib.profile_report()
+    finally:
+        _lib.profile(False)
+    assert "decode_plan" in stages and "patches" in stages, sorted(stages)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_plane_planned_mutations(wz, oracle, dtype):
+    a = bytearray(oracle.compress(field(PLANE, "outliers", dtype), 1e-2))
+    assert _agree(wz, oracle, a, PLANE, dtype, "whole")
+    cuts = sorted(set(range(64)) | set(range(64, len(a), max(1, len(a) // 128))) | set(range(len(a) - 64, len(a))))
+    for k in cuts:
+        assert not _agree(wz, oracle, a[:k], PLANE, dtype, f"truncated to {k}")
+    assert not _agree(wz, oracle, a + b"\0" * 8, PLANE, dtype, "trailing bytes")
+    for pos in range(32):
+        for mask in (0x01, 0x10, 0x80, 0xFF):
+            m = bytearray(a)
+            m[pos] ^= mask
+            _agree(wz, oracle, m, PLANE, dtype, f"header byte {pos} ^ {mask:#x}")

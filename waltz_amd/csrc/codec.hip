@@ -421,7 +421,7 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
     // samples (its key table in the magnitude scratch, free once the
     // coefficients are dequantized); a list out of order comes back as
     // kUnordered, and the general pass patches separately
-    const uint64_t keys = canonical ? patch_table_words(g.d, np, sizeof(T)) : 0;
+    const uint64_t keys = canonical && dyadic ? patch_table_words(g.d, np, sizeof(T)) : 0;
     if (keys) {
         screen.pidx = reinterpret_cast<const uint32_t*>(archive + L.pi);
         screen.perr = reinterpret_cast<const T*>(archive + L.pv);
@@ -480,11 +480,14 @@ uint32_t decode_pass(const uint8_t* archive, const uint8_t* hdr, const Geometry&
 // first waiting for the header. It assumes the canonical archive of the
 // geometry (the one compress writes); any other header leaves every planned
 // kernel idle (record not ok) and the caller decodes on the host-header path.
-// Applies to fields whose finest inverse level runs fused (it applies the
-// repairs, so every count the launch needs comes from the record).
+// Applies to dyadic fields whose finest inverse level runs fused (it applies
+// the repairs, so every count the launch needs comes from the record), and to
+// plane-schedule fields whose inverse runs as fused XY levels and one Z phase
+// (which screens the output; the repairs follow in a pass of their own).
 bool planned_decode_fits(const Geometry& g, uint64_t archive_bytes, unsigned width) {
-    return archive_bytes != ~uint64_t(0) && archive_bytes >= 32 && g.dyadic && g.lx >= 1 &&
-           tuning_for(g.d, g.lx, width, true).fused >= 1 && patch_table_words(g.d, g.n, width) != 0;
+    if (archive_bytes == ~uint64_t(0) || archive_bytes < 32) return false;
+    if (!g.dyadic) return plane_fused_fits(g.d, g.lx, g.lz, width);
+    return g.lx >= 1 && tuning_for(g.d, g.lx, width, true).fused >= 1 && patch_table_words(g.d, g.n, width) != 0;
 }
 
 template <class T>
@@ -516,7 +519,8 @@ uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Ge
     const bool pairs = pair_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz);
     const uint16_t* rank = pairs ? nullptr : m.ranks(g.d, g.bz, s);
     const bool ranks = rank_bricks_fit(g.d.nx, g.d.ny, g.d.nz, g.bz) && reinterpret_cast<uintptr_t>(rank) % 8 == 0;
-    const bool keys_on_side = (pairs || ranks) && side;
+    const bool fold = g.dyadic;  // repairs folded into the finest fused level (else a pass of their own)
+    const bool keys_on_side = fold && (pairs || ranks) && side;
     if (side) hip_check(hipEventRecord(side->ready, s), "event record");
     if (pairs || ranks) {
         StageClock ck("decode_dequantize", s);
@@ -543,13 +547,27 @@ uint32_t decode_planned(const uint8_t* archive, uint64_t archive_bytes, const Ge
     } else {
         wzp_directory_planned(archive + 32, rec, n, status, s);
     }
-    if (!keys_on_side) {
+    if (fold && !keys_on_side) {
         StageClock ck("patch_keys", s);
         repair_keys_planned<T>(archive, rec, g.d, table, status, s);
     }
     {
         StageClock ck("qoutliers", s);
         apply_qoutliers_planned<T>(coeff, archive, rec, n, status, s);
+    }
+    if (!fold) {
+        Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}, 0.0};
+        const bool screened = transform_inverse<T>(coeff, output, t0, t1, g.d, g.lx, g.lz, false, s, &screen);
+        {
+            StageClock ck("patches", s);
+            apply_repairs_planned<T>(output, archive, rec, n, status, s);
+        }
+        if (!screened) check_finite<T>(output, n, status, s);
+        if (side) hip_check(hipStreamWaitEvent(s, side->done, 0), "stream wait");
+        hip_check(hipMemcpyAsync(m.host, rec, sizeof(DecodeRecord), hipMemcpyDeviceToHost, s), "status copy");
+        hip_check(hipStreamSynchronize(s), "decompress");
+        hip_check(hipGetLastError(), "decompress launch");
+        return m.host->status;
     }
     Level0Emit<T> screen{kEmitStoreScreen, status, nullptr, 0.0, SegList{}, 0.0};
     screen.np = n;  // an upper bound: the count comes from the record

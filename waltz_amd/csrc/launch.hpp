@@ -93,6 +93,10 @@ void transform_adjoint(T* data, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsig
 // finest level fused with the same requantization and `emit`. Applies when
 // coarse_split_fits (dyadic, >= 2 levels, both leading levels fused).
 bool coarse_split_fits(Dims full, unsigned lxy, unsigned width);
+// Whether the plane schedule's inverse of this geometry runs as fused XY
+// levels and one LDS-staged Z phase; that inverse screens its output for
+// non-finite samples when asked (a kEmitStoreScreen emit without repairs).
+bool plane_fused_fits(Dims full, unsigned lxy, unsigned lz, unsigned width);
 template <class T>
 void inverse_coarse_requant(const T* coeff, T* t0, T* t1, Dims full, unsigned lxy, double quantum, hipStream_t s);
 template <class T>
@@ -261,6 +265,9 @@ void apply_qoutliers_planned(T* coeff, const uint8_t* archive, const DecodeRecor
 template <class T>
 void repair_keys_planned(const uint8_t* archive, const DecodeRecord* rec, Dims d, uint32_t* table, uint32_t* status,
                          hipStream_t s);
+template <class T>
+void apply_repairs_planned(T* out, const uint8_t* archive, const DecodeRecord* rec, uint32_t n, uint32_t* status,
+                           hipStream_t s);
 // Flags kCorrupt when an index of the list repeats (or is >= n); `bitmap`:
 // ceil(n/32) words, cleared here.
 void check_unique(const uint8_t* idx, uint32_t count, uint32_t n, uint32_t* bitmap, uint32_t* status, hipStream_t s);

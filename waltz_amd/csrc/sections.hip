@@ -117,11 +117,10 @@ __global__ void qpatch_kernel(T* __restrict__ c, const uint32_t* __restrict__ id
     if (j < count) qpatch_one(c, idx, val, j, n, q, status);
 }
 
+// Repair record j: the sample's correction subtracted, with the record checks.
 template <class T>
-__global__ void rpatch_kernel(T* __restrict__ out, const uint32_t* __restrict__ idx, const T* __restrict__ err,
-                              uint32_t count, uint32_t n, uint32_t* __restrict__ status) {
-    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-    if (j >= count) return;
+WZ_D void rpatch_one(T* __restrict__ out, const uint32_t* __restrict__ idx, const T* __restrict__ err, uint32_t j,
+                     uint32_t n, uint32_t* __restrict__ status) {
     const uint32_t i = idx[j];
     const T e = err[j];
     if (i >= n || !__builtin_isfinite(e)) {
@@ -132,6 +131,13 @@ __global__ void rpatch_kernel(T* __restrict__ out, const uint32_t* __restrict__ 
     const T v = out[i] - e;  // cpu/src/codec.cpp:330
     out[i] = v;
     if (!__builtin_isfinite(v)) atomicOr(status, uint32_t(kNonFiniteOutput));
+}
+
+template <class T>
+__global__ void rpatch_kernel(T* __restrict__ out, const uint32_t* __restrict__ idx, const T* __restrict__ err,
+                              uint32_t count, uint32_t n, uint32_t* __restrict__ status) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j < count) rpatch_one(out, idx, err, j, n, status);
 }
 
 // Repair record j: the checks of rpatch_kernel, and the record that opens
@@ -186,6 +192,18 @@ __global__ void __launch_bounds__(256) qpatch_planned_kernel(T* __restrict__ c, 
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) rpatch_planned_kernel(T* __restrict__ out, const uint8_t* __restrict__ archive,
+                                                             const DecodeRecord* __restrict__ rec, uint32_t n,
+                                                             uint32_t* __restrict__ status) {
+    if (!rec->ok) return;
+    const uint32_t count = rec->np;
+    const auto* idx = reinterpret_cast<const uint32_t*>(archive + rec->pi);
+    const auto* err = reinterpret_cast<const T*>(archive + rec->pv);
+    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < count; j += gridDim.x * 256u)
+        rpatch_one(out, idx, err, j, n, status);
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) repair_keys_planned_kernel(const uint8_t* __restrict__ archive,
                                                                   const DecodeRecord* __restrict__ rec, uint32_t n,
                                                                   uint32_t nx, uint32_t txn,
@@ -235,6 +253,13 @@ void apply_repairs(T* out, const uint8_t* idx, const uint8_t* err, uint32_t coun
     if (!count) return;
     hipLaunchKernelGGL((rpatch_kernel<T>), dim3((count + 255) / 256), dim3(256), 0, s, out,
                        reinterpret_cast<const uint32_t*>(idx), reinterpret_cast<const T*>(err), count, n, status);
+}
+
+template <class T>
+void apply_repairs_planned(T* out, const uint8_t* archive, const DecodeRecord* rec, uint32_t n, uint32_t* status,
+                           hipStream_t s) {
+    hipLaunchKernelGGL((rpatch_planned_kernel<T>), dim3(kPlannedBlocks), dim3(256), 0, s, out, archive, rec, n,
+                       status);
 }
 
 template <class T>
@@ -300,6 +325,8 @@ void check_finite(const T* out, uint32_t n, uint32_t* status, hipStream_t s) {
                                              hipStream_t);                                                        \
     template void repair_keys_planned<T>(const uint8_t*, const DecodeRecord*, Dims, uint32_t*, uint32_t*,           \
                                          hipStream_t);                                                            \
+    template void apply_repairs_planned<T>(T*, const uint8_t*, const DecodeRecord*, uint32_t, uint32_t*,           \
+                                           hipStream_t);                                                          \
     template void check_finite<T>(const T*, uint32_t, uint32_t*, hipStream_t);
 WZ_INST(float)
 WZ_INST(double)

@@ -183,7 +183,8 @@ bool plane_pair(const T* src, T* dst, Dims full, Dims act, int a0, int a1, uint3
 // workgroup per row of 64 columns stages the columns, all z, in LDS, runs the
 // levels' Z lifting there (lds_pass: the tiles and arithmetic of the Z axis
 // passes, so the same bits) and writes them once. src may equal dst; status:
-// screen the input for non-finite samples (the forward's first pass).
+// screen for non-finite samples the forward's input (its first pass) or the
+// inverse's output (its last).
 constexpr uint32_t kZColBytes = 66560;
 constexpr uint32_t kZColumns = 64;
 bool zcol_fits(uint32_t nz, unsigned width) { return nz <= kZColBytes / kZColumns / width; }
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(kLdsThreads) zcol_levels_kernel(const T* src, 
         for (uint32_t k = 0; k < kBurst; ++k) {
             const uint32_t i = i0 + k * kLdsThreads + t, r = by_cols.div(i);
             if (i < cells) {
-                ok = ok && (!status || __builtin_isfinite(v[k]));
+                ok = ok && (Inverse || !status || __builtin_isfinite(v[k]));
                 A[r * kZColumns + (i - r * cols)] = v[k];
             }
         }
@@ -227,10 +228,14 @@ __global__ void __launch_bounds__(kLdsThreads) zcol_levels_kernel(const T* src, 
         const uint32_t l = Inverse ? levels - 1 - k : k;
         lds_pass<T, Inverse, 2, ZTile>(A, kZColumns, ex[l], cols);  // ends with a barrier
     }
+    bool fin = true;
     for (uint32_t i = t; i < cells; i += kLdsThreads) {
         const uint32_t r = by_cols.div(i), c = i - r * cols;
-        dst[uint64_t(r) * plane + base + c] = A[r * kZColumns + c];
+        const T v = A[r * kZColumns + c];
+        if (Inverse && status) fin = fin && __builtin_isfinite(v);
+        dst[uint64_t(r) * plane + base + c] = v;
     }
+    if (__ballot(!fin) && (t & 63u) == 0) atomicOr(status, uint32_t(kNonFiniteOutput));
 }
 
 template <class T, bool Inverse>
@@ -864,6 +869,8 @@ bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy
     // then the compact boxes in tmp0) and writes its box to tmp0, the finest
     // into tmp1, from which the Z phase (one LDS-staged pass) writes output
     if (lxy && (lz ? zcol_fits(full.nz, sizeof(T)) : output != coeff)) {
+        // the Z phase screens the output when asked (no repairs to fold here)
+        uint32_t* screen = emit && emit->mode == kEmitStoreScreen && !emit->np && lz ? emit->status : nullptr;
         static const char* names[] = {"inverse_L0", "inverse_L1", "inverse_L2", "inverse_L3", "inverse_L4"};
         const T* lo = coeff;
         uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
@@ -886,9 +893,9 @@ bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy
         }
         if (lz) {
             StageClock clock("inverse_Z", s);
-            zcol_levels<T, true>(tmp1, output, full, lz, nullptr, s);
+            zcol_levels<T, true>(tmp1, output, full, lz, screen, s);
         }
-        return false;
+        return screen != nullptr;
     }
     for (int l = int(lxy) - 1; l >= 0; --l) p.level(full, xy[l], xyo, 2, lz == 0 && l == 0, unsigned(l));
     if (lz && zcol_fits(full.nz, sizeof(T))) {  // the Z phase in one LDS-staged pass
@@ -946,6 +953,10 @@ bool transform_inverse_preserving(const T* coeff, T* output, T* tmp0, T* tmp1, D
     const Level0Emit<T> store{kEmitStore, nullptr, nullptr, 0.0, SegList{}, 0.0};
     inverse_finest_level<T>(coeff, low, output, full, lxy, store, s);
     return true;
+}
+
+bool plane_fused_fits(Dims full, unsigned lxy, unsigned lz, unsigned width) {
+    return lxy && lz && zcol_fits(full.nz, width);
 }
 
 bool coarse_split_fits(Dims full, unsigned lxy, unsigned width) {
