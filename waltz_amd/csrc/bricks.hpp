@@ -90,14 +90,7 @@ WZ_D void quantize_brick(T* __restrict__ coeff, const BrickGeometry& g, T q, T i
         coeff[at] = m ? T(negative ? -m : m) * q : T(0);
         mag[k.base + r] = uint16_t(m > 65535 ? 65535 : m);
         if (negative && m) atomicOr(&bits[(r + shift) >> 5], 1u << ((r + shift) & 31u));
-        if (m > 65535) {
-            atomicOr(&qo.bitmap[at >> 5], 1u << (at & 31u));
-            const uint32_t slot = atomicAdd(qo.count, 1u);
-            if (slot < qo.capacity) {
-                qo.index[slot] = uint32_t(at);
-                static_cast<uint32_t*>(qo.payload)[slot] = uint32_t(m);
-            }
-        }
+        append_qoutlier(qo, m > 65535, uint32_t(at), uint32_t(m));
       }
     }
     if (bad) atomicOr(status, uint32_t(kQuantDomain));

@@ -211,6 +211,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     auto plan = [&](Carve& c) {
         struct P {
             T *coeff, *t0, *t1, *rec;
+            uint8_t* stage;
             uint16_t* mag;
             uint32_t *sign, *qbm, *rbm, *qidx, *qval, *ridx, *rcount, *seg, *zeroed;
             T* rerr;
@@ -220,10 +221,18 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         p.coeff = c.take<T>(n);
         p.t0 = c.take<T>(n);
         p.t1 = c.take<T>(n);
-        // the reconstruction of an unfused finest level, and before it the
-        // encoder's staging slots (both are dead by the time the other is used)
-        p.rec = reinterpret_cast<T*>(
-            c.take<uint8_t>(std::max<uint64_t>(uint64_t(n) * sizeof(T), wzp_staging_bytes(ws.chunks))));
+        // the reconstruction of an unfused finest level and the encoder's
+        // staging slots: one region with a fused finest level (which never
+        // writes the reconstruction), else two, so that the WZP group pass
+        // can run beside the inverse
+        if (fsegs) {
+            p.rec = reinterpret_cast<T*>(
+                c.take<uint8_t>(std::max<uint64_t>(uint64_t(n) * sizeof(T), wzp_staging_bytes(ws.chunks))));
+            p.stage = reinterpret_cast<uint8_t*>(p.rec);
+        } else {
+            p.rec = c.take<T>(n);
+            p.stage = c.take<uint8_t>(wzp_staging_bytes(ws.chunks));
+        }
         p.mag = c.take<uint16_t>(n);
         p.ridx = c.take<uint32_t>(rslots);
         p.rerr = c.take<T>(rslots);
@@ -254,16 +263,15 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
 
     transform_forward<T>(input, p.coeff, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, status, s);
     SideList qo{p.qbm, p.qidx, p.qval, p.counters + 1, nq};
-    WzpScratch wsc{p.desc, reinterpret_cast<uint8_t*>(p.rec), p.size, p.counters};
+    WzpScratch wsc{p.desc, p.stage, p.size, p.counters};
     // with the fused repair emitter and a quantizing encoder the finest
     // level's high bands stay unquantized in coeff; that kernel requantizes
     // them on load
     const bool requant = fsegs && fusedq;
-    // With a fused finest level the reconstruction buffer (which shares the
-    // staging slots) is never written, so the group pass runs on the side
-    // stream beside the inverse levels, whose coarse passes leave most of the
-    // device idle; the archive sections wait for it.
-    const SideStream* side = fsegs ? &w.side : nullptr;
+    // The group pass runs on the side stream beside the inverse levels (the
+    // staging slots it reads are never written by them); the archive sections
+    // wait for it.
+    const SideStream* side = &w.side;
     const SideJoin join{side};
     // When the coarser inverse levels requantize what they load as well (the
     // two leading levels fused), the encoder writes nothing back. They run
@@ -271,7 +279,10 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
     // it as much as they hid, and the group pass then collided with the
     // finest level, which needs every CU's two workgroup slots.
     const bool split = requant && coarse_split_fits(g.d, g.lx, sizeof(T));
-    const WriteBack wb = split ? kWriteNone : requant ? kWriteCoarseBox : kWriteAll;
+    // a plane-schedule field whose inverse runs as fused XY levels quantizes
+    // and dequantizes the coefficients as they load: nothing written back
+    const bool plane_requant = !g.dyadic && fusedq && plane_fused_fits(g.d, g.lx, g.lz, sizeof(T));
+    const WriteBack wb = split || plane_requant ? kWriteNone : requant ? kWriteCoarseBox : kWriteAll;
     if (pairs) {
         StageClock c("quantize_encode", s);
         quantize_encode_pairs<T>(p.coeff, g.d, quantum, qo, archive + 32, capacity - 32, wsc, &res->wzp_bytes,
@@ -300,7 +311,7 @@ uint64_t compress(const T* input, Dims d, double abs_bound, uint8_t* archive, ui
         inverse_coarse_requant<T>(p.coeff, p.t0, p.t1, g.d, g.lx, quantum, s);
         inverse_finest_requant<T>(p.coeff, p.t0, g.d, g.lx, emit, s);
     } else if (!transform_inverse<T>(p.coeff, p.rec, p.t0, p.t1, g.d, g.lx, g.lz, g.dyadic, s,
-                                     fsegs ? &emit : nullptr)) {
+                                     fsegs ? &emit : nullptr, plane_requant ? quantum : 0.0)) {
         rp = SegList{p.rbm, p.ridx, p.rerr, p.rcount, rtiles, kRepairTile};
         StageClock c("repairs", s);
         detect_repairs<T>(input, p.rec, n, abs_bound, rp, status, s);

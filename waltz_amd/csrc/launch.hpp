@@ -53,9 +53,12 @@ struct Level0Emit {
 uint64_t patch_table_words(Dims full, uint32_t np, unsigned width);
 // Returns true when `emit` was carried out inside the fused finest level;
 // false means the caller must run the equivalent separate pass on `output`.
+// requant > 0 (plane-schedule fields with plane_fused_fits only): coeff holds
+// the unquantized coefficients, quantized with this quantum and dequantized
+// as the levels load them.
 template <class T>
 bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       hipStream_t s, const Level0Emit<T>* emit = nullptr);
+                       hipStream_t s, const Level0Emit<T>* emit = nullptr, double requant = 0.0);
 // transform_inverse of a dyadic field with a fused finest level, in two
 // parts, so that a caller can order other work between them: the levels
 // above the finest one (they read and write coeff inside the level-1 box
@@ -147,6 +150,26 @@ struct SideList {          // unsorted (index, payload) list + row-major flag bi
     uint32_t* count;       // device counter (zeroed)
     uint32_t capacity;
 };
+
+// Appends this lane's q-outlier (row-major index g, magnitude m) when `over`,
+// with one counter atomic per wave for all of the wave's entries (thousands
+// of entries on one counter otherwise serialise); the list's order is free,
+// the archive orders it by the flag bitmap. Called by every active lane.
+WZ_D void append_qoutlier(const SideList& qo, bool over, uint32_t g, uint32_t m) {
+    const unsigned long long mask = __ballot(over);
+    if (!mask) return;
+    const uint32_t lane = __lane_id(), leader = uint32_t(__ffsll((long long)mask)) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(qo.count, uint32_t(__popcll(mask)));
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), int(leader)));  // leader: wave-uniform
+    if (!over) return;
+    atomicOr(&qo.bitmap[g >> 5], 1u << (g & 31u));
+    const uint32_t slot = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (slot < qo.capacity) {
+        qo.index[slot] = g;
+        static_cast<uint32_t*>(qo.payload)[slot] = m;
+    }
+}
 
 template <class T>
 void quantize_bricks(T* coeff, const BrickGeometry& g, double quantum, const uint16_t* rank, uint16_t* mag,

@@ -286,6 +286,11 @@ struct XYInverse {
     T* out;  // the level's reconstruction (natural layout)
     uint64_t opy, opz;
     Dims n;  // level box: nx x ny, every plane
+    // q > 0: the coefficients hold unquantized values, quantized and
+    // dequantized as they load (the low box too when lo_coeff: the coarsest
+    // level, whose low box is the coefficient volume's)
+    T q, inv;
+    uint32_t lo_coeff;
 };
 
 template <class T>
@@ -312,8 +317,13 @@ __global__ void __launch_bounds__(kXYThreads) xy_inverse_kernel(XYInverse<T> a) 
         for (int j = 0; j < P; ++j) {
             const int32_t q = int32_t(start) - 4 + 2 * j;
             const uint32_t ce = split_pos(q, nx), co = split_pos(q + 1, nx);
-            e[j] = (lowrow && ce < hx) ? lrow[ce] : hrow[ce];
-            o[j] = (lowrow && co < hx) ? lrow[co] : hrow[co];
+            const bool le = lowrow && ce < hx, lo_ = lowrow && co < hx;
+            e[j] = le ? lrow[ce] : hrow[ce];
+            o[j] = lo_ ? lrow[co] : hrow[co];
+            if (a.q > T(0)) {  // (wave-uniform)
+                if (!le || a.lo_coeff) e[j] = requantize<T>(e[j], a.q, a.inv);
+                if (!lo_ || a.lo_coeff) o[j] = requantize<T>(o[j], a.q, a.inv);
+            }
         }
         lift_inverse<T, P, false>(e, o);
 #pragma unroll
@@ -834,7 +844,9 @@ namespace {
 // field's level 0 (a coarse box run as a field of its own), for stage names.
 template <class T>
 bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                  hipStream_t s, const Level0Emit<T>* emit, const Tuning& tu, unsigned base) {
+                  hipStream_t s, const Level0Emit<T>* emit, const Tuning& tu, unsigned base, double requant = 0.0) {
+    if (requant > 0 && (dyadic || !plane_fused_fits(full, lxy, lz, sizeof(T))))
+        throw std::logic_error("requantizing inverse without fused XY levels");
     Passes<T, true> p{coeff, {tmp0, tmp1}, coeff, output == coeff ? nullptr : output, nullptr, s, tu};
     p.base = base;
     if (lxy == 0 && lz == 0) {
@@ -876,7 +888,8 @@ bool inverse_impl(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy
         uint64_t lpy = full.nx, lpz = uint64_t(full.nx) * full.ny;
         for (int l = int(lxy) - 1; l >= 0; --l) {
             StageClock clock(names[l + base], s);
-            XYInverse<T> a{coeff, full.nx, uint64_t(full.nx) * full.ny, lo, lpy, lpz, nullptr, 0, 0, xy[l]};
+            XYInverse<T> a{coeff, full.nx, uint64_t(full.nx) * full.ny, lo, lpy, lpz, nullptr, 0, 0, xy[l],
+                           T(requant), requant > 0 ? T(1.0 / requant) : T(0), uint32_t(lo == coeff)};
             if (l == 0) {
                 a.out = lz ? tmp1 : output;
                 a.opy = full.nx;
@@ -913,9 +926,9 @@ uint64_t volume(Dims d) { return uint64_t(d.nx) * d.ny * d.nz; }
 
 template <class T>
 bool transform_inverse(T* coeff, T* output, T* tmp0, T* tmp1, Dims full, unsigned lxy, unsigned lz, bool dyadic,
-                       hipStream_t s, const Level0Emit<T>* emit) {
+                       hipStream_t s, const Level0Emit<T>* emit, double requant) {
     return inverse_impl<T>(coeff, output, tmp0, tmp1, full, lxy, lz, dyadic, s, emit,
-                           tuning_for(full, lxy, sizeof(T), true), 0);
+                           tuning_for(full, lxy, sizeof(T), true), 0, requant);
 }
 
 template <class T>
@@ -1029,9 +1042,9 @@ template void transform_forward<float>(const float*, float*, float*, float*, Dim
 template void transform_forward<double>(const double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
                                         uint32_t*, hipStream_t);
 template bool transform_inverse<float>(float*, float*, float*, float*, Dims, unsigned, unsigned, bool, hipStream_t,
-                                       const Level0Emit<float>*);
+                                       const Level0Emit<float>*, double);
 template bool transform_inverse<double>(double*, double*, double*, double*, Dims, unsigned, unsigned, bool,
-                                        hipStream_t, const Level0Emit<double>*);
+                                        hipStream_t, const Level0Emit<double>*, double);
 template void inverse_coarse_requant<float>(const float*, float*, float*, Dims, unsigned, double, hipStream_t);
 template void inverse_coarse_requant<double>(const double*, double*, double*, Dims, unsigned, double, hipStream_t);
 template void inverse_finest_requant<float>(const float*, const float*, Dims, unsigned, const Level0Emit<float>&,

@@ -99,18 +99,10 @@ struct BrickSource {
                 sw |= uint32_t(rounded < T(0)) << i;
                 over |= uint32_t(mv[dx] > 65535u) << dx;
             }
-            if (over) {  // q-outliers (rare): one branch per row
+            if (__ballot(over != 0)) {  // q-outliers (rare): one wave-uniform branch per row
 #pragma unroll
-                for (uint32_t dx = 0; dx < 4; ++dx) {
-                    if (!((over >> dx) & 1u)) continue;
-                    const uint64_t g = at + dx;
-                    atomicOr(&qo.bitmap[g >> 5], 1u << (g & 31u));
-                    const uint32_t slot = atomicAdd(qo.count, 1u);
-                    if (slot < qo.capacity) {
-                        qo.index[slot] = uint32_t(g);
-                        static_cast<uint32_t*>(qo.payload)[slot] = mv[dx];
-                    }
-                }
+                for (uint32_t dx = 0; dx < 4; ++dx)
+                    append_qoutlier(qo, (over >> dx) & 1u, uint32_t(at + dx), mv[dx]);
             }
             if (X < hx && Y + dy < hy && Z + dz < hz) {  // dequantized write-back inside the box
                 T d[4];
@@ -166,6 +158,7 @@ struct RankSource {
                 const RankQuad& r = rq[u];
                 T d[4];
                 bool all = true;
+                uint32_t mvs[4], over = 0;
 #pragma unroll
                 for (uint32_t k = 0; k < 4; ++k) {
                     const bool in = r.j[k] < E;
@@ -177,20 +170,17 @@ struct RankSource {
                     const T rounded = rint_t(scaled);
                     const T am = abs_t(rounded);
                     const uint32_t mv = am >= T(2147483648.0) ? uint32_t(INT32_MAX) : uint32_t(int32_t(am));
+                    mvs[k] = mv;
                     d[k] = requantize<T>(v[u][k], q, inv);
                     if (!in) continue;
+                    over |= uint32_t(mv > 65535u) << k;
                     bad = bad || !(abs_t(scaled) < T(9223372036854775808.0));
                     mg[r.j[k]] = uint16_t(mv > 65535u ? 65535u : mv);
                     if (rounded < T(0)) atomicOr(&sg[r.j[k] >> 5], 1u << (r.j[k] & 31u));
-                    if (mv > 65535u) {  // q-outlier (rare)
-                        const uint32_t gi = r.at + k;
-                        atomicOr(&qo.bitmap[gi >> 5], 1u << (gi & 31u));
-                        const uint32_t slot = atomicAdd(qo.count, 1u);
-                        if (slot < qo.capacity) {
-                            qo.index[slot] = gi;
-                            static_cast<uint32_t*>(qo.payload)[slot] = mv;
-                        }
-                    }
+                }
+                if (__ballot(over != 0)) {  // q-outliers (rare): one wave-uniform branch per quad
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) append_qoutlier(qo, (over >> k) & 1u, r.at + k, mvs[k]);
                 }
                 // dequantized write-back inside the box (a quad never
                 // straddles hx: hx is 0, nx or nx / 2, multiples of 8)
