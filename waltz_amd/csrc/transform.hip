@@ -153,14 +153,17 @@ void axis(const T* src, T* dst, Dims full, Dims act, int axis, const uint32_t* t
     else launch_axis_tile<T, Inverse, 2>(src, dst, full, act, tiles[2], status, s);
 }
 
-// X and Y passes of a box whose planes fit the LDS stage, in one launch.
-// limit (tuned): planes up to 1024 or 4096 samples, 0 = never; by default
-// 32 x 32 (larger planes kept a workgroup busy for longer than two launches of
-// the separate passes took at 64 x 64 on this device).
+template <class T, bool Inverse>
+bool xy_pair(const T* src, T* dst, Dims full, Dims act, int a0, hipStream_t s);
+
+// X and Y passes of a box in one launch: planes up to `limit` samples (tuned:
+// 1024 or 4096, 0 = none) staged whole in LDS by one workgroup, larger ones
+// (forward Y then X, inverse X then Y) as 64 x 32 tiles with their halos.
 template <class T, bool Inverse>
 bool plane_pair(const T* src, T* dst, Dims full, Dims act, int a0, int a1, uint32_t limit, hipStream_t s) {
     const uint64_t cells = uint64_t(act.nx) * act.ny;
-    if (cells > limit || act.nz == 0 || a0 + a1 != 1) return false;
+    if (act.nz == 0 || a0 + a1 != 1) return false;
+    if (cells > limit) return xy_pair<T, Inverse>(src, dst, full, act, a0, s);
     if (cells <= 1024) {
         if (a0 == 0)
             hipLaunchKernelGGL((plane_pair_pass<T, Inverse, 0, 1, 1024>), dim3(act.nz), dim3(256), 0, s, src, dst,
@@ -436,6 +439,22 @@ void xy_inverse(const XYInverse<T>& a, hipStream_t s) {
 template <class T>
 void xy_forward(const XYForward<T>& a, hipStream_t s) {
     hipLaunchKernelGGL((xy_forward_kernel<T>), dim3(xy_blocks(a.n)), dim3(kXYThreads), 0, s, a);
+}
+
+// The X and Y passes of box `act` (in full-pitched arrays, src != dst) as one
+// tiled launch, in the order the tiled kernels lift (forward Y then X,
+// inverse X then Y); false for the other order.
+template <class T, bool Inverse>
+bool xy_pair(const T* src, T* dst, Dims full, Dims act, int a0, hipStream_t s) {
+    const uint64_t py = full.nx, pz = uint64_t(full.nx) * full.ny;
+    if constexpr (Inverse) {
+        if (a0 != 0) return false;
+        xy_inverse<T>(XYInverse<T>{src, py, pz, src, py, pz, dst, py, pz, act, T(0), T(0), 0u}, s);
+    } else {
+        if (a0 != 1) return false;
+        xy_forward<T>(XYForward<T>{src, py, pz, dst, py, pz, dst, py, pz, act, nullptr}, s);
+    }
+    return true;
 }
 
 // Compact XY boxes of the plane schedule's levels >= 1 (every plane) in
