@@ -395,9 +395,15 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     // Waves then read from one base pointer (wave-uniform) with 32-bit byte
     // offsets; only the one wave that straddles the two groups selects per lane.
     constexpr uint32_t kLow = (FW / 2) * (FH / 2);
+    // A position reads the LLL box exactly when its row and column are even
+    // (a reflection keeps the parity, and even positions split into the low
+    // band): slot order puts them first, so slot 0 is all low, slot 1 below
+    // thread kLow - IT, the later slots never. Compile-time per slot, so no
+    // per-slot lane masks stay live through the steps.
+    static_assert(kLow >= uint32_t(IT) && kLow < 2u * uint32_t(IT), "slot classes");
+    auto slot_low = [&](int s) -> bool { return s == 0 || (s == 1 && t < kLow - uint32_t(IT)); };
     uint32_t eoff[ISLOTS], ooff[ISLOTS];   // even-plane and odd-plane byte offsets
     uint32_t zpk[(ISLOTS + 1) / 2] = {};     // Zp offsets, two 16-bit halves per word
-    uint32_t low_lanes = 0, lo_wave = 0, hi_wave = 0;   // per slot: lane / whole-wave classes
 #pragma unroll
     for (int s = 0; s < ISLOTS; ++s) {
         const uint32_t idx = t + IT * s < uint32_t(FW * FH) ? t + IT * s : t + IT * (s - 1);
@@ -412,15 +418,11 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
         }
         const uint32_t cy = split_pos(int32_t(y0) - 4 + int32_t(r), n.ny);
         const uint32_t cx = split_pos(int32_t(x0) - 4 + int32_t(c), n.nx);
-        const bool low = cx < hx && cy < hy;
+        const bool low = idx < kLow;  // = cx < hx && cy < hy
         const uint32_t ohi = (cy * uint32_t(a.hpy) + cx) * uint32_t(sizeof(T));
         eoff[s] = low ? (cy * uint32_t(a.lpy) + cx) * uint32_t(sizeof(T)) : ohi;
         ooff[s] = ohi;
         zpk[s / 2] |= (r * PA + c) << (16 * (s & 1));
-        const unsigned long long lw = __ballot(low);
-        low_lanes |= uint32_t(low) << s;
-        lo_wave |= uint32_t(lw == ~0ull) << s;
-        hi_wave |= uint32_t(lw == 0ull) << s;
     }
     auto at = [](const T* base, uint32_t off) {
         return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
@@ -436,7 +438,7 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
         return PairSrc{a.hi + uint64_t(ze) * a.hpz, a.hi + uint64_t(zo) * a.hpz, a.lo + uint64_t(ze) * a.lpz};
     };
     auto load_slot = [&](const PairSrc& q, int s, T& ev, T& ov) {
-        ev = at(((low_lanes >> s) & 1u) ? q.le : q.he, eoff[s]);  // one load on every path
+        ev = at(slot_low(s) ? q.le : q.he, eoff[s]);  // one load on every path
         ov = buf_load<T>(window(q.ho), ooff[s]);
     };
     // Values of slot s as the Z synthesis takes them (at use, so the loads
@@ -448,9 +450,7 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
     };
     auto even_band = [&](int s, T v) {
         if constexpr (!Requant) return v;
-        else if ((lo_wave >> s) & 1u) return v;
-        else if ((hi_wave >> s) & 1u) return band(v);
-        else return ((low_lanes >> s) & 1u) ? v : band(v);
+        else return slot_low(s) ? v : band(v);
     };
     // Y item of this thread (t < 512): plane yp, column yx, y tile yt; a wave
     // is one (plane, tile) and its lanes are 64 consecutive x
