@@ -179,10 +179,12 @@ WZ_HD WzpShape wzp_shape(uint32_t n) {
 
 // Division by a divisor that stays fixed over a loop, by one multiply:
 // floor(n / d) = (n * ceil(2^32 / d)) >> 32, exact while n * d < 2^32.
+// ceil(2^32 / d) = floor((2^32 - 1) / d) + 1 for every d >= 1, a 32-bit
+// division (a 64-bit one is a long software routine on the GPU).
 struct FastDiv {
     uint32_t d;
     uint64_t m;
-    WZ_HD explicit FastDiv(uint32_t divisor) : d(divisor), m(((uint64_t(1) << 32) + divisor - 1) / divisor) {}
+    WZ_HD explicit FastDiv(uint32_t divisor) : d(divisor), m(uint64_t(0xFFFFFFFFu / divisor) + 1u) {}
     WZ_HD uint32_t div(uint32_t n) const { return uint32_t((uint64_t(n) * m) >> 32); }
     WZ_HD uint32_t mod(uint32_t n) const { return n - div(n) * d; }
 };
