@@ -692,15 +692,20 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
             // its records take one LDS reservation
             const uint32_t lane = t & 63u;
             const T nb = T(a.bound);
+            // Wave masks: ballots of single compares combined in scalar
+            // registers (a ballot of a combined predicate costs a vector
+            // select and compare); lane predicates come back from them.
+            const unsigned long long xz_m = __ballot(xz);  // lanes inside the field
+            // rows of the Y item inside the field; the empty asm keeps the
+            // eight row tests in the step (hoisted, they were loop-invariant
+            // scalars spilled to vector lanes and restored every step)
+            uint32_t rows_in = __builtin_amdgcn_readfirstlane(n.ny > y0 + 8 * yt ? n.ny - (y0 + 8 * yt) : 0u);
+            asm volatile("" : "+s"(rows_in));
             bool ok = true;
             unsigned long long m8[8];  // wave-uniform patch masks of the eight rows
-            uint32_t pbits = 0;        // this lane's patched rows
-            T err8[8];
+            T err8[8];                 // read only where the sample is patched
             auto decide = [&](uint32_t j, T v, T orig) {
-                const uint32_t Y = y0 + 8 * yt + j;
-                const bool inb = xz && Y < n.ny;
-                bool patch = false;
-                T err = T(0);
+                const unsigned long long in_m = j < rows_in ? xz_m : 0ull;
                 if constexpr (sizeof(T) == 4) {
                     // the same float operations as repair_decision; its exact
                     // bound test is settled by the rounded difference d of
@@ -716,22 +721,27 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
                     const bool p = abs_t(e) > nb;
                     const T d = p ? (v - e) - orig : e;
                     const bool fast = abs_t(d) < T(a.bound_lo);
-                    patch = inb && p;
-                    err = inb ? e : T(0);
-                    if (__ballot(inb && !fast)) {
+                    unsigned long long pm = __ballot(p) & in_m;
+                    T err = e;
+                    if (in_m & ~__ballot(fast)) {
+                        const bool inb = __builtin_amdgcn_inverse_ballot_w64(in_m);
                         bool okj = true;
                         T ej = T(0);
                         const bool pj = repair_decision<T>(v, orig, nb, a.bound, &ej, &okj);
-                        patch = inb && pj;
-                        err = inb ? ej : T(0);
+                        pm = __ballot(pj) & in_m;
+                        err = ej;
                         ok = ok && (!inb || okj);
                     }
-                } else if (inb) {
-                    patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
+                    m8[j] = pm;
+                    err8[j] = err;
+                } else {
+                    const bool inb = __builtin_amdgcn_inverse_ballot_w64(in_m);
+                    bool patch = false;
+                    T err = T(0);
+                    if (inb) patch = repair_decision<T>(v, orig, nb, a.bound, &err, &ok);
+                    m8[j] = __ballot(patch);
+                    err8[j] = err;
                 }
-                m8[j] = __ballot(patch);
-                pbits |= uint32_t(patch) << j;
-                err8[j] = err;
             };
             inv_lift_stream<T, 8, true>(ld, [&](int j, T e, T o) {
                 decide(2 * (j - 2), e, ref[2 * (j - 2)]);
@@ -742,7 +752,7 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
             if constexpr (!RowWords) {  // bitmap words straddle rows: one atomic per patched sample
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
-                    if ((pbits >> r) & 1u) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(m8[r])) {
                         const uint32_t at = row0 + uint32_t(r) * n.nx;
                         atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
                     }
@@ -773,7 +783,7 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
                 // entries of lower lanes: mbcnt counts the mask bits below this lane
                 const uint32_t k = before + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
                                                                       __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-                const bool mine = (pbits >> r) & 1u;
+                const bool mine = __builtin_amdgcn_inverse_ballot_w64(m);
                 __builtin_amdgcn_raw_buffer_store_b32(row0 + uint32_t(r) * n.nx, seg_index, int(mine ? 4 * k : kOutside), 0, 0);
                 buf_store(err8[r], seg_payload, mine ? uint32_t(sizeof(T)) * k : kOutside);
                 before += uint32_t(__popcll(m));

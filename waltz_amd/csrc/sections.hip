@@ -54,14 +54,22 @@ __global__ void __launch_bounds__(1024) scan_layout_kernel(uint8_t* __restrict__
     scan_layout<T>(archive, capacity, n, rbsum, qbsum, blocks, np_dev, qo, flags, lx, lz, bz, quantum, res);
 }
 
-// Both side channels to their index-order slots in one launch: workgroup
+// Both side channels to their index-order slots in one launch: item
 // g < segments * kPlaceSlices places repairs of segment g / kPlaceSlices,
-// the remaining kPlaceQBlocks place q-outliers.
+// the remaining kPlaceQBlocks place q-outliers. Items map to workgroups
+// XCD-aware: the dispatcher deals workgroups round-robin to the 8 XCDs, and
+// XCD k takes the k-th contiguous range of items instead, so the slices of
+// one segment and the neighbouring segments whose entries share output rows
+// (segments follow the finest level's x-fastest tile order) write those
+// lines, and read those bitmap words, through one L2. Grid = 8 * ceil(items
+// / 8); spare workgroups return at once.
 template <class T>
 __global__ void __launch_bounds__(256) place_kernel(uint8_t* __restrict__ archive, SegList rp, Ranks rr, SideList qo,
                                                     Ranks qr, const EncodeResult* __restrict__ res) {
     if (res->status) return;
-    const uint32_t g = blockIdx.x, rblocks = rp.segments * kPlaceSlices;
+    const uint32_t rblocks = rp.segments * kPlaceSlices, per = gridDim.x / 8;
+    const uint32_t g = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (g >= rblocks + kPlaceQBlocks) return;
     if (g < rblocks) place_repairs<T>(archive, rp, rr, res, g / kPlaceSlices, g % kPlaceSlices);
     else place_qoutliers(archive, qo, qr, uint64_t(sizeof(T)), res, g - rblocks);
 }
@@ -91,7 +99,8 @@ void finish_archive(uint8_t* archive, uint64_t capacity, uint32_t n, SegList rep
     hipLaunchKernelGGL((scan_layout_kernel<T>), dim3(1), dim3(1024), 0, s, archive, capacity, n,
                        const_cast<uint32_t*>(rr.bsum), const_cast<uint32_t*>(qr.bsum), blocks, np_total, qout, flags,
                        lx, lz, bz, quantum, result);
-    hipLaunchKernelGGL((place_kernel<T>), dim3(repairs.segments * kPlaceSlices + kPlaceQBlocks), dim3(256), 0, s,
+    const uint32_t items = repairs.segments * kPlaceSlices + kPlaceQBlocks;
+    hipLaunchKernelGGL((place_kernel<T>), dim3(8 * ((items + 7) / 8)), dim3(256), 0, s,
                        archive, repairs, rr, qout, qr, result);
 }
 
