@@ -24,6 +24,7 @@
 #include "launch.hpp"
 
 #include <type_traits>
+#include <utility>
 
 namespace wz {
 namespace {
@@ -167,6 +168,22 @@ WZ_D void inv_lift_stream(Ld&& ld, Out&& out) {
         po0 = o0;
         pe1 = e1;
     }
+}
+
+// Lane L of v set to the scalar s (v_writelane: one vector instruction, no
+// lane-index compare; the lane is an immediate).
+template <int L>
+WZ_D uint32_t write_lane(uint32_t v, uint32_t s) {
+    static_assert(L >= 0 && L < 64, "lane");
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(v) : "s"(s), "n"(L));
+    return v;
+}
+// Lane 2r + h of the result: word h of the 64-bit wave mask m[r].
+template <int... R>
+WZ_D uint32_t bitmap_words(const unsigned long long (&m)[8], std::integer_sequence<int, R...>) {
+    uint32_t v = 0;
+    ((v = write_lane<2 * R>(v, uint32_t(m[R])), v = write_lane<2 * R + 1>(v, uint32_t(m[R] >> 32))), ...);
+    return v;
 }
 
 // Four consecutive staged values to a 16-byte aligned destination.
@@ -757,12 +774,13 @@ __global__ void __launch_bounds__(IT) inv_level_kernel(InvLevel<T> a) {
                         atomicOr(&a.rep.bitmap[at >> 5], 1u << (at & 31u));
                     }
             } else {  // 64 consecutive x at a 32-aligned row start: each row owns two words
-                uint32_t wv = 0;  // lane 2r + h holds word h of row r
-#pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    if ((lane >> 1) == uint32_t(r)) wv = (lane & 1u) ? uint32_t(m8[r] >> 32) : uint32_t(m8[r]);
-                const uint32_t Y = y0 + 8 * yt + (lane >> 1);
-                const bool put = lane < 16 && Z < n.nz && Y < n.ny && ((lane & 1u) == 0 || x0 + 32 < n.nx);
+                // lane 2r + h holds word h of row r, written from the masks
+                const uint32_t wv = bitmap_words(m8, std::make_integer_sequence<int, 8>{});
+                // lanes 2r + h that store: rows inside the field, and the
+                // second word only where the row tile reaches x0 + 32
+                const uint64_t rows_m = rows_in >= 8 ? 0xFFFFull : (1ull << (2 * rows_in)) - 1ull;
+                const uint64_t put_m = Z < n.nz ? rows_m & (x0 + 32 < n.nx ? 0xFFFFull : 0x5555ull) : 0ull;
+                const bool put = __builtin_amdgcn_inverse_ballot_w64(put_m);
                 const uint32_t words = uint32_t((uint64_t(n.nx) * n.ny * n.nz + 31) / 32);
                 __builtin_amdgcn_raw_buffer_store_b32(
                     wv, window_n(a.rep.bitmap, 4 * words),
