@@ -14,8 +14,9 @@
 // the other parity), and fold every extended position back onto the sample it
 // reflects to. The inverse pass is the mirror image (split input, natural
 // output). The support of L^T is 4 samples wide on each side, so a line chunk
-// of 64 outputs is computed in LDS from a zero-extended window with 16
-// samples of margin.
+// of 64 outputs is computed from a zero-extended window with 16 samples of
+// margin staged in LDS, one 8-sample tile per thread in registers (from the
+// tile's 16-sample neighbourhood), with no barrier between the updates.
 //
 // The adjoint of a multi-level transform runs the transposed passes in the
 // reverse order of the transform's schedule.
@@ -28,7 +29,6 @@ namespace {
 constexpr int kChunkOut = 64;                  // outputs per line chunk
 constexpr int kMargin = 16;                    // window margin on each side
 constexpr int kWin = kChunkOut + 2 * kMargin;  // 96 window samples
-constexpr int kPairs = kWin / 2;
 
 WZ_HD uint32_t split_index(uint32_t p, uint32_t n) { return (p >> 1) + ((p & 1u) ? half_len(n) : 0u); }
 
@@ -36,19 +36,45 @@ WZ_HD uint32_t split_index(uint32_t p, uint32_t n) { return (p >> 1) + ((p & 1u)
 // 32 consecutive x, so every global access is a 128-byte (f32) row segment.
 constexpr int kLines = 32;
 
-// One update step on the LDS window: for every pair j of every line,
-// OddFromEven: o_j += c (e_j + e_{j+1}), else e_j += c (o_{j-1} + o_j).
-template <class T, int L, bool OddFromEven>
-WZ_D void step(T (*v)[kWin + 1], T c) {
-    for (uint32_t w = threadIdx.x; w < uint32_t(L * kPairs); w += 256) {
-        const uint32_t l = w % L, j = w / L;
-        if (OddFromEven) {
-            if (j + 1 < kPairs) v[l][2 * j + 1] = fmadd(c, v[l][2 * j] + v[l][2 * j + 2], v[l][2 * j + 1]);
-        } else {
-            if (j >= 1) v[l][2 * j] = fmadd(c, v[l][2 * j - 1] + v[l][2 * j + 1], v[l][2 * j]);
-        }
+// Transposed lifting of one 8-sample tile (window positions [i0, i0 + 8))
+// from its 16-sample neighbourhood [i0 - 4, i0 + 12) in registers: the four
+// updates, each "x[k] += c (x[k-1] + x[k+1])" on one parity, lose one pair of
+// validity at the tile ends, so pairs 2..5 of the 8 loaded are final. Every
+// value is the same sequence of operations as the same update run over the
+// whole window.
+template <class T, bool OfInverse>
+WZ_D void tile_transposed(const T* w, T* out) {
+    using C = Lift<T>;
+    T e[8], o[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        e[j] = w[2 * j];
+        o[j] = w[2 * j + 1];
     }
-    __syncthreads();
+    if constexpr (OfInverse) {  // transposes of A', B', G', D' in that order
+#pragma unroll
+        for (int j = 1; j < 8; ++j) e[j] = fmadd(-C::a, o[j - 1] + o[j], e[j]);
+#pragma unroll
+        for (int j = 1; j < 7; ++j) o[j] = fmadd(-C::b, e[j] + e[j + 1], o[j]);
+#pragma unroll
+        for (int j = 2; j < 7; ++j) e[j] = fmadd(-C::g, o[j - 1] + o[j], e[j]);
+#pragma unroll
+        for (int j = 2; j < 6; ++j) o[j] = fmadd(-C::d, e[j] + e[j + 1], o[j]);
+    } else {  // transposes of D, G, B, A in that order (after the scaling)
+#pragma unroll
+        for (int j = 0; j < 7; ++j) o[j] = fmadd(C::d, e[j] + e[j + 1], o[j]);
+#pragma unroll
+        for (int j = 1; j < 7; ++j) e[j] = fmadd(C::g, o[j - 1] + o[j], e[j]);
+#pragma unroll
+        for (int j = 1; j < 6; ++j) o[j] = fmadd(C::b, e[j] + e[j + 1], o[j]);
+#pragma unroll
+        for (int j = 2; j < 6; ++j) e[j] = fmadd(C::a, o[j - 1] + o[j], e[j]);
+    }
+#pragma unroll
+    for (int j = 2; j < 6; ++j) {
+        out[2 * (j - 2)] = e[j];
+        out[2 * (j - 2) + 1] = o[j];
+    }
 }
 
 // OfInverse: transpose of the inverse pass (natural -> split), else of the
@@ -58,6 +84,10 @@ __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T
                                                     Dims act, uint32_t chunks, uint32_t groups) {
     constexpr int L = kLines;
     __shared__ T v[L][kWin + 1];
+    // transposed-lifting results at positions [c0 - 8, c0 + 72): the chunk
+    // and the extended positions the fold reads
+    constexpr int kRes = kChunkOut + 16;
+    __shared__ T r[L][kRes + 1];
     __shared__ uint64_t lbase[L];
     using C = Lift<T>;
     const uint32_t n = Axis == 0 ? act.nx : Axis == 1 ? act.ny : act.nz;
@@ -98,17 +128,18 @@ __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T
         v[l][i] = x;
     }
     __syncthreads();
-    if constexpr (OfInverse) {  // transposes of A', B', G', D' in that order, then 1/K
-        step<T, L, false>(v, -C::a);
-        step<T, L, true>(v, -C::b);
-        step<T, L, false>(v, -C::g);
-        step<T, L, true>(v, -C::d);
-    } else {  // transposes of D, G, B, A in that order (after the scaling)
-        step<T, L, true>(v, C::d);
-        step<T, L, false>(v, C::g);
-        step<T, L, true>(v, C::b);
-        step<T, L, false>(v, C::a);
-    }
+    // the chunk's 8 tiles of every line, one per thread; the tile before it
+    // (images -p, p = 1..4, at the line start) and the one after it (images
+    // 2n - 2 - p at the line end) only where those exist
+    static_assert(L * (kChunkOut / 8) == 256, "one chunk tile per thread");
+    auto tile = [&](uint32_t l, int k) {  // window positions [kMargin + 8k, +8)
+        tile_transposed<T, OfInverse>(&v[l][kMargin + 8 * k - 4], &r[l][8 + 8 * k]);
+    };
+    tile(threadIdx.x % L, int(threadIdx.x / L));
+    if (c0 == 0 && threadIdx.x < uint32_t(L)) tile(threadIdx.x, -1);
+    if (c0 + kChunkOut + 8 > n && threadIdx.x >= uint32_t(L) && threadIdx.x < uint32_t(2 * L))
+        tile(threadIdx.x - L, kChunkOut / 8);
+    __syncthreads();
 
     // fold the extended positions onto the samples they reflect to
     const int32_t lo = -4, hi = int32_t(n) + 4;  // support of the transposed lifting
@@ -117,7 +148,7 @@ __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T
         const uint32_t p = c0 + i;
         const uint64_t base = lbase[l];
         if (base == ~uint64_t(0) || p >= n) continue;
-        auto u = [&](int32_t q) { return v[l][q - int32_t(c0) + kMargin]; };
+        auto u = [&](int32_t q) { return r[l][q - int32_t(c0) + 8]; };
         T s = u(int32_t(p));
         if (n >= 9) {  // at most two more images: -p and 2n-2-p
             if (p >= 1 && p <= 4) s += u(-int32_t(p));
