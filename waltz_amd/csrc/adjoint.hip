@@ -42,14 +42,15 @@ constexpr int kLines = 32;
 // validity at the tile ends, so pairs 2..5 of the 8 loaded are final. Every
 // value is the same sequence of operations as the same update run over the
 // whole window.
+// (`ws`, `os`: element strides of the neighbourhood and the result).
 template <class T, bool OfInverse>
-WZ_D void tile_transposed(const T* w, T* out) {
+WZ_D void tile_transposed(const T* w, T* out, int ws = 1, int os = 1) {
     using C = Lift<T>;
     T e[8], o[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        e[j] = w[2 * j];
-        o[j] = w[2 * j + 1];
+        e[j] = w[(2 * j) * ws];
+        o[j] = w[(2 * j + 1) * ws];
     }
     if constexpr (OfInverse) {  // transposes of A', B', G', D' in that order
 #pragma unroll
@@ -72,8 +73,8 @@ WZ_D void tile_transposed(const T* w, T* out) {
     }
 #pragma unroll
     for (int j = 2; j < 6; ++j) {
-        out[2 * (j - 2)] = e[j];
-        out[2 * (j - 2) + 1] = o[j];
+        out[(2 * (j - 2)) * os] = e[j];
+        out[(2 * (j - 2) + 1) * os] = o[j];
     }
 }
 
@@ -163,6 +164,91 @@ __global__ void __launch_bounds__(256) adjoint_pass(const T* __restrict__ src, T
     }
 }
 
+// The transposes of the inverse's X and Y passes (Y^T, then X^T) of one z
+// plane in one launch: a workgroup owns 64 x 32 outputs (natural positions
+// [x0, x0 + 64) x [y0, y0 + 32)) and stages the zero-extended input window
+// [x0 - 12, x0 + 76) x [y0 - 12, y0 + 44) in LDS. Y^T runs as 8-sample
+// register tiles on all 88 columns of rows [y0 - 8, y0 + 40), its fold gives
+// rows [y0, y0 + 32) (columns outside the box stay zero, as X^T's
+// zero extension needs), X^T then runs on those rows over [x0 - 8, x0 + 72),
+// and its fold writes the outputs at their split positions in both axes.
+// Each value is the same sequence of operations as in the two separate
+// passes. Lines of at least 9 samples (the fold of adjoint_pass's long-line
+// case).
+constexpr int kXYOx = 64, kXYOy = 32;                 // outputs per tile
+constexpr int kXYWx = kXYOx + 24, kXYWy = kXYOy + 24;  // input window
+template <class T>
+__global__ void __launch_bounds__(256) adjoint_xy_kernel(const T* __restrict__ src, T* __restrict__ dst, Dims full,
+                                                         Dims act, uint32_t tx, uint32_t ty) {
+    __shared__ T A[kXYWy][kXYWx + 1];      // window; then the Y-folded rows (rows 0..31)
+    __shared__ T B[kXYOy + 16][kXYWx + 1];  // Y^T results of rows [y0 - 8, y0 + 40); then X^T results
+    using C = Lift<T>;
+    const uint32_t bx = blockIdx.x % tx, rest = blockIdx.x / tx, by = rest % ty, z = rest / ty;
+    const int32_t x0 = int32_t(bx) * kXYOx, y0 = int32_t(by) * kXYOy;
+    const int32_t nx = int32_t(act.nx), ny = int32_t(act.ny);
+    const uint64_t plane = uint64_t(full.nx) * full.ny;
+    const T* sp = src + uint64_t(z) * plane;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t w = t; w < uint32_t(kXYWy * kXYWx); w += 256) {
+        const int32_t i = int32_t(w / kXYWx), j = int32_t(w % kXYWx), y = y0 - 12 + i, x = x0 - 12 + j;
+        A[i][j] = y >= 0 && y < ny && x >= 0 && x < nx ? sp[uint64_t(y) * full.nx + uint32_t(x)] : T(0);
+    }
+    __syncthreads();
+    // Y^T: column j, tile k -> rows [y0 - 8 + 8k, +8) from rows [y0 - 12 + 8k, +16)
+    for (uint32_t w = t; w < uint32_t(6 * kXYWx); w += 256) {
+        const uint32_t j = w % kXYWx, k = w / kXYWx;
+        tile_transposed<T, true>(&A[8 * k][j], &B[8 * k][j], kXYWx + 1, kXYWx + 1);
+    }
+    __syncthreads();
+    for (uint32_t w = t; w < uint32_t(kXYOy * kXYWx); w += 256) {
+        const int32_t i = int32_t(w / kXYWx), j = int32_t(w % kXYWx), p = y0 + i;
+        T v = T(0);
+        if (p < ny) {
+            auto u = [&](int32_t q) { return B[q - y0 + 8][j]; };
+            v = u(p);
+            if (p >= 1 && p <= 4) v += u(-p);
+            const int32_t m = 2 * ny - 2 - p;
+            if (m != p && m < ny + 4) v += u(m);
+            v = (p & 1) ? v * (-C::k) : v * C::ik;
+        }
+        A[i][j] = v;
+    }
+    __syncthreads();
+    // X^T: row i, tile k -> columns [x0 - 8 + 8k, +8) from [x0 - 12 + 8k, +16)
+    for (uint32_t w = t; w < uint32_t(10 * kXYOy); w += 256) {
+        const uint32_t i = w % kXYOy, k = w / kXYOy;
+        tile_transposed<T, true>(&A[i][8 * k], &B[i][8 * k]);
+    }
+    __syncthreads();
+    T* dp = dst + uint64_t(z) * plane;
+    for (uint32_t w = t; w < uint32_t(kXYOy * kXYOx); w += 256) {
+        const int32_t i = int32_t(w / kXYOx), x = x0 + int32_t(w % kXYOx), p = y0 + i;
+        if (x >= nx || p >= ny) continue;
+        auto u = [&](int32_t q) { return B[i][q - x0 + 8]; };
+        T v = u(x);
+        if (x >= 1 && x <= 4) v += u(-x);
+        const int32_t m = 2 * nx - 2 - x;
+        if (m != x && m < nx + 4) v += u(m);
+        v = (x & 1) ? v * (-C::k) : v * C::ik;
+        dp[uint64_t(split_index(uint32_t(p), act.ny)) * full.nx + split_index(uint32_t(x), act.nx)] = v;
+    }
+}
+
+// Only for planes up to 128 x 128: there the launch saved beats the halo
+// (each workgroup stages 2.4x its outputs); on a 256 x 256 plane the two
+// passes took 82 µs and the joint launch 100 µs.
+bool adjoint_xy_fits(Dims act) {
+    return act.nx >= 9 && act.ny >= 9 && act.nz && uint64_t(act.nx) * act.ny <= 128u * 128u;
+}
+
+template <class T>
+void launch_adjoint_xy(const T* src, T* dst, Dims full, Dims act, hipStream_t s) {
+    const uint32_t tx = (act.nx + kXYOx - 1) / kXYOx, ty = (act.ny + kXYOy - 1) / kXYOy;
+    const uint64_t blocks = uint64_t(tx) * ty * act.nz;
+    check_launch_size(blocks * 256);
+    hipLaunchKernelGGL((adjoint_xy_kernel<T>), dim3(uint32_t(blocks)), dim3(256), 0, s, src, dst, full, act, tx, ty);
+}
+
 Dims shrink_box(Dims d, bool xy, bool z) {
     if (xy) {
         d.nx = half_len(d.nx);
@@ -211,6 +297,16 @@ void box_passes(T* home, T* tmp0, T* tmp1, Dims full, Dims act, const int* axes,
                 const T* first = nullptr) {
     const T* src = first ? first : home;
     for (int k = 0; k < count; ++k) {
+        // the transposes of the inverse's X and Y passes, Y^T then X^T: one launch
+        if (OfInverse && k + 1 < count && axes[k] == 1 && axes[k + 1] == 0 && adjoint_xy_fits(act)) {
+            T* dst = k + 1 == count - 1 ? home : ((k + 1) & 1 ? tmp1 : tmp0);
+            if (dst != src) {
+                launch_adjoint_xy<T>(src, dst, full, act, s);
+                src = dst;
+                ++k;
+                continue;
+            }
+        }
         T* dst = k == count - 1 ? home : (k & 1 ? tmp1 : tmp0);
         if (dst == src) dst = tmp0;
         adjoint_axis<T, OfInverse>(src, dst, full, act, axes[k], s);
