@@ -235,7 +235,7 @@ struct Ranks {
 constexpr uint32_t kPlaceSlices = 8;
 template <class T>
 WZ_D void place_repairs(uint8_t* __restrict__ archive, SegList rp, Ranks rank, const EncodeResult* __restrict__ res,
-                        uint32_t k, uint32_t slice) {
+                        uint32_t k, uint32_t slice, uint32_t slices = kPlaceSlices) {
     const uint32_t c = rp.count[k];
     if (!c) return;
     const uint32_t np = res->np;
@@ -244,7 +244,7 @@ WZ_D void place_repairs(uint8_t* __restrict__ archive, SegList rp, Ranks rank, c
     const T* val = static_cast<const T*>(rp.payload) + uint64_t(k) * rp.cap;
     uint32_t* oi = reinterpret_cast<uint32_t*>(archive + L.pi);
     T* ov = reinterpret_cast<T*>(archive + L.pv);
-    for (uint32_t j = slice * 256u + threadIdx.x; j < c; j += 256u * kPlaceSlices) {
+    for (uint32_t j = slice * 256u + threadIdx.x; j < c; j += 256u * slices) {
         const uint32_t i = idx[j];
         const uint32_t pos = rank.of(i);
         oi[pos] = i;
