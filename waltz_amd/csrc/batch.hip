@@ -1437,9 +1437,17 @@ struct DevicePlan {
     uint32_t* flds_pfx = nullptr;   // forward steps x (jobs + 1), LDS kernel
     EncodeResult* last_res = nullptr;  // results of the last compress (device, in its workspace)
     uint16_t* ranks = nullptr;         // kRankSet entries per distinct geometry
+    // compress: the WZP group pass runs on `side` (default priority) beside
+    // the inverse passes, between the two events
+    hipStream_t side = nullptr;
+    hipEvent_t pack_ready = nullptr, pack_done = nullptr;
     std::string error;
 
     ~DevicePlan() {
+        if (side) (void)hipStreamSynchronize(side);
+        if (pack_ready) (void)hipEventDestroy(pack_ready);
+        if (pack_done) (void)hipEventDestroy(pack_done);
+        if (side) (void)hipStreamDestroy(side);
         if (mem) (void)hipFree(mem);
         if (ranks) (void)hipFree(ranks);
         if (host_status) (void)hipHostFree(host_status);
@@ -1466,6 +1474,9 @@ struct DevicePlan {
                      o_fpp = carve(4 * (J + 1) * FS), o_lp = carve(4 * (J + 1) * S), o_flp = carve(4 * (J + 1) * FS);
         hip_check(hipMalloc(reinterpret_cast<void**>(&mem), used), "batch plan allocation");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_status), 4), "batch status");
+        hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "batch side stream");
+        hip_check(hipEventCreateWithFlags(&pack_ready, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&pack_done, hipEventDisableTiming), "event");
         shape = reinterpret_cast<JobShape*>(mem + o_shape);
         run = reinterpret_cast<JobRun*>(mem + o_run);
         chunk_pfx = reinterpret_cast<uint32_t*>(mem + o_cp);
@@ -1603,9 +1614,17 @@ struct DevicePlan {
         if (p.rank_pfx[J])
             hipLaunchKernelGGL((batch_encode_ranks_kernel<T>), dim3(p.rank_pfx[J]), dim3(256), 0, s, shape, comp,
                                rank_pfx, J, ranks, quantum, b);
-        if (p.group_pfx[J])
-            hipLaunchKernelGGL(batch_pack_kernel, dim3(p.group_pfx[J]), dim3(kPackThreads), 0, s, shape, comp,
+        // the group pass beside the inverse passes and the repair pass (none of
+        // them reads or writes its staging slots, descriptors or archive
+        // payload); the layout waits for it
+        const bool pack = p.group_pfx[J] != 0;
+        if (pack) {
+            hip_check(hipEventRecord(pack_ready, s), "event record");
+            hip_check(hipStreamWaitEvent(side, pack_ready, 0), "stream wait");
+            hipLaunchKernelGGL(batch_pack_kernel, dim3(p.group_pfx[J]), dim3(kPackThreads), 0, side, shape, comp,
                                group_pfx, J, b);
+            hip_check(hipEventRecord(pack_done, side), "event record");
+        }
         const Banks bk{b.home, b.tmp0, b.tmp1, b.recon};
         const StageBanks isb{nullptr, b.home, b.tmp0, b.tmp1, b.recon, nullptr};
         for (size_t st = 0; st < p.step.size(); ++st) {
@@ -1622,6 +1641,7 @@ struct DevicePlan {
         if (p.pblock_pfx[J])
             hipLaunchKernelGGL(batch_prefix_kernel, dim3(2 * p.pblock_pfx[J]), dim3(256), 0, s, shape, comp,
                                pblock_pfx, J, b);
+        if (pack) hip_check(hipStreamWaitEvent(s, pack_done, 0), "stream wait");
         hipLaunchKernelGGL((batch_layout_kernel<T>), dim3(J), dim3(1024), 0, s, shape, comp, quantum, b);
         hipLaunchKernelGGL((batch_place_kernel<T>), dim3(p.place_pfx[J]), dim3(256), 0, s, shape, comp, place_pfx,
                            J, b);
